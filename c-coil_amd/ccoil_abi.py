@@ -1,0 +1,185 @@
+"""ctypes front end of the C ABI in include/ccoil_hip.h (libccoil_hip.so), for numpy and torch callers.
+
+This is the thin Python binding a maintainer would add next to the reference's own bindings: it passes
+plain pointers and sizes across the boundary and never computes anything itself. Loading fails loudly
+(OSError) when libccoil_hip.so has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libccoil_hip.so")
+
+OK, ERR_ARGUMENT, ERR_OUT_OF_MEMORY, ERR_DEVICE, ERR_NO_DEVICE = 0, -1, -2, -3, -4
+A, B, E, G, G_JACOBIAN = 1, 2, 4, 8, 16
+FP64, FP32C, FP32, FP64_REFERENCE = 0, 1, 2, 3
+DEVICE_POINTERS, NO_NEAR_FP64, FORCE_REFERENCE_SCALE = 1, 2, 4
+PAIR_M, PAIR_FT = 1, 2
+
+EXPORTED = ["ccoil_create", "ccoil_destroy", "ccoil_last_error", "ccoil_version", "ccoil_set_stream",
+            "ccoil_synchronize", "ccoil_gauss_legendre", "ccoil_fields", "ccoil_pairs"]
+
+
+class Coils(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64)] + [(k, ctypes.POINTER(ctypes.c_int32)) for k in
+                                          ("use_fast", "n_ang", "n_thk", "n_len", "blocks")] + \
+               [(k, ctypes.POINTER(ctypes.c_double)) for k in
+                ("R", "a", "b", "J", "current", "turns", "frequency", "pos", "alpha", "beta")]
+
+
+class Report(ctypes.Structure):
+    _fields_ = [("node_evaluations", ctypes.c_int64), ("near_pairs", ctypes.c_int64), ("kernel_ms", ctypes.c_double),
+                ("block_size", ctypes.c_int32), ("split", ctypes.c_int32)]
+
+
+class CCoilError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"ccoil status {status}: {msg}")
+        self.status = status
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.ccoil_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+        L.ccoil_destroy.argtypes = [vp]
+        L.ccoil_last_error.argtypes = [vp]
+        L.ccoil_last_error.restype = ctypes.c_char_p
+        L.ccoil_version.restype = ctypes.c_char_p
+        L.ccoil_set_stream.argtypes = [vp, vp]
+        L.ccoil_synchronize.argtypes = [vp]
+        L.ccoil_gauss_legendre.argtypes = [vp, ctypes.c_int32, dp, dp]
+        L.ccoil_fields.argtypes = [vp, ctypes.POINTER(Coils), vp, ctypes.c_int64, ctypes.c_uint32, ctypes.c_int32,
+                                   ctypes.c_uint32, vp, vp, vp, vp, ctypes.POINTER(Report)]
+        L.ccoil_pairs.argtypes = [vp, ctypes.POINTER(Coils), ctypes.POINTER(Coils), vp, vp, ctypes.c_int64, vp, vp,
+                                  ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32, vp, vp, vp, ctypes.POINTER(Report)]
+        for f in ("ccoil_create", "ccoil_destroy", "ccoil_set_stream", "ccoil_synchronize", "ccoil_gauss_legendre",
+                  "ccoil_fields", "ccoil_pairs"):
+            getattr(L, f).restype = ctypes.c_int32
+        _lib = L
+    return _lib
+
+
+def version():
+    return lib().ccoil_version().decode()
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data_as(ctypes.c_void_p)
+    return ctypes.c_void_p(int(a.data_ptr()))   # torch tensor on the context's device
+
+
+class CoilSet:
+    """SoA coil description (host memory) as the C ABI takes it. Arrays are kept alive by the object."""
+
+    def __init__(self, use_fast, n_ang, n_thk, R, a, b, J, pos, alpha, beta, n_len=None, blocks=None,
+                 current=None, turns=None, frequency=None):
+        i32 = lambda v: None if v is None else np.ascontiguousarray(v, dtype=np.int32)
+        f64 = lambda v: None if v is None else np.ascontiguousarray(v, dtype=np.float64)
+        self.arrays = dict(use_fast=i32(use_fast), n_ang=i32(n_ang), n_thk=i32(n_thk), n_len=i32(n_len),
+                           blocks=i32(blocks), R=f64(R), a=f64(a), b=f64(b), J=f64(J), current=f64(current),
+                           turns=f64(turns), frequency=f64(frequency), pos=f64(pos), alpha=f64(alpha),
+                           beta=f64(beta))
+        self.n = len(self.arrays["R"])
+        s = Coils()
+        s.n = self.n
+        for k, v in self.arrays.items():
+            if v is None:
+                continue
+            ct = ctypes.c_int32 if v.dtype == np.int32 else ctypes.c_double
+            setattr(s, k, v.ctypes.data_as(ctypes.POINTER(ct)))
+        self.struct = s
+
+    @staticmethod
+    def from_rows(rows):
+        """rows: iterable of dicts with keys use_fast, n_ang, n_thk, n_len, blocks(3), R, a, b, J, current, turns,
+        frequency, pos(3), alpha, beta."""
+        rows = list(rows)
+        col = lambda k, d=None: [r.get(k, d) for r in rows]
+        return CoilSet(col("use_fast"), col("n_ang"), col("n_thk"), col("R"), col("a"), col("b"), col("J"),
+                       np.array(col("pos")).reshape(-1), col("alpha"), col("beta"), n_len=col("n_len", 1),
+                       blocks=np.array(col("blocks", (1, 1, 1))).reshape(-1), current=col("current", 1.0),
+                       turns=col("turns", 1.0), frequency=col("frequency", 0.0))
+
+
+class Context:
+    def __init__(self, device=0):
+        self.h = ctypes.c_void_p()
+        st = lib().ccoil_create(device, ctypes.byref(self.h))
+        if st != OK:
+            raise CCoilError(st, lib().ccoil_last_error(None).decode())
+
+    def close(self):
+        if self.h:
+            lib().ccoil_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != OK:
+            raise CCoilError(st, lib().ccoil_last_error(self.h).decode())
+
+    def set_stream(self, stream_ptr):
+        self._check(lib().ccoil_set_stream(self.h, ctypes.c_void_p(stream_ptr) if stream_ptr else None))
+
+    def synchronize(self):
+        self._check(lib().ccoil_synchronize(self.h))
+
+    def gauss_legendre(self, n):
+        x = np.zeros(n)
+        w = np.zeros(n)
+        dp = ctypes.POINTER(ctypes.c_double)
+        self._check(lib().ccoil_gauss_legendre(self.h, n, x.ctypes.data_as(dp), w.ctypes.data_as(dp)))
+        return x, w
+
+    def fields(self, coils, points, quantities, precision=FP32C, flags=0, out=None):
+        """Host numpy (n,3) points -> dict of numpy outputs; with DEVICE_POINTERS pass torch tensors and `out`."""
+        rep = Report()
+        dev = flags & DEVICE_POINTERS
+        n = points.shape[0]
+        if out is None:
+            out = {}
+            for key, bit, w in (("A", A, 3), ("B", B, 3), ("E", E, 3), ("G", G | G_JACOBIAN, 9)):
+                if quantities & bit:
+                    out[key] = np.zeros((n, w)) if not dev else None
+        if not dev:
+            points = np.ascontiguousarray(points, dtype=np.float64)
+        self._check(lib().ccoil_fields(self.h, ctypes.byref(coils.struct), _ptr(points), n, quantities, precision,
+                                       flags, _ptr(out.get("A")), _ptr(out.get("B")), _ptr(out.get("E")),
+                                       _ptr(out.get("G")), ctypes.byref(rep)))
+        return out, rep
+
+    def pairs(self, prim, sec, n_pairs, what, precision=FP32C, flags=0, prim_index=None, sec_index=None,
+              ring_offset=None, half_ring=None, out=None):
+        rep = Report()
+        if out is None:
+            out = {}
+            if what & PAIR_M:
+                out["M"] = np.zeros(n_pairs)
+            if what & PAIR_FT:
+                out["F"] = np.zeros((n_pairs, 3))
+                out["T"] = np.zeros((n_pairs, 3))
+        pi = None if prim_index is None else np.ascontiguousarray(prim_index, dtype=np.int64)
+        si = None if sec_index is None else np.ascontiguousarray(sec_index, dtype=np.int64)
+        ro = None if ring_offset is None else np.ascontiguousarray(ring_offset, dtype=np.float64)
+        hr = None if half_ring is None else np.ascontiguousarray(half_ring, dtype=np.int32)
+        self._check(lib().ccoil_pairs(self.h, ctypes.byref(prim.struct), ctypes.byref(sec.struct), _ptr(pi), _ptr(si),
+                                      n_pairs, _ptr(ro), _ptr(hr), what, precision, flags, _ptr(out.get("M")),
+                                      _ptr(out.get("F")), _ptr(out.get("T")), ctypes.byref(rep)))
+        return out, rep

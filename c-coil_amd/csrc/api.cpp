@@ -1,0 +1,358 @@
+// C ABI implementation (include/ccoil_hip.h): context, device tables, buffers, field entry point.
+#include "context.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+using namespace ccoil;
+
+namespace {
+thread_local std::string g_create_error;
+}
+
+namespace ccoil {
+
+ccoil_status fail(ccoil_ctx ctx, ccoil_status code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+hipError_t ensure(DevBuf &b, size_t bytes)
+{
+    if (bytes <= b.cap) return hipSuccess;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+        if (e != hipSuccess) return e;
+    }
+    size_t want = bytes + bytes / 4 + 256;
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return e;
+    }
+    b.cap = want;
+    return hipSuccess;
+}
+
+void free_buf(DevBuf &b)
+{
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+// Transformation matrices of Coil::calculateTransformationMatrices (CalculateAttributes.cxx:63-78).
+void rotation(double alpha, double beta, double *T, double *Ti)
+{
+    const double cosY = std::cos(alpha), sinY = std::sin(alpha);
+    const double cosZ = std::cos(beta), sinZ = std::sin(beta);
+    const double t[9] = {cosZ * cosZ * cosY - sinZ * sinZ, -sinZ * cosZ * cosY - sinZ * cosZ, cosZ * sinY,
+                         sinZ * cosZ * cosY + sinZ * cosZ, cosZ * cosZ - sinZ * sinZ * cosY, sinZ * sinY,
+                         -sinY * cosZ, sinY * sinZ, cosY};
+    const double ti[9] = {cosZ * cosZ * cosY - sinZ * sinZ, sinZ * cosZ * cosY + sinZ * cosZ, -cosZ * sinY,
+                          -sinZ * cosZ * cosY - sinZ * cosZ, cosZ * cosZ - sinZ * sinZ * cosY, sinZ * sinY,
+                          sinY * cosZ, sinY * sinZ, cosY};
+    std::memcpy(T, t, sizeof t);
+    std::memcpy(Ti, ti, sizeof ti);
+}
+
+}  // namespace ccoil
+
+namespace ccoil {
+
+// Validates a coil descriptor and packs coil i into the device layout. constFactor follows
+// generateCoilData (SupportFunctions.cxx:169-172): 1e-7 J a pi/2, times b for the slow method.
+std::string pack_coils(const ccoil_coils *c, bool need_len, bool need_freq, std::vector<CoilDev> &out)
+{
+    if (!c) return "coil descriptor is NULL";
+    if (c->n < 0) return "coil count is negative";
+    if (c->n == 0) {
+        out.clear();
+        return "";
+    }
+    if (!c->use_fast || !c->n_ang || !c->n_thk || !c->R || !c->a || !c->b || !c->J || !c->pos || !c->alpha ||
+        !c->beta)
+        return "coil descriptor has a NULL required array";
+    if (need_len && (!c->n_len || !c->current || !c->turns)) return "pair coils need n_len, current and turns";
+    if (need_freq && !c->frequency) return "E requested but coil frequency array is NULL";
+    out.resize((size_t)c->n);
+    for (int64_t i = 0; i < c->n; ++i) {
+        CoilDev d;
+        std::memset(&d, 0, sizeof d);
+        const int na = c->n_ang[i], nt = c->n_thk[i], nl = c->n_len ? c->n_len[i] : 1;
+        if (na < 1 || na > kMaxOrder || nt < 1 || nt > kMaxOrder || nl < 1 || nl > kMaxOrder)
+            return "coil " + std::to_string(i) + ": increments must be in [1, 100]";
+        if (!(c->R[i] > 0.0) || !(c->a[i] > 0.0) || !(c->b[i] > 0.0) || !std::isfinite(c->J[i]))
+            return "coil " + std::to_string(i) + ": R, a, b must be positive and J finite";
+        d.use_fast = c->use_fast[i] ? 1 : 0;
+        d.n_ang = na;
+        d.n_thk = nt;
+        d.n_len = nl;
+        d.blk_ang = c->blocks ? c->blocks[3 * i] : 1;
+        d.blk_thk = c->blocks ? c->blocks[3 * i + 1] : 1;
+        d.blk_len = c->blocks ? c->blocks[3 * i + 2] : 1;
+        if (d.blk_ang < 1 || d.blk_thk < 1 || d.blk_len < 1) return "coil " + std::to_string(i) + ": blocks < 1";
+        d.R = c->R[i];
+        d.a = c->a[i];
+        d.b = c->b[i];
+        const double pi = kPi;
+        d.cf = d.use_fast ? kMuReduced * c->J[i] * d.a * pi * 0.5 : kMuReduced * c->J[i] * d.a * d.b * pi * 0.5;
+        d.omega = c->frequency ? 2.0 * pi * c->frequency[i] : 0.0;
+        d.current = c->current ? c->current[i] : 0.0;
+        d.turns = c->turns ? c->turns[i] : 0.0;
+        double w = std::fmax(std::fmax(d.a, d.b), 1e-3 * d.R);
+        d.near_w2 = w * w;
+        for (int k = 0; k < 3; ++k) d.pos[k] = c->pos[3 * i + k];
+        rotation(c->alpha[i], c->beta[i], d.T, d.Tinv);
+        d.sa = std::sin(c->alpha[i]);
+        d.ca = std::cos(c->alpha[i]);
+        d.sb = std::sin(c->beta[i]);
+        d.cb = std::cos(c->beta[i]);
+        out[(size_t)i] = d;
+    }
+    return "";
+}
+
+Tables ctx_tables(ccoil_ctx ctx)
+{
+    Tables t;
+    t.x = ctx->gl_x;
+    t.w = ctx->gl_w;
+    t.cosv = ctx->gl_cos;
+    t.omc = ctx->gl_omc;
+    t.xf = ctx->gl_xf;
+    t.wf = ctx->gl_wf;
+    t.cosf = ctx->gl_cosf;
+    t.omcf = ctx->gl_omcf;
+    t.rcos = ctx->gl_rcos;
+    t.rsin = ctx->gl_rsin;
+    return t;
+}
+
+}  // namespace ccoil
+
+extern "C" {
+
+const char *ccoil_version(void) { return "ccoil-mi355x 0.1.0 (gfx950)"; }
+
+const char *ccoil_last_error(ccoil_ctx ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+ccoil_status ccoil_create(int device, ccoil_ctx *out)
+{
+    if (!out) return fail(nullptr, CCOIL_ERR_ARGUMENT, "ccoil_create: out is NULL");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        return fail(nullptr, CCOIL_ERR_NO_DEVICE, "no HIP device available");
+    if (device < 0 || device >= count) return fail(nullptr, CCOIL_ERR_ARGUMENT, "device index out of range");
+    ccoil_ctx ctx = new ccoil_ctx_s();
+    ctx->device = device;
+    auto bail = [&](hipError_t e, const char *what) {
+        g_create_error = std::string(what) + ": " + hipGetErrorString(e);
+        ccoil_destroy(ctx);
+        return e == hipErrorOutOfMemory ? CCOIL_ERR_OUT_OF_MEMORY : CCOIL_ERR_DEVICE;
+    };
+    hipError_t e;
+    if ((e = hipSetDevice(device)) != hipSuccess) return bail(e, "hipSetDevice");
+    if ((e = hipDeviceGetAttribute(&ctx->cu_count, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess)
+        return bail(e, "hipDeviceGetAttribute");
+    if ((e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking)) != hipSuccess)
+        return bail(e, "hipStreamCreate");
+    ctx->stream = ctx->own_stream;
+    if ((e = hipEventCreate(&ctx->ev0)) != hipSuccess) return bail(e, "hipEventCreate");
+    if ((e = hipEventCreate(&ctx->ev1)) != hipSuccess) return bail(e, "hipEventCreate");
+    const size_t nd = sizeof(double) * kTableSize, nf = sizeof(float) * kTableSize;
+    double **dp[6] = {&ctx->gl_x, &ctx->gl_w, &ctx->gl_cos, &ctx->gl_omc, &ctx->gl_rcos, &ctx->gl_rsin};
+    float **fp[4] = {&ctx->gl_xf, &ctx->gl_wf, &ctx->gl_cosf, &ctx->gl_omcf};
+    for (auto p : dp)
+        if ((e = hipMalloc((void **)p, nd)) != hipSuccess) return bail(e, "hipMalloc(tables)");
+    for (auto p : fp)
+        if ((e = hipMalloc((void **)p, nf)) != hipSuccess) return bail(e, "hipMalloc(tables)");
+    launch_gl_tables(ctx->stream, ctx->gl_x, ctx->gl_w, ctx->gl_cos, ctx->gl_omc, ctx->gl_xf, ctx->gl_wf,
+                     ctx->gl_cosf, ctx->gl_omcf, ctx->gl_rcos, ctx->gl_rsin);
+    if ((e = hipGetLastError()) != hipSuccess) return bail(e, "gl_table_kernel launch");
+    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return bail(e, "gl_table_kernel");
+    *out = ctx;
+    return CCOIL_OK;
+}
+
+ccoil_status ccoil_destroy(ccoil_ctx ctx)
+{
+    if (!ctx) return CCOIL_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->own_stream) (void)hipStreamSynchronize(ctx->own_stream);
+    double *dp[6] = {ctx->gl_x, ctx->gl_w, ctx->gl_cos, ctx->gl_omc, ctx->gl_rcos, ctx->gl_rsin};
+    float *fp[4] = {ctx->gl_xf, ctx->gl_wf, ctx->gl_cosf, ctx->gl_omcf};
+    for (auto p : dp)
+        if (p) (void)hipFree(p);
+    for (auto p : fp)
+        if (p) (void)hipFree(p);
+    DevBuf *bufs[] = {&ctx->coils, &ctx->pts,   &ctx->outA,  &ctx->outB,     &ctx->outE,     &ctx->outG,
+                      &ctx->queue, &ctx->counter, &ctx->pprim, &ctx->psec, &ctx->ppairs, &ctx->ppartial,
+                      &ctx->pcounter, &ctx->pM, &ctx->pF, &ctx->pT};
+    for (auto b : bufs) free_buf(*b);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+    return CCOIL_OK;
+}
+
+ccoil_status ccoil_set_stream(ccoil_ctx ctx, void *stream)
+{
+    if (!ctx) return fail(nullptr, CCOIL_ERR_ARGUMENT, "ccoil_set_stream: ctx is NULL");
+    ctx->stream = stream ? (hipStream_t)stream : ctx->own_stream;
+    return CCOIL_OK;
+}
+
+ccoil_status ccoil_synchronize(ccoil_ctx ctx)
+{
+    if (!ctx) return fail(nullptr, CCOIL_ERR_ARGUMENT, "ccoil_synchronize: ctx is NULL");
+    CC_HIP(ctx, hipSetDevice(ctx->device));
+    CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return CCOIL_OK;
+}
+
+ccoil_status ccoil_gauss_legendre(ccoil_ctx ctx, int32_t n, double *x, double *w)
+{
+    if (!ctx) return fail(nullptr, CCOIL_ERR_ARGUMENT, "ccoil_gauss_legendre: ctx is NULL");
+    if (n < 1 || n > kMaxOrder) return fail(ctx, CCOIL_ERR_ARGUMENT, "order must be in [1, 100]");
+    if (!x || !w) return fail(ctx, CCOIL_ERR_ARGUMENT, "x and w must not be NULL");
+    CC_HIP(ctx, hipSetDevice(ctx->device));
+    const int off = table_offset(n);
+    CC_HIP(ctx, hipMemcpyAsync(x, ctx->gl_x + off, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    CC_HIP(ctx, hipMemcpyAsync(w, ctx->gl_w + off, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return CCOIL_OK;
+}
+
+ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, const double *points, int64_t n_points,
+                          uint32_t quantities, int32_t precision, uint32_t flags, double *A, double *B, double *E,
+                          double *G, ccoil_report *report)
+{
+    if (!ctx) return fail(nullptr, CCOIL_ERR_ARGUMENT, "ccoil_fields: ctx is NULL");
+    if (n_points < 0) return fail(ctx, CCOIL_ERR_ARGUMENT, "n_points is negative");
+    if (n_points > ((int64_t)1 << 31) - 1) return fail(ctx, CCOIL_ERR_ARGUMENT, "n_points exceeds 2^31 - 1");
+    if (precision != CCOIL_FP64 && precision != CCOIL_FP32C && precision != CCOIL_FP32 &&
+        precision != CCOIL_FP64_REFERENCE)
+        return fail(ctx, CCOIL_ERR_ARGUMENT, "unknown precision");
+    const uint32_t known = CCOIL_A | CCOIL_B | CCOIL_E | CCOIL_G | CCOIL_G_JACOBIAN;
+    if (quantities == 0 || (quantities & ~known)) return fail(ctx, CCOIL_ERR_ARGUMENT, "bad quantity mask");
+    if ((quantities & CCOIL_G) && (quantities & CCOIL_G_JACOBIAN))
+        return fail(ctx, CCOIL_ERR_ARGUMENT, "CCOIL_G and CCOIL_G_JACOBIAN share the G output; request one");
+    if (((quantities & CCOIL_A) && !A) || ((quantities & CCOIL_B) && !B) || ((quantities & CCOIL_E) && !E) ||
+        ((quantities & (CCOIL_G | CCOIL_G_JACOBIAN)) && !G))
+        return fail(ctx, CCOIL_ERR_ARGUMENT, "an output for a requested quantity is NULL");
+    if (n_points > 0 && !points) return fail(ctx, CCOIL_ERR_ARGUMENT, "points is NULL");
+    std::vector<CoilDev> hc;
+    std::string msg = pack_coils(coils, false, (quantities & CCOIL_E) != 0, hc);
+    if (!msg.empty()) return fail(ctx, CCOIL_ERR_ARGUMENT, msg);
+    if (hc.size() > (size_t)INT32_MAX) return fail(ctx, CCOIL_ERR_ARGUMENT, "too many coils");
+    if (report) std::memset(report, 0, sizeof *report);
+    CC_HIP(ctx, hipSetDevice(ctx->device));
+    const bool dev = flags & CCOIL_DEVICE_POINTERS;
+    const int64_t n = n_points;
+    const size_t s3 = sizeof(double) * 3 * (size_t)n, s9 = sizeof(double) * 9 * (size_t)n;
+
+    if (n == 0) return CCOIL_OK;
+    if (hc.empty()) {
+        // No coils: every field is zero (the reference returns a zero-filled array).
+        double *outs[4] = {(quantities & CCOIL_A) ? A : nullptr, (quantities & CCOIL_B) ? B : nullptr,
+                           (quantities & CCOIL_E) ? E : nullptr,
+                           (quantities & (CCOIL_G | CCOIL_G_JACOBIAN)) ? G : nullptr};
+        size_t sz[4] = {s3, s3, s3, s9};
+        for (int k = 0; k < 4; ++k) {
+            if (!outs[k]) continue;
+            if (dev) launch_fill_zero(ctx->stream, outs[k], (int64_t)(sz[k] / sizeof(double)));
+            else std::memset(outs[k], 0, sz[k]);
+        }
+        if (dev) CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return CCOIL_OK;
+    }
+
+    CC_HIP(ctx, ensure(ctx->coils, sizeof(CoilDev) * hc.size()));
+    CC_HIP(ctx, hipMemcpyAsync(ctx->coils.p, hc.data(), sizeof(CoilDev) * hc.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+    FieldParams p;
+    std::memset(&p, 0, sizeof p);
+    p.coils = (const CoilDev *)ctx->coils.p;
+    p.n_coils = (int32_t)hc.size();
+    p.n_pts = n;
+    p.q = quantities;
+    p.tab = ctx_tables(ctx);
+    if (dev) {
+        p.pts = points;
+        p.A = (quantities & CCOIL_A) ? A : nullptr;
+        p.B = (quantities & CCOIL_B) ? B : nullptr;
+        p.E = (quantities & CCOIL_E) ? E : nullptr;
+        p.G = (quantities & (CCOIL_G | CCOIL_G_JACOBIAN)) ? G : nullptr;
+    } else {
+        CC_HIP(ctx, ensure(ctx->pts, s3));
+        CC_HIP(ctx, hipMemcpyAsync(ctx->pts.p, points, s3, hipMemcpyHostToDevice, ctx->stream));
+        p.pts = (const double *)ctx->pts.p;
+        if (quantities & CCOIL_A) { CC_HIP(ctx, ensure(ctx->outA, s3)); p.A = (double *)ctx->outA.p; }
+        if (quantities & CCOIL_B) { CC_HIP(ctx, ensure(ctx->outB, s3)); p.B = (double *)ctx->outB.p; }
+        if (quantities & CCOIL_E) { CC_HIP(ctx, ensure(ctx->outE, s3)); p.E = (double *)ctx->outE.p; }
+        if (quantities & (CCOIL_G | CCOIL_G_JACOBIAN)) { CC_HIP(ctx, ensure(ctx->outG, s9)); p.G = (double *)ctx->outG.p; }
+    }
+
+    // Launch shape: split the thickness rows over 4 lanes when there are too few points to give every
+    // SIMD of the 256 CUs several waves.
+    const int split = n < (int64_t)256 * 4 * 64 * 4 ? 4 : 1;
+    int64_t near = 0;
+    const bool near_fp64 = precision == CCOIL_FP32C && !(flags & CCOIL_NO_NEAR_FP64);
+    CC_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    if (near_fp64) {
+        p.near_fp64 = 1;
+        int64_t cap = n * 4 + 4096;
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            CC_HIP(ctx, ensure(ctx->queue, sizeof(int2) * (size_t)cap));
+            CC_HIP(ctx, ensure(ctx->counter, sizeof(unsigned long long)));
+            p.near_queue = (int2 *)ctx->queue.p;
+            p.near_count = (unsigned long long *)ctx->counter.p;
+            p.near_capacity = cap;
+            launch_fill_zero(ctx->stream, p.near_count, 1);
+            launch_near_classify(ctx->stream, p);
+            CC_HIP(ctx, hipGetLastError());
+            unsigned long long cnt = 0;
+            CC_HIP(ctx, hipMemcpyAsync(&cnt, p.near_count, sizeof cnt, hipMemcpyDeviceToHost, ctx->stream));
+            CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            near = (int64_t)cnt;
+            if (near <= cap) break;
+            cap = near + 4096;
+        }
+    }
+    launch_field(ctx->stream, p, precision, split);
+    CC_HIP(ctx, hipGetLastError());
+    if (near_fp64 && near > 0) {
+        launch_near_eval(ctx->stream, p, near);
+        CC_HIP(ctx, hipGetLastError());
+    }
+    CC_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    if (!dev) {
+        if (p.A) CC_HIP(ctx, hipMemcpyAsync(A, p.A, s3, hipMemcpyDeviceToHost, ctx->stream));
+        if (p.B) CC_HIP(ctx, hipMemcpyAsync(B, p.B, s3, hipMemcpyDeviceToHost, ctx->stream));
+        if (p.E) CC_HIP(ctx, hipMemcpyAsync(E, p.E, s3, hipMemcpyDeviceToHost, ctx->stream));
+        if (p.G) CC_HIP(ctx, hipMemcpyAsync(G, p.G, s9, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (report) {
+        float ms = 0.f;
+        CC_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        int64_t nodes = 0;
+        for (const auto &c : hc) nodes += (int64_t)c.n_ang * c.n_thk;
+        report->node_evaluations = nodes * n;
+        report->near_pairs = near;
+        report->kernel_ms = ms;
+        report->block_size = kFieldBlock;
+        report->split = split;
+    }
+    return CCOIL_OK;
+}
+
+}  // extern "C"

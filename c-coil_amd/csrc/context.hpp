@@ -1,0 +1,57 @@
+// Library-private context state shared by the C ABI translation units.
+#pragma once
+
+#include "ccoil_hip.h"
+#include "kernels.hpp"
+#include "pairs.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+namespace ccoil {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+hipError_t ensure(DevBuf &b, size_t bytes);
+void free_buf(DevBuf &b);
+void rotation(double alpha, double beta, double *T, double *Ti);
+std::string pack_coils(const ccoil_coils *c, bool need_len, bool need_freq, std::vector<CoilDev> &out);
+
+}  // namespace ccoil
+
+struct ccoil_ctx_s {
+    int device = 0;
+    int cu_count = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // Gauss-Legendre tables (FP64 generated on device, FP32 copies, ring tables)
+    double *gl_x = nullptr, *gl_w = nullptr, *gl_cos = nullptr, *gl_omc = nullptr;
+    float *gl_xf = nullptr, *gl_wf = nullptr, *gl_cosf = nullptr, *gl_omcf = nullptr;
+    double *gl_rcos = nullptr, *gl_rsin = nullptr;
+    ccoil::DevBuf coils, pts, outA, outB, outE, outG, queue, counter;
+    ccoil::DevBuf pprim, psec, ppairs, ppartial, pcounter, pM, pF, pT;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace ccoil {
+
+ccoil_status fail(ccoil_ctx ctx, ccoil_status code, const std::string &msg);
+Tables ctx_tables(ccoil_ctx ctx);
+
+#define CC_HIP(ctx, call)                                                                                   \
+    do {                                                                                                    \
+        hipError_t e_ = (call);                                                                             \
+        if (e_ != hipSuccess) {                                                                             \
+            if (e_ == hipErrorOutOfMemory)                                                                  \
+                return ::ccoil::fail(ctx, CCOIL_ERR_OUT_OF_MEMORY, std::string(#call) + ": " + hipGetErrorString(e_)); \
+            return ::ccoil::fail(ctx, CCOIL_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+        }                                                                                                   \
+    } while (0)
+
+}  // namespace ccoil

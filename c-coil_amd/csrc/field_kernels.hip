@@ -1,0 +1,253 @@
+// Field kernels for gfx950: all coils of a CoilGroup, all requested quantities, one launch.
+//
+// Mapping: SPLIT consecutive lanes own one point; the coil loop is wave-uniform, so every coil
+// attribute and every Gauss-Legendre table entry is a scalar (SMEM) load broadcast to the wave and
+// the method branch (fast/slow) never diverges. Lanes of a point split the thickness rows and are
+// reduced with cross-lane shuffles. Results are accumulated in registers over coils (FP64) and each
+// output is written once -- the reference instead launches one kernel per coil and read-modify-writes
+// the result array each time (FieldBGroup.cu:175-181).
+#include "kernels.hpp"
+#include "integrands.hpp"
+
+namespace ccoil {
+
+// ---------------------------------------------------------------------------------------------
+// Gauss-Legendre tables in FP64 (Newton on P_n), one workgroup per order.
+// ---------------------------------------------------------------------------------------------
+__global__ void gl_table_kernel(double *x, double *w, double *cosv, double *omc, float *xf, float *wf,
+                                float *cosf, float *omcf, double *rcos, double *rsin)
+{
+    const int n = blockIdx.x + 1;
+    const int i = threadIdx.x;
+    if (i >= (n + 1) / 2) return;
+    double z = cos(kPi * (i + 0.75) / (n + 0.5));
+    double p0 = 1.0, p1 = z, dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+        p0 = 1.0;
+        p1 = z;
+        for (int k = 2; k <= n; ++k) {
+            double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        if (n == 1) break;
+        dp = n * (z * p1 - p0) / (z * z - 1.0);
+        double dz = p1 / dp;
+        z -= dz;
+        if (fabs(dz) < 1e-17) break;
+    }
+    p0 = 1.0;
+    p1 = z;
+    for (int k = 2; k <= n; ++k) {
+        double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+    }
+    dp = (n == 1) ? 1.0 : n * (z * p1 - p0) / (z * z - 1.0);
+    const double wi = 2.0 / ((1.0 - z * z) * dp * dp);
+    const int off = table_offset(n);
+    int hi = n - 1 - i, lo = i;
+    double xh = z, xl = -z;
+    if (hi == lo) { xh = 0.0; xl = 0.0; }
+    const int idx[2] = {hi, lo};
+    const double xv[2] = {xh, xl};
+    for (int k = 0; k < 2; ++k) {
+        const int e = off + idx[k];
+        const double xx = xv[k];
+        x[e] = xx;
+        w[e] = wi;
+        // cos(pi/2 (1 + x)) = -sin(pi x / 2): exactly antisymmetric in x.
+        const double cv = -sinpi(0.5 * xx);
+        const double sv = sinpi(0.25 * (1.0 + xx));
+        cosv[e] = cv;
+        omc[e] = 2.0 * sv * sv;
+        xf[e] = (float)xx;
+        wf[e] = (float)wi;
+        cosf[e] = (float)cv;
+        omcf[e] = (float)(2.0 * sv * sv);
+        rcos[e] = -cospi(xx);
+        rsin[e] = -sinpi(xx);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Main field kernel.
+// ---------------------------------------------------------------------------------------------
+template <int SPLIT>
+__device__ inline void reduce_local(Local &L)
+{
+#pragma unroll
+    for (int m = 1; m < SPLIT; m <<= 1) {
+        L.A += __shfl_xor(L.A, m, SPLIT);
+        L.Bh += __shfl_xor(L.Bh, m, SPLIT);
+        L.Bz += __shfl_xor(L.Bz, m, SPLIT);
+        L.RP += __shfl_xor(L.RP, m, SPLIT);
+        L.RR += __shfl_xor(L.RR, m, SPLIT);
+        L.RZ += __shfl_xor(L.RZ, m, SPLIT);
+        L.ZZ += __shfl_xor(L.ZZ, m, SPLIT);
+    }
+}
+
+template <int PREC, bool WA, bool WB, bool WG, int SPLIT>
+__global__ __launch_bounds__(kFieldBlock) void field_kernel(FieldParams p)
+{
+    const int64_t t = (int64_t)blockIdx.x * kFieldBlock + threadIdx.x;
+    const int64_t pi = t / SPLIT;
+    const int lane = (int)(t % SPLIT);
+    const bool active = pi < p.n_pts;
+    const int64_t pc = active ? pi : (p.n_pts - 1);   // inactive lanes shadow a valid point
+    const double px = p.pts[3 * pc], py = p.pts[3 * pc + 1], pz = p.pts[3 * pc + 2];
+
+    Cart acc;
+    for (int k = 0; k < 3; ++k) { acc.A[k] = 0.0; acc.E[k] = 0.0; acc.B[k] = 0.0; }
+    for (int k = 0; k < 9; ++k) acc.G[k] = 0.0;
+
+    for (int ci = 0; ci < p.n_coils; ++ci) {
+        const CoilDev &c = p.coils[ci];
+        double z, r, cph, sph;
+        to_local(c, px, py, pz, z, r, cph, sph);
+        Local L = {0, 0, 0, 0, 0, 0, 0};
+        if constexpr (PREC == kFP64) {
+            eval_stable<double, WA, WB, WG>(c, p.tab, z, r, lane, SPLIT, L);
+        } else if constexpr (PREC == kFP64Reference) {
+            eval_plain<double, WA, WB, WG>(c, p.tab, z, r, lane, SPLIT, L);
+        } else if constexpr (PREC == kFP32) {
+            eval_plain<float, WA, WB, WG>(c, p.tab, z, r, lane, SPLIT, L);
+        } else {
+            const bool skip = p.near_fp64 && is_near(c, z, r);
+            if (!skip) eval_stable<float, WA, WB, WG>(c, p.tab, z, r, lane, SPLIT, L);
+        }
+        if (SPLIT > 1) reduce_local<SPLIT>(L);
+        accumulate_global(c, L, r, cph, sph, p.q, acc);
+    }
+    if (!active || lane != 0) return;
+    if (p.A) for (int k = 0; k < 3; ++k) p.A[3 * pi + k] = acc.A[k];
+    if (p.E) for (int k = 0; k < 3; ++k) p.E[3 * pi + k] = acc.E[k];
+    if (p.B) for (int k = 0; k < 3; ++k) p.B[3 * pi + k] = acc.B[k];
+    if (p.G) for (int k = 0; k < 9; ++k) p.G[9 * pi + k] = acc.G[k];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Near-winding pass (FP32C): classify (coil, point) pairs, then re-evaluate them in FP64.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void near_classify_kernel(FieldParams p)
+{
+    const int64_t pi = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (pi >= p.n_pts) return;
+    const double px = p.pts[3 * pi], py = p.pts[3 * pi + 1], pz = p.pts[3 * pi + 2];
+    for (int ci = 0; ci < p.n_coils; ++ci) {
+        const CoilDev &c = p.coils[ci];
+        double z, r, cph, sph;
+        to_local(c, px, py, pz, z, r, cph, sph);
+        if (is_near(c, z, r)) {
+            unsigned long long slot = atomicAdd(p.near_count, 1ull);
+            if (slot < (unsigned long long)p.near_capacity) p.near_queue[slot] = make_int2((int)pi, ci);
+        }
+    }
+}
+
+template <bool WA, bool WB, bool WG>
+__global__ __launch_bounds__(256) void near_eval_kernel(FieldParams p, int64_t count)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= count) return;
+    const int2 ent = p.near_queue[e];
+    const int64_t pi = ent.x;
+    const CoilDev &c = p.coils[ent.y];
+    const double px = p.pts[3 * pi], py = p.pts[3 * pi + 1], pz = p.pts[3 * pi + 2];
+    double z, r, cph, sph;
+    to_local(c, px, py, pz, z, r, cph, sph);
+    Local L;
+    eval_stable<double, WA, WB, WG>(c, p.tab, z, r, 0, 1, L);
+    Cart acc;
+    for (int k = 0; k < 3; ++k) { acc.A[k] = 0.0; acc.E[k] = 0.0; acc.B[k] = 0.0; }
+    for (int k = 0; k < 9; ++k) acc.G[k] = 0.0;
+    accumulate_global(c, L, r, cph, sph, p.q, acc);
+    if (p.A) for (int k = 0; k < 3; ++k) unsafeAtomicAdd(&p.A[3 * pi + k], acc.A[k]);
+    if (p.E) for (int k = 0; k < 3; ++k) unsafeAtomicAdd(&p.E[3 * pi + k], acc.E[k]);
+    if (p.B) for (int k = 0; k < 3; ++k) unsafeAtomicAdd(&p.B[3 * pi + k], acc.B[k]);
+    if (p.G) for (int k = 0; k < 9; ++k) unsafeAtomicAdd(&p.G[9 * pi + k], acc.G[k]);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Launchers.
+// ---------------------------------------------------------------------------------------------
+void launch_gl_tables(hipStream_t s, double *x, double *w, double *cosv, double *omc, float *xf, float *wf,
+                      float *cosf, float *omcf, double *rcos, double *rsin)
+{
+    hipLaunchKernelGGL(gl_table_kernel, dim3(kMaxOrder), dim3(64), 0, s, x, w, cosv, omc, xf, wf, cosf, omcf,
+                       rcos, rsin);
+}
+
+__global__ void fill_zero_kernel(unsigned long long *p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+void launch_fill_zero(hipStream_t s, void *p, int64_t n_words)
+{
+    if (n_words <= 0) return;
+    const unsigned blocks = (unsigned)((n_words + 255) / 256);
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(blocks), dim3(256), 0, s, (unsigned long long *)p, n_words);
+}
+
+template <int PREC, bool WA, bool WB, bool WG>
+static void launch_field_q(hipStream_t s, const FieldParams &p, int split)
+{
+    const int64_t threads = p.n_pts * split;
+    const unsigned blocks = (unsigned)((threads + kFieldBlock - 1) / kFieldBlock);
+    if (split == 1) hipLaunchKernelGGL((field_kernel<PREC, WA, WB, WG, 1>), dim3(blocks), dim3(kFieldBlock), 0, s, p);
+    else hipLaunchKernelGGL((field_kernel<PREC, WA, WB, WG, 4>), dim3(blocks), dim3(kFieldBlock), 0, s, p);
+}
+
+template <int PREC>
+static void launch_field_p(hipStream_t s, const FieldParams &p, int split)
+{
+    const bool wa = p.q & (Q_A | Q_E), wb = p.q & Q_B, wg = p.q & (Q_G | Q_GJ);
+    const int code = (wa ? 4 : 0) | (wb ? 2 : 0) | (wg ? 1 : 0);
+    switch (code) {
+        case 1: launch_field_q<PREC, false, false, true>(s, p, split); break;
+        case 2: launch_field_q<PREC, false, true, false>(s, p, split); break;
+        case 3: launch_field_q<PREC, false, true, true>(s, p, split); break;
+        case 4: launch_field_q<PREC, true, false, false>(s, p, split); break;
+        case 5: launch_field_q<PREC, true, false, true>(s, p, split); break;
+        case 6: launch_field_q<PREC, true, true, false>(s, p, split); break;
+        default: launch_field_q<PREC, true, true, true>(s, p, split); break;
+    }
+}
+
+void launch_field(hipStream_t s, const FieldParams &p, int precision, int split)
+{
+    if (p.n_pts <= 0) return;
+    if (precision == kFP64) launch_field_p<kFP64>(s, p, split);
+    else if (precision == kFP64Reference) launch_field_p<kFP64Reference>(s, p, split);
+    else if (precision == kFP32) launch_field_p<kFP32>(s, p, split);
+    else launch_field_p<kFP32C>(s, p, split);
+}
+
+void launch_near_classify(hipStream_t s, const FieldParams &p)
+{
+    if (p.n_pts <= 0) return;
+    const unsigned blocks = (unsigned)((p.n_pts + 255) / 256);
+    hipLaunchKernelGGL(near_classify_kernel, dim3(blocks), dim3(256), 0, s, p);
+}
+
+void launch_near_eval(hipStream_t s, const FieldParams &p, int64_t count)
+{
+    if (count <= 0) return;
+    const unsigned blocks = (unsigned)((count + 255) / 256);
+    const bool wa = p.q & (Q_A | Q_E), wb = p.q & Q_B, wg = p.q & (Q_G | Q_GJ);
+    const int code = (wa ? 4 : 0) | (wb ? 2 : 0) | (wg ? 1 : 0);
+    switch (code) {
+        case 1: hipLaunchKernelGGL((near_eval_kernel<false, false, true>), dim3(blocks), dim3(256), 0, s, p, count); break;
+        case 2: hipLaunchKernelGGL((near_eval_kernel<false, true, false>), dim3(blocks), dim3(256), 0, s, p, count); break;
+        case 3: hipLaunchKernelGGL((near_eval_kernel<false, true, true>), dim3(blocks), dim3(256), 0, s, p, count); break;
+        case 4: hipLaunchKernelGGL((near_eval_kernel<true, false, false>), dim3(blocks), dim3(256), 0, s, p, count); break;
+        case 5: hipLaunchKernelGGL((near_eval_kernel<true, false, true>), dim3(blocks), dim3(256), 0, s, p, count); break;
+        case 6: hipLaunchKernelGGL((near_eval_kernel<true, true, false>), dim3(blocks), dim3(256), 0, s, p, count); break;
+        default: hipLaunchKernelGGL((near_eval_kernel<true, true, true>), dim3(blocks), dim3(256), 0, s, p, count); break;
+    }
+}
+
+}  // namespace ccoil

@@ -1,0 +1,328 @@
+// Quadrature integrands of one (coil, point) pair in the coil's local cylindrical frame.
+//
+// eval_plain<T>: the reference's integrands verbatim in arithmetic T (double: the FP64 comparator and
+//   the near-winding evaluator; float: what a TYPE=float build of the reference computes).
+//   Fast (THIN/RECTANGULAR) B: FieldBGroup.cu:47-77, G: FieldGGroup.cu:257-305, A: FieldAGroup.cu;
+//   slow (FLAT/FILAMENT) B: FieldBGroup.cu:78-101, G: FieldGGroup.cu:306-339, A: FieldAGroup.cu.
+// eval_stable<T>: the same integrands, algebraically rearranged so that no subtraction cancels; per-node
+//   work in T (float: FP32C; double: the accurate FP64 mode), per-row work and row sums in FP64.
+//
+// Both evaluate thickness rows j = row0, row0 + rstep, ... so several lanes can share one point.
+#pragma once
+
+#include "common.hpp"
+
+namespace ccoil {
+
+template <typename T> __device__ inline T rsq(T x);
+template <> __device__ inline double rsq<double>(double x) { return 1.0 / sqrt(x); }
+template <> __device__ inline float rsq<float>(float x) { return __builtin_amdgcn_rsqf(x); }
+
+template <typename T> __device__ inline T div_(T a, T b);
+template <> __device__ inline double div_<double>(double a, double b) { return a / b; }
+template <> __device__ inline float div_<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
+template <typename T> __device__ inline T sqrt_(T x);
+template <> __device__ inline double sqrt_<double>(double x) { return sqrt(x); }
+template <> __device__ inline float sqrt_<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+template <typename T> __device__ inline T log_(T x);
+template <> __device__ inline double log_<double>(double x) { return log(x); }
+template <> __device__ inline float log_<float>(float x) { return __logf(x); }
+
+template <typename T> __device__ inline const T *tab_x(const Tables &t);
+template <> __device__ inline const double *tab_x<double>(const Tables &t) { return t.x; }
+template <> __device__ inline const float *tab_x<float>(const Tables &t) { return t.xf; }
+template <typename T> __device__ inline const T *tab_w(const Tables &t);
+template <> __device__ inline const double *tab_w<double>(const Tables &t) { return t.w; }
+template <> __device__ inline const float *tab_w<float>(const Tables &t) { return t.wf; }
+template <typename T> __device__ inline const T *tab_cos(const Tables &t);
+template <> __device__ inline const double *tab_cos<double>(const Tables &t) { return t.cosv; }
+template <> __device__ inline const float *tab_cos<float>(const Tables &t) { return t.cosf; }
+
+template <typename T, bool WA, bool WB, bool WG>
+__device__ inline void eval_plain(const CoilDev &c, const Tables &tab, double zd, double rd, int row0, int rstep,
+                                  Local &L)
+{
+    const T *xt = tab_x<T>(tab) + table_offset(c.n_thk);
+    const T *wt = tab_w<T>(tab) + table_offset(c.n_thk);
+    const T *ct = tab_cos<T>(tab) + table_offset(c.n_ang);
+    const T *wa = tab_w<T>(tab) + table_offset(c.n_ang);
+    const int na = c.n_ang, nt = c.n_thk;
+    const T cf = (T)c.cf, R = (T)c.R, a = (T)c.a, b = (T)c.b;
+    const T z = (T)zd, r = (T)rd;
+    T sA = 0, sBh = 0, sBz = 0, sRP = 0, sRR = 0, sRZ = 0, sZZ = 0;
+
+    if (c.use_fast) {
+        const T top = z + (T)0.5 * b, bot = z - (T)0.5 * b;
+        for (int j = row0; j < nt; j += rstep) {
+            const T rho = R + (T)0.5 * a * ((T)1 + xt[j]);
+            const T wtj = wt[j];
+            // B: FieldBGroup.cu:51-58
+            const T A_ = rho * rho;
+            const T Bb = (T)2 * rho * r;
+            const T C_ = A_ + r * r;
+            const T D1 = top * top + C_, D2 = bot * bot + C_;
+            // G: FieldGGroup.cu:261-272
+            const T gB = r * r, gC = rho * r, gD = A_ + gB, gE = A_ * A_ + gB * gB, gF = gC * gC;
+            // A: FieldAGroup.cu (tempConstA/B)
+            const T aA = A_ + r * r, aB = (T)2 * rho * r;
+            for (int i = 0; i < na; ++i) {
+                const T cp = ct[i];
+                const T g = cf * wtj * wa[i];
+                if (WB) {
+                    const T E_ = Bb * cp;
+                    const T F1 = rsq<T>(D1 - E_), F2 = rsq<T>(D2 - E_);
+                    sBh += g * rho * cp * (F2 - F1);
+                    sBz += g * div_<T>(A_ - (T)0.5 * E_, C_ - E_) * (top * F1 - bot * F2);
+                }
+                if (WG) {
+                    const T cp2 = cp * cp;
+                    const T phiE = (T)2 * gC * cp;
+                    const T J1 = gD + top * top - phiE, J2 = gD + bot * bot - phiE;
+                    const T K1 = rsq<T>(J1), K2 = rsq<T>(J2);
+                    const T L1 = div_<T>(K1, J1), L2 = div_<T>(K2, J2);
+                    const T M_ = gD - phiE;
+                    const T N_ = (T)2 * gF * cp * (cp2 + (T)2) - gC * ((T)3 * cp2 + (T)1) * gD + cp * gE;
+                    const T O_ = cp * gD - (T)2 * gC;
+                    sRP += g * div_<T>(rho * cp, r) * (K2 - K1);
+                    sRR += g * (gC - A_ * cp) * cp * (L1 - L2);
+                    sZZ += g * (A_ - gC * cp) * (L1 - L2);
+                    sRZ += g * div_<T>(rho, M_ * M_) * (top * L1 * (O_ * J1 + N_) - bot * L2 * (O_ * J2 + N_));
+                }
+                if (WA) {
+                    const T C = rsq<T>(aA - aB * cp);
+                    const T d1 = top * C, d2 = bot * C;
+                    const T e1 = sqrt_<T>(d1 * d1 + (T)1), e2 = sqrt_<T>(d2 * d2 + (T)1);
+                    sA += g * rho * cp * log_<T>(div_<T>(e1 + d1, e2 + d2));
+                }
+            }
+        }
+    } else {
+        for (int j = row0; j < nt; j += rstep) {
+            const T rho = R + (T)0.5 * a * ((T)1 + xt[j]);
+            const T wtj = wt[j];
+            // B: FieldBGroup.cu:82-87
+            const T A_ = rho * rho, B_ = rho * r, C_ = A_ + r * r + z * z, D_ = rho * z;
+            // G: FieldGGroup.cu:310-319
+            const T gB = r * r, gC = z * z, gD = r * rho, gE = rho * z;
+            const T gF = (T)2 * A_ + (T)2 * gB - gC, gG = A_ + gB + gC;
+            for (int i = 0; i < na; ++i) {
+                const T cp = ct[i];
+                const T h = cf * wtj * wa[i];
+                if (WB) {
+                    const T E_ = C_ - (T)2 * B_ * cp;
+                    const T F_ = E_ * sqrt_<T>(E_);
+                    const T G_ = div_<T>(h, F_);
+                    sBh += G_ * (D_ * cp);
+                    sBz += G_ * (A_ - B_ * cp);
+                }
+                if (WG) {
+                    const T I_ = gG - (T)2 * gD * cp;
+                    const T J_ = I_ * sqrt_<T>(I_);
+                    const T X = div_<T>(h, J_), Y = div_<T>(h, J_ * I_);
+                    sRP += div_<T>(X * gE * cp, r);
+                    sRR += Y * ((T)-3 * gE * (r - rho * cp)) * cp;
+                    sZZ += Y * ((T)-3 * gE * (rho - r * cp));
+                    sRZ += Y * (rho * (gF - gD * cp) * cp - (T)3 * A_ * r);
+                }
+                if (WA) {
+                    const T aA = A_ + r * r + z * z, aB = (T)2 * rho * r;
+                    sA += h * rho * cp * rsq<T>(aA - aB * cp);
+                }
+            }
+        }
+    }
+    L.A = (double)sA;
+    L.Bh = (double)sBh;
+    L.Bz = (double)sBz;
+    L.RP = (double)sRP;
+    L.RR = (double)sRR;
+    L.RZ = (double)sRZ;
+    L.ZZ = (double)sZZ;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Cancellation-free integrands (the same discretisation as eval_plain, algebraically rearranged).
+// Notation: rho = source radius of the row, d = rho - r, base = d^2, q = 2 rho r,
+// s = 1 - cos(phi) (table), M = base + q s (in-plane distance squared: a sum of non-negative terms),
+// J_k = M + z_k^2 with z_1 = z + b/2, z_2 = z - b/2, K_k = J_k^-1/2.
+//   K1 - K2       = -2 b z (K1 K2)^2 / (K1 + K2)                  (exact identity, no cancellation)
+//   z1 K1 - z2 K2 = 2 b z M (K1 K2)^2 / (z1 K1 + z2 K2)            when z1 z2 > 0
+//                 = |z1| K1 + |z2| K2 (direct)                      otherwise
+//   z1 K1^3 - z2 K2^3 = (z1 K1 - z2 K2) K1^2 - 2 b z z2 K2 (K1 K2)^2  when z1 z2 > 0, direct otherwise
+//   N (gradient RZ term) as a polynomial in s with coefficients in base and q.
+// One reciprocal serves both 1/M and 1/(z1 K1 + z2 K2): Y = M (z1 K1 + z2 K2) (or M), R = 1/Y.
+// The factor rho common to every integrand and the thickness weight are applied per row in FP64, and row
+// sums are accumulated in FP64, so T = float computes only the per-node work in FP32 (FP32C).
+// ---------------------------------------------------------------------------------------------
+template <typename T> __device__ inline T rcp_(T x);
+template <> __device__ inline double rcp_<double>(double x) { return 1.0 / x; }
+template <> __device__ inline float rcp_<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
+template <typename T> __device__ inline T fma_(T a, T b, T c);
+template <> __device__ inline double fma_<double>(double a, double b, double c) { return fma(a, b, c); }
+template <> __device__ inline float fma_<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+template <typename T> __device__ inline T log1p_(T x);
+template <> __device__ inline double log1p_<double>(double x) { return log1p(x); }
+template <> __device__ inline float log1p_<float>(float x) { return log1pf(x); }
+
+template <typename T> __device__ inline const T *tab_omc(const Tables &t);
+template <> __device__ inline const double *tab_omc<double>(const Tables &t) { return t.omc; }
+template <> __device__ inline const float *tab_omc<float>(const Tables &t) { return t.omcf; }
+
+template <typename T, bool WA, bool WB, bool WG>
+__device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double zd, double rd, int row0, int rstep,
+                                   Local &L)
+{
+    const double *xt = tab.x + table_offset(c.n_thk);
+    const double *wt = tab.w + table_offset(c.n_thk);
+    const T *ct = tab_cos<T>(tab) + table_offset(c.n_ang);
+    const T *st = tab_omc<T>(tab) + table_offset(c.n_ang);
+    const T *wa = tab_w<T>(tab) + table_offset(c.n_ang);
+    const int na = c.n_ang, nt = c.n_thk;
+    double aA = 0, aBh = 0, aBz = 0, aRR = 0, aRZ = 0, aZZ = 0;
+
+    if (c.use_fast) {
+        const double z1d = zd + 0.5 * c.b, z2d = zd - 0.5 * c.b;
+        const T z1 = (T)z1d, z2 = (T)z2d;
+        const T z1s = (T)(z1d * z1d), z2s = (T)(z2d * z2d);
+        const T tbz = (T)(2.0 * c.b * zd);
+        const bool same = z1d * z2d > 0.0;
+        // A is even in z: with |z| the upper edge is positive.
+        const double zad = fabs(zd);
+        const T za1 = (T)(zad + 0.5 * c.b), za2 = (T)(zad - 0.5 * c.b);
+        const T za1s = (T)((zad + 0.5 * c.b) * (zad + 0.5 * c.b)), za2s = (T)((zad - 0.5 * c.b) * (zad - 0.5 * c.b));
+        const T tbza = (T)(2.0 * c.b * zad);
+        const bool samea = (zad - 0.5 * c.b) > 0.0;
+        for (int j = row0; j < nt; j += rstep) {
+            const double rho_d = c.R + 0.5 * c.a * (1.0 + xt[j]);
+            const double dd = rho_d - rd;
+            const T base = (T)(dd * dd);
+            const T q = (T)(2.0 * rho_d * rd);
+            const T dm = (T)dd;                 // rho - r
+            const T rf = (T)rd, rho = (T)rho_d;
+            const T base2 = (T)(dd * dd * dd * dd);
+            const T n1 = (T)(dd * dd * (2.0 * rho_d * rd - dd * dd));
+            const T n2 = (T)(-3.0 * rho_d * rd * dd * dd);
+            const T n3 = (T)(-2.0 * rho_d * rho_d * rd * rd);
+            T rBh = 0, rBz = 0, rRR = 0, rRZ = 0, rZZ = 0, rA = 0;
+            for (int i = 0; i < na; ++i) {
+                const T cp = ct[i], s = st[i], w = wa[i];
+                const T M = fma_<T>(q, s, base);
+                if (WB || WG) {
+                    const T J1 = M + z1s, J2 = M + z2s;
+                    const T K1 = rsq<T>(J1), K2 = rsq<T>(J2);
+                    const T S = rcp_<T>(K1 + K2);
+                    const T K12 = K1 * K2;
+                    const T tP2 = tbz * (K12 * K12);
+                    const T dK = -tP2 * S;                              // K1 - K2
+                    const T g1 = z1 * K1, g2 = z2 * K2;
+                    const T gs = g1 + g2;
+                    const T Y = same ? M * gs : M;
+                    const T Rr = rcp_<T>(Y);
+                    const T invM = same ? gs * Rr : Rr;
+                    const T dgM = same ? tP2 * (M * Rr) : (g1 - g2) * Rr;   // (z1 K1 - z2 K2) / M
+                    const T rmc = fma_<T>(rf, s, dm);                       // rho - r cos
+                    if (WB) {
+                        rBh = fma_<T>(w, -cp * dK, rBh);
+                        rBz = fma_<T>(w, rmc * dgM, rBz);
+                    }
+                    if (WG) {
+                        const T K1s = K1 * K1, K2s = K2 * K2;
+                        const T dL = dK * (K1s + K12 + K2s);                 // K1^3 - K2^3
+                        const T rcm = fma_<T>(rho, s, -dm);                 // r - rho cos
+                        rRR = fma_<T>(w, rcm * cp * dL, rRR);
+                        rZZ = fma_<T>(w, rmc * dL, rZZ);
+                        const T dg = dgM * M;                               // z1 K1 - z2 K2
+                        const T dh = same ? fma_<T>(dg, K1s, -tP2 * g2) : (g1 * K1s - g2 * K2s);
+                        const T O = fma_<T>(cp, base, -q * s);              // cos (rho^2 + r^2) - 2 rho r
+                        const T N = fma_<T>(s, fma_<T>(s, fma_<T>(s, n3, n2), n1), base2);
+                        const T t = fma_<T>(O, dg, N * dh);
+                        rRZ = fma_<T>(w, t * (invM * invM), rRZ);
+                    }
+                }
+                if (WA) {
+                    const T J1 = M + za1s, J2 = M + za2s;
+                    const T K1 = rsq<T>(J1), K2 = rsq<T>(J2);
+                    const T sJ1 = J1 * K1, sJ2 = J2 * K2;                   // sqrt(J)
+                    // asinh(z1 / sqrt(M)) - asinh(z2 / sqrt(M)) = asinh(t)
+                    const T num = samea ? tbza : (za1 * sJ2 - za2 * sJ1);
+                    const T den = samea ? (za1 * sJ2 + za2 * sJ1) : M;
+                    const T t = num * rcp_<T>(den);
+                    const T y = sqrt_<T>(fma_<T>(t, t, (T)1));
+                    const T as = log1p_<T>(t + t * t * rcp_<T>((T)1 + y));
+                    rA = fma_<T>(w, cp * as, rA);
+                }
+            }
+            const double W = c.cf * wt[j] * rho_d;
+            aBh += W * (double)rBh;
+            aBz += W * (double)rBz;
+            aRR += W * (double)rRR;
+            aZZ += W * (double)rZZ;
+            aRZ += W * (double)rRZ;
+            aA += W * (double)rA;
+        }
+        L.A = aA;
+        L.Bh = aBh;
+        L.Bz = aBz;
+        L.RP = aBh / rd;
+        L.RR = aRR;
+        L.RZ = aRZ;
+        L.ZZ = aZZ;
+    } else {
+        for (int j = row0; j < nt; j += rstep) {
+            const double rho_d = c.R + 0.5 * c.a * (1.0 + xt[j]);
+            const double dd = rho_d - rd;
+            const double zz = zd * zd;
+            const T baseJ = (T)(dd * dd + zz);
+            const T q = (T)(2.0 * rho_d * rd);
+            const T dm = (T)dd;
+            const T rf = (T)rd, rho = (T)rho_d;
+            const T k0 = (T)(2.0 * dd * dd - zz);
+            const T k1 = (T)(2.0 * dd * dd + 2.0 * rho_d * rd - zz);
+            const T k2 = (T)(rho_d * rd);
+            T rBh = 0, rBz = 0, rRR = 0, rRZ = 0, rZZ = 0, rA = 0;
+            for (int i = 0; i < na; ++i) {
+                const T cp = ct[i], s = st[i], w = wa[i];
+                const T J = fma_<T>(q, s, baseJ);
+                const T K = rsq<T>(J);
+                if (WB || WG) {
+                    const T K2 = K * K;
+                    const T K3 = K2 * K;
+                    const T rmc = fma_<T>(rf, s, dm);     // rho - r cos
+                    if (WB) {
+                        rBh = fma_<T>(w, cp * K3, rBh);
+                        rBz = fma_<T>(w, rmc * K3, rBz);
+                    }
+                    if (WG) {
+                        const T K5 = K3 * K2;
+                        const T rcm = fma_<T>(rho, s, -dm); // r - rho cos
+                        rRR = fma_<T>(w, rcm * cp * K5, rRR);
+                        rZZ = fma_<T>(w, rmc * K5, rZZ);
+                        const T poly = k0 - s * fma_<T>(k2, s, k1);
+                        rRZ = fma_<T>(w, poly * K5, rRZ);
+                    }
+                }
+                if (WA) rA = fma_<T>(w, cp * K, rA);
+            }
+            const double W = c.cf * wt[j] * rho_d;
+            aBh += W * (double)rBh;
+            aBz += W * (double)rBz;
+            aRR += W * (double)rRR;
+            aZZ += W * (double)rZZ;
+            aRZ += W * (double)rRZ;
+            aA += W * (double)rA;
+        }
+        L.A = aA;
+        L.Bh = aBh * zd;
+        L.Bz = aBz;
+        L.RP = aBh * zd / rd;
+        L.RR = -3.0 * zd * aRR;
+        L.RZ = aRZ;
+        L.ZZ = -3.0 * zd * aZZ;
+    }
+}
+
+}  // namespace ccoil

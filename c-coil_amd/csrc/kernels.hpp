@@ -1,0 +1,35 @@
+// Kernel parameter blocks and launcher declarations (host <-> device contract inside the library).
+#pragma once
+
+#include "common.hpp"
+
+namespace ccoil {
+
+// Arithmetic of the quadrature (values of the C ABI precision argument).
+enum { kFP64 = 0, kFP32C = 1, kFP32 = 2, kFP64Reference = 3 };
+constexpr int kFieldBlock = 256;
+
+struct FieldParams {
+    const CoilDev *coils;
+    int32_t n_coils;
+    const double *pts;     // AoS xyz
+    int64_t n_pts;
+    double *A, *E, *B, *G; // AoS outputs (NULL = not requested)
+    uint32_t q;            // Q_* mask
+    Tables tab;
+    int32_t near_fp64;     // FP32C: skip near pairs here, re-evaluate in FP64
+    int2 *near_queue;      // (point, coil)
+    unsigned long long *near_count;
+    int64_t near_capacity;
+};
+
+void launch_gl_tables(hipStream_t s, double *x, double *w, double *cosv, double *omc, float *xf, float *wf,
+                      float *cosf, float *omcf, double *rcos, double *rsin);
+// split: 1 or 4 lanes per point.
+// Zeroes n_words 8-byte words with a kernel (no hipMemsetAsync, so callers may capture it in a graph).
+void launch_fill_zero(hipStream_t s, void *p, int64_t n_words);
+void launch_field(hipStream_t s, const FieldParams &p, int precision, int split);
+void launch_near_classify(hipStream_t s, const FieldParams &p);
+void launch_near_eval(hipStream_t s, const FieldParams &p, int64_t count);
+
+}  // namespace ccoil

@@ -1,0 +1,170 @@
+// Pair kernels: M, F and T of many (primary, secondary) pairs of mixed type and uneven cost in one launch.
+//
+// A work item is (pair, chunk of secondary nodes) and one wave (one 64-lane workgroup) owns one item:
+// item = blockIdx.x. Items are laid out longest-first on the host, and workgroups are dispatched in index
+// order, so the launch is an LPT schedule without a shared cursor. Each lane evaluates the primary's A or B
+// at one secondary node at a time (the primary coil is wave-uniform, so its loops and tables are scalar),
+// the 7 weighted sums are reduced across the wave in FP64 and one partial is written per item. A second
+// kernel adds the partials of each pair in item order, so results are deterministic. The reference instead
+// runs one thread per (configuration, secondary node) and FP32 atomicAdds per configuration
+// (MInductance.cu:157, Force.cu:172-178).
+#include "integrands.hpp"
+#include "pairs.hpp"
+
+namespace ccoil {
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// PREC: 0 FP64 (stable integrands), 1 FP32C (stable in FP32, near-winding nodes in FP64),
+// 2 FP32 (reference integrands in float), 3 FP64 reference integrands.
+template <int PREC, bool WM, bool WF>
+__device__ inline void eval_prim(const CoilDev &pc, const Tables &tab, double z, double r, Local &L)
+{
+    if constexpr (PREC == 0) {
+        eval_stable<double, WM, WF, false>(pc, tab, z, r, 0, 1, L);
+    } else if constexpr (PREC == 2) {
+        eval_plain<float, WM, WF, false>(pc, tab, z, r, 0, 1, L);
+    } else if constexpr (PREC == 3) {
+        eval_plain<double, WM, WF, false>(pc, tab, z, r, 0, 1, L);
+    } else if (is_near(pc, z, r)) {
+        eval_stable<double, WM, WF, false>(pc, tab, z, r, 0, 1, L);
+    } else {
+        eval_stable<float, WM, WF, false>(pc, tab, z, r, 0, 1, L);
+    }
+}
+
+template <int PREC, bool WM, bool WF>
+__global__ __launch_bounds__(64) void pair_kernel(PairParams p)
+{
+    const int lane = threadIdx.x;
+    const int64_t item = blockIdx.x;
+    {
+        int lo = 0, hi = p.n_pairs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (p.pairs[mid].first_item <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const PairDev &pd = p.pairs[lo];
+        const CoilDev &pc = p.prim[pd.prim];
+        const CoilDev &sc = p.sec[pd.sec];
+        const int64_t n0 = (item - pd.first_item) * pd.nodes_per_item;
+        const int64_t n1 = min(n0 + (int64_t)pd.nodes_per_item, pd.n_nodes);
+        const int NA = sc.blk_ang * sc.n_ang, NT = sc.blk_thk * sc.n_thk;
+        const bool table_ring = sc.blk_ang == 1 && pd.half_ring == 0 && pd.ring_offset == 0.0;
+        const double ang_blk = (pd.half_ring ? kPi : 2.0 * kPi) / sc.blk_ang;
+        const double tbs = sc.a / sc.blk_thk, lbs = sc.b / sc.blk_len;
+        const double ax[3] = {sc.sa * sc.cb, sc.sa * sc.sb, sc.ca};
+        const double u[3] = {sc.cb * sc.ca, sc.sb * sc.ca, -sc.sa};
+        const double v[3] = {-sc.sb, sc.cb, 0.0};
+        const int oa = table_offset(sc.n_ang), ot = table_offset(sc.n_thk), ol = table_offset(sc.n_len);
+        double m = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0;
+        for (int64_t k = n0 + lane; k < n1; k += 64) {
+            const int ia = (int)(k % NA);
+            const int64_t rest = k / NA;
+            const int it = (int)(rest % NT);
+            const int il = (int)(rest / NT);
+            const int i = ia % sc.n_ang, j = it % sc.n_thk, l = il % sc.n_len;
+            double cph, sph;
+            if (table_ring) {
+                cph = p.tab.rcos[oa + i];
+                sph = p.tab.rsin[oa + i];
+            } else {
+                const double phi = pd.ring_offset + ang_blk * ((ia / sc.n_ang) + 0.5) + 0.5 * ang_blk * p.tab.x[oa + i];
+                sincos(phi, &sph, &cph);
+            }
+            const double rho = sc.R + tbs * ((it / sc.n_thk) + 0.5) + 0.5 * tbs * p.tab.x[ot + j];
+            const double ell = -0.5 * sc.b + lbs * ((il / sc.n_len) + 0.5) + 0.5 * lbs * p.tab.x[ol + l];
+            const double w = 0.125 * rho * p.tab.w[ol + l] * p.tab.w[ot + j] * p.tab.w[oa + i];
+            const double arm[3] = {ell * ax[0] + rho * (cph * u[0] + sph * v[0]),
+                                   ell * ax[1] + rho * (cph * u[1] + sph * v[1]),
+                                   ell * ax[2] + rho * (cph * u[2] + sph * v[2])};
+            const double tan[3] = {-sph * u[0] + cph * v[0], -sph * u[1] + cph * v[1], -sph * u[2] + cph * v[2]};
+            double z, r, pcph, psph;
+            to_local(pc, sc.pos[0] + arm[0], sc.pos[1] + arm[1], sc.pos[2] + arm[2], z, r, pcph, psph);
+            Local L = {0, 0, 0, 0, 0, 0, 0};
+            eval_prim<PREC, WM, WF>(pc, p.tab, z, r, L);
+            if (WM) {
+                const double vl[3] = {-L.A * psph, L.A * pcph, 0.0};
+                double ag[3];
+                mat_vec(pc.T, vl, ag);
+                m += w * (ag[0] * tan[0] + ag[1] * tan[1] + ag[2] * tan[2]);
+            }
+            if (WF) {
+                const double vl[3] = {L.Bh * pcph, L.Bh * psph, L.Bz};
+                double bg[3];
+                mat_vec(pc.T, vl, bg);
+                const double df0 = w * (tan[1] * bg[2] - tan[2] * bg[1]);
+                const double df1 = w * (tan[2] * bg[0] - tan[0] * bg[2]);
+                const double df2 = w * (tan[0] * bg[1] - tan[1] * bg[0]);
+                f0 += df0;
+                f1 += df1;
+                f2 += df2;
+                t0 += arm[1] * df2 - arm[2] * df1;
+                t1 += arm[2] * df0 - arm[0] * df2;
+                t2 += arm[0] * df1 - arm[1] * df0;
+            }
+        }
+        double *out = p.partial + 7 * item;
+        if (WM) {
+            m = wave_sum(m);
+            if (lane == 0) out[0] = m;
+        }
+        if (WF) {
+            f0 = wave_sum(f0); f1 = wave_sum(f1); f2 = wave_sum(f2);
+            t0 = wave_sum(t0); t1 = wave_sum(t1); t2 = wave_sum(t2);
+            if (lane == 0) {
+                out[1] = f0; out[2] = f1; out[3] = f2;
+                out[4] = t0; out[5] = t1; out[6] = t2;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void pair_reduce_kernel(PairParams p)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= p.n_pairs) return;
+    const PairDev &pd = p.pairs[k];
+    const int64_t end = (k + 1 < p.n_pairs) ? p.pairs[k + 1].first_item : p.n_items;
+    double s[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int64_t it = pd.first_item; it < end; ++it)
+        for (int q = 0; q < 7; ++q) s[q] += p.partial[7 * it + q];
+    if (p.what & 1) p.M[pd.out] = s[0] * pd.m_scale;
+    if (p.what & 2) {
+        for (int q = 0; q < 3; ++q) p.F[3 * pd.out + q] = s[1 + q] * pd.f_scale;
+        for (int q = 0; q < 3; ++q) p.T[3 * pd.out + q] = s[4 + q] * pd.f_scale;
+    }
+}
+
+template <int PREC>
+static void launch_pairs_p(hipStream_t s, const PairParams &p)
+{
+    const bool wm = p.what & 1, wf = p.what & 2;
+    const dim3 grid((unsigned)p.n_items);
+    if (wm && wf) hipLaunchKernelGGL((pair_kernel<PREC, true, true>), grid, dim3(64), 0, s, p);
+    else if (wm) hipLaunchKernelGGL((pair_kernel<PREC, true, false>), grid, dim3(64), 0, s, p);
+    else hipLaunchKernelGGL((pair_kernel<PREC, false, true>), grid, dim3(64), 0, s, p);
+}
+
+void launch_pairs(hipStream_t s, const PairParams &p)
+{
+    if (p.n_items <= 0) return;
+    if (p.precision == 0) launch_pairs_p<0>(s, p);
+    else if (p.precision == 2) launch_pairs_p<2>(s, p);
+    else if (p.precision == 3) launch_pairs_p<3>(s, p);
+    else launch_pairs_p<1>(s, p);
+}
+
+void launch_pair_reduce(hipStream_t s, const PairParams &p)
+{
+    if (p.n_pairs <= 0) return;
+    hipLaunchKernelGGL(pair_reduce_kernel, dim3((p.n_pairs + 255) / 256), dim3(256), 0, s, p);
+}
+
+}  // namespace ccoil

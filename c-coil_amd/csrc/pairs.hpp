@@ -1,0 +1,40 @@
+// Pair (mutual inductance, force, torque) work queue: parameters, buffers and launchers.
+#pragma once
+
+#include "common.hpp"
+
+namespace ccoil {
+
+// Per-pair description, sorted by cost (longest first) on the host.
+struct PairDev {
+    int32_t prim;          // index into the primary coil array
+    int32_t sec;           // index into the secondary coil array
+    int32_t out;           // output slot (original pair index)
+    int32_t nodes_per_item;
+    int64_t first_item;    // first work item of this pair
+    int64_t n_nodes;       // secondary nodes of this pair
+    double ring_offset;    // improved-precision angular offset (SupportFunctions.cxx:152-157)
+    int32_t half_ring;     // 1: integrate phi over [offset, offset + pi] (SupportFunctions.cxx:86-87)
+    int32_t pad;
+    double m_scale;        // 2 pi N2 / (I1 * blocks)   (CalculateMInductanceGeneral.cxx:115-116)
+    double f_scale;        // 2 pi N2 I2 / blocks, or 2 pi N2 / I1 (reference arrangement scale)
+};
+
+struct PairParams {
+    const CoilDev *prim;
+    const CoilDev *sec;
+    const PairDev *pairs;
+    int32_t n_pairs;
+    int64_t n_items;
+    double *partial;                 // 7 doubles per item: M, F[3], T[3]
+    uint32_t what;                   // 1: M, 2: F and T
+    int32_t precision;
+    Tables tab;
+    double *M, *F, *T;               // outputs (device), indexed by PairDev::out
+};
+
+// One 64-lane workgroup per work item (grid = n_items, at most 2^31 - 1).
+void launch_pairs(hipStream_t s, const PairParams &p);
+void launch_pair_reduce(hipStream_t s, const PairParams &p);
+
+}  // namespace ccoil

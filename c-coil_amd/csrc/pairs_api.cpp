@@ -1,0 +1,155 @@
+// C ABI: ccoil_pairs (mutual inductance, force and torque of many coil pairs, include/ccoil_hip.h).
+#include "context.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+using namespace ccoil;
+
+namespace {
+
+// Nodes per work item: enough secondary nodes that every lane of a wave performs about 8192 primary-node
+// evaluations, at least one node per lane, at most 64.
+int nodes_per_item(int64_t prim_nodes)
+{
+    int64_t per_lane = (8192 + prim_nodes - 1) / prim_nodes;
+    per_lane = std::max<int64_t>(1, std::min<int64_t>(64, per_lane));
+    return (int)(64 * per_lane);
+}
+
+}  // namespace
+
+extern "C" ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coils *sec,
+                                    const int64_t *prim_index, const int64_t *sec_index, int64_t n_pairs,
+                                    const double *ring_offset, const int32_t *half_ring, uint32_t what,
+                                    int32_t precision, uint32_t flags, double *M, double *F, double *T,
+                                    ccoil_report *report)
+{
+    if (!ctx) return fail(nullptr, CCOIL_ERR_ARGUMENT, "ccoil_pairs: ctx is NULL");
+    if (n_pairs < 0) return fail(ctx, CCOIL_ERR_ARGUMENT, "n_pairs is negative");
+    if (n_pairs > INT32_MAX) return fail(ctx, CCOIL_ERR_ARGUMENT, "n_pairs exceeds 2^31 - 1");
+    if (what == 0 || (what & ~(uint32_t)(CCOIL_PAIR_M | CCOIL_PAIR_FT)))
+        return fail(ctx, CCOIL_ERR_ARGUMENT, "bad pair quantity mask");
+    if (precision != CCOIL_FP64 && precision != CCOIL_FP32C && precision != CCOIL_FP32 &&
+        precision != CCOIL_FP64_REFERENCE)
+        return fail(ctx, CCOIL_ERR_ARGUMENT, "unknown precision");
+    if ((what & CCOIL_PAIR_M) && !M) return fail(ctx, CCOIL_ERR_ARGUMENT, "M requested but M is NULL");
+    if ((what & CCOIL_PAIR_FT) && (!F || !T)) return fail(ctx, CCOIL_ERR_ARGUMENT, "F/T requested but F or T is NULL");
+    std::vector<CoilDev> hp, hs;
+    std::string msg = pack_coils(prim, false, false, hp);
+    if (msg.empty()) msg = pack_coils(sec, true, false, hs);
+    if (!msg.empty()) return fail(ctx, CCOIL_ERR_ARGUMENT, msg);
+    if (!prim->current || !prim->turns) return fail(ctx, CCOIL_ERR_ARGUMENT, "primary coils need current and turns");
+    if (report) std::memset(report, 0, sizeof *report);
+    if (n_pairs == 0) return CCOIL_OK;
+    if (hp.empty() || hs.empty()) return fail(ctx, CCOIL_ERR_ARGUMENT, "pairs given but a coil array is empty");
+
+    // Pair table, longest work item first (LPT order of the persistent queue).
+    std::vector<PairDev> pd((size_t)n_pairs);
+    int64_t node_pairs = 0;
+    const bool ref_scale = flags & CCOIL_FORCE_REFERENCE_SCALE;
+    for (int64_t k = 0; k < n_pairs; ++k) {
+        const int64_t pi = prim_index ? prim_index[k] : k, si = sec_index ? sec_index[k] : k;
+        if (pi < 0 || pi >= (int64_t)hp.size() || si < 0 || si >= (int64_t)hs.size())
+            return fail(ctx, CCOIL_ERR_ARGUMENT, "pair " + std::to_string(k) + ": coil index out of range");
+        const CoilDev &p = hp[(size_t)pi], &s = hs[(size_t)si];
+        PairDev d;
+        std::memset(&d, 0, sizeof d);
+        d.prim = (int32_t)pi;
+        d.sec = (int32_t)si;
+        d.out = (int32_t)k;
+        d.n_nodes = (int64_t)s.blk_ang * s.n_ang * s.blk_thk * s.n_thk * s.blk_len * s.n_len;
+        const int64_t pn = (int64_t)p.n_ang * p.n_thk;
+        d.nodes_per_item = nodes_per_item(pn);
+        d.ring_offset = ring_offset ? ring_offset[k] : 0.0;
+        d.half_ring = half_ring ? (half_ring[k] ? 1 : 0) : 0;
+        const double blocks = (double)s.blk_ang * s.blk_thk * s.blk_len;
+        if (p.current == 0.0 && (what & CCOIL_PAIR_M))
+            return fail(ctx, CCOIL_ERR_ARGUMENT, "pair " + std::to_string(k) + ": primary current is zero");
+        d.m_scale = 2.0 * kPi * s.turns / (p.current * blocks);
+        d.f_scale = ref_scale ? 2.0 * kPi * s.turns / (p.current * blocks) : 2.0 * kPi * s.turns * s.current / blocks;
+        node_pairs += d.n_nodes * pn;
+        pd[(size_t)k] = d;
+    }
+    std::vector<int64_t> cost((size_t)n_pairs);
+    for (int64_t k = 0; k < n_pairs; ++k) {
+        const CoilDev &p = hp[(size_t)pd[(size_t)k].prim];
+        cost[(size_t)k] = (int64_t)std::min<int64_t>(pd[(size_t)k].nodes_per_item, pd[(size_t)k].n_nodes) *
+                          p.n_ang * p.n_thk * (p.use_fast ? 2 : 1);
+    }
+    std::vector<int32_t> order((size_t)n_pairs);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return cost[(size_t)x] > cost[(size_t)y]; });
+    std::vector<PairDev> sorted((size_t)n_pairs);
+    // items are counted below; one workgroup per item must fit a 1-D grid
+    int64_t items = 0;
+    for (int64_t k = 0; k < n_pairs; ++k) {
+        PairDev d = pd[(size_t)order[(size_t)k]];
+        d.first_item = items;
+        items += (d.n_nodes + d.nodes_per_item - 1) / d.nodes_per_item;
+        sorted[(size_t)k] = d;
+    }
+
+    if (items > INT32_MAX)
+        return fail(ctx, CCOIL_ERR_ARGUMENT, "pair batch needs more than 2^31 - 1 work items; split the batch");
+    CC_HIP(ctx, hipSetDevice(ctx->device));
+    const bool dev = flags & CCOIL_DEVICE_POINTERS;
+    CC_HIP(ctx, ensure(ctx->pprim, sizeof(CoilDev) * hp.size()));
+    CC_HIP(ctx, ensure(ctx->psec, sizeof(CoilDev) * hs.size()));
+    CC_HIP(ctx, ensure(ctx->ppairs, sizeof(PairDev) * sorted.size()));
+    CC_HIP(ctx, ensure(ctx->ppartial, sizeof(double) * 7 * (size_t)items));
+    CC_HIP(ctx, hipMemcpyAsync(ctx->pprim.p, hp.data(), sizeof(CoilDev) * hp.size(), hipMemcpyHostToDevice, ctx->stream));
+    CC_HIP(ctx, hipMemcpyAsync(ctx->psec.p, hs.data(), sizeof(CoilDev) * hs.size(), hipMemcpyHostToDevice, ctx->stream));
+    CC_HIP(ctx, hipMemcpyAsync(ctx->ppairs.p, sorted.data(), sizeof(PairDev) * sorted.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+    PairParams pp;
+    std::memset(&pp, 0, sizeof pp);
+    pp.prim = (const CoilDev *)ctx->pprim.p;
+    pp.sec = (const CoilDev *)ctx->psec.p;
+    pp.pairs = (const PairDev *)ctx->ppairs.p;
+    pp.n_pairs = (int32_t)n_pairs;
+    pp.n_items = items;
+    pp.partial = (double *)ctx->ppartial.p;
+    pp.what = what;
+    pp.precision = precision;
+    pp.tab = ctx_tables(ctx);
+    const size_t s1 = sizeof(double) * (size_t)n_pairs, s3 = 3 * s1;
+    if (dev) {
+        pp.M = M;
+        pp.F = F;
+        pp.T = T;
+    } else {
+        if (what & CCOIL_PAIR_M) { CC_HIP(ctx, ensure(ctx->pM, s1)); pp.M = (double *)ctx->pM.p; }
+        if (what & CCOIL_PAIR_FT) {
+            CC_HIP(ctx, ensure(ctx->pF, s3));
+            CC_HIP(ctx, ensure(ctx->pT, s3));
+            pp.F = (double *)ctx->pF.p;
+            pp.T = (double *)ctx->pT.p;
+        }
+    }
+    CC_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    launch_pairs(ctx->stream, pp);
+    CC_HIP(ctx, hipGetLastError());
+    launch_pair_reduce(ctx->stream, pp);
+    CC_HIP(ctx, hipGetLastError());
+    CC_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    if (!dev) {
+        if (what & CCOIL_PAIR_M) CC_HIP(ctx, hipMemcpyAsync(M, pp.M, s1, hipMemcpyDeviceToHost, ctx->stream));
+        if (what & CCOIL_PAIR_FT) {
+            CC_HIP(ctx, hipMemcpyAsync(F, pp.F, s3, hipMemcpyDeviceToHost, ctx->stream));
+            CC_HIP(ctx, hipMemcpyAsync(T, pp.T, s3, hipMemcpyDeviceToHost, ctx->stream));
+        }
+    }
+    CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (report) {
+        float ms = 0.f;
+        CC_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        report->node_evaluations = node_pairs;
+        report->kernel_ms = ms;
+        report->block_size = 64;
+        report->split = 1;
+    }
+    return CCOIL_OK;
+}

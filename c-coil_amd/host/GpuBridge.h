@@ -1,0 +1,36 @@
+// Bridge from the C++ Coil API to the C ABI: per-thread default context, status -> exception mapping,
+// and SoA packing of Coil objects into ccoil_coils.
+#pragma once
+
+#include "Coil.h"
+#include "ccoil_hip.h"
+
+#include <vector>
+
+namespace ccoil {
+namespace bridge {
+
+// The calling thread's context on device CCOIL_DEVICE (default 0), created on first use.
+ccoil_ctx context();
+
+// Maps a non-OK status to the exception the reference raises in the same situation:
+// CCOIL_ERR_ARGUMENT -> std::invalid_argument, CCOIL_ERR_NO_DEVICE -> std::logic_error,
+// CCOIL_ERR_OUT_OF_MEMORY -> std::bad_alloc, CCOIL_ERR_DEVICE -> std::runtime_error.
+void check(ccoil_status st, ccoil_ctx ctx, const char *what);
+
+int precision();
+void setPrecision(int precision);
+
+struct CoilSoA {
+    std::vector<int32_t> use_fast, n_ang, n_thk, n_len, blocks;
+    std::vector<double> R, a, b, J, current, turns, freq, pos, alpha, beta;
+
+    void add(const Coil &c, const PrecisionArguments &args);
+    void add(const Coil &c, const PrecisionArguments &args, const vec3::Vector3 &position, double yAngle,
+             double zAngle);
+    [[nodiscard]] size_t size() const { return R.size(); }
+    [[nodiscard]] ccoil_coils view() const;
+};
+
+}  // namespace bridge
+}  // namespace ccoil

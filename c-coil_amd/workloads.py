@@ -1,0 +1,125 @@
+"""Synthetic workloads of the benchmark configurations, and packing of coils for the C ABI.
+
+Distributions (numpy PCG64, float64, drawn in the order written):
+  single RECTANGULAR coil    R = a = b = 0.1 m, N = 100, I = 1 A at the origin; points uniform in the
+                             cylinder r <= 0.3 m, |z| <= 0.3 m (r = 0.3 sqrt(u0), phi = 2 pi u1, z = 0.6 u2 - 0.3)
+  mixed coil group           type ~ U{FILAMENT, THIN, FLAT, RECTANGULAR}; R ~ U[0.02, 0.5]; a ~ U[0.005, 0.2] for
+                             FLAT/RECTANGULAR else 0; b ~ U[0.005, 0.2] for THIN/RECTANGULAR else 0;
+                             N ~ U{1..2000}; I ~ U[-10, 10]; centre ~ U[-1, 1]^3; yaw ~ U[0, 2 pi); pitch ~ U[0, pi]
+                             points ~ U[-1.5, 1.5]^3
+  uneven pairs               both coils of a pair drawn as in the mixed group; per active quadrature dimension
+                             increments ~ U{4, 8, ..., 80}
+Planning (PrecisionArguments) comes from the c_coil module, i.e. the same C++ code the drop-in API uses.
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+
+import ccoil_abi  # noqa: E402
+
+FILAMENT, THIN, FLAT, RECTANGULAR = "FILAMENT", "THIN", "FLAT", "RECTANGULAR"
+TYPES = (FILAMENT, THIN, FLAT, RECTANGULAR)
+
+
+def cylinder_points(rng, n, r_max=0.3, z_half=0.3):
+    u = rng.random((n, 3))
+    r = r_max * np.sqrt(u[:, 0])
+    phi = 2 * np.pi * u[:, 1]
+    z = 2 * z_half * u[:, 2] - z_half
+    return np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)
+
+
+def cube_points(rng, n, half=1.5):
+    return rng.uniform(-half, half, size=(n, 3))
+
+
+def random_coil_params(rng, n):
+    """Mixed-type coils; returns a list of dicts (R, a, b, N, I, pos, alpha, beta, type)."""
+    out = []
+    for _ in range(n):
+        t = TYPES[rng.integers(4)]
+        R = rng.uniform(0.02, 0.5)
+        a = rng.uniform(0.005, 0.2) if t in (FLAT, RECTANGULAR) else 0.0
+        b = rng.uniform(0.005, 0.2) if t in (THIN, RECTANGULAR) else 0.0
+        N = int(rng.integers(1, 2001))
+        I = rng.uniform(-10, 10)
+        pos = rng.uniform(-1, 1, size=3)
+        beta = rng.uniform(0, 2 * np.pi)   # zAxisAngle (yaw)
+        alpha = rng.uniform(0, np.pi)      # yAxisAngle (pitch)
+        out.append(dict(type=t, R=R, a=a, b=b, N=N, I=I, pos=pos, alpha=alpha, beta=beta))
+    return out
+
+
+def _c_coil():
+    import c_coil
+    return c_coil
+
+
+def make_coil(p, pf=5.0):
+    c = _c_coil()
+    V = c.tensor.Vector3
+    return c.coil.Coil(p["R"], p["a"], p["b"], int(p["N"]), p.get("I", 1.0), p.get("freq", 50.0),
+                       c.coil.PrecisionFactor(pf), 8, V(*[float(x) for x in p["pos"]]), p["alpha"], p["beta"])
+
+
+def coil_row(coil, args=None, pos=None, alpha=None, beta=None):
+    """One C ABI row from a c_coil Coil and its PrecisionArguments (GPU defaults if None)."""
+    if args is None:
+        args = coil.get_precision_settings_GPU()
+    P = coil.get_position_vector()
+    ang = coil.get_rotation_angles()
+    return dict(use_fast=int(coil.is_using_fast_method()), n_ang=args.angular_increment_count,
+                n_thk=args.thickness_increment_count, n_len=args.length_increment_count,
+                blocks=(args.angular_block_count, args.thickness_block_count, args.length_block_count),
+                R=coil.get_inner_radius(), a=coil.get_thickness(), b=coil.get_length(), J=coil.get_current_density(),
+                current=coil.get_current(), turns=float(coil.get_num_of_turns()), frequency=coil.get_sine_frequency(),
+                pos=(P.x, P.y, P.z) if pos is None else tuple(pos), alpha=ang[0] if alpha is None else alpha,
+                beta=ang[1] if beta is None else beta)
+
+
+def coilset(coils, pf=5.0):
+    """CoilSet of c_coil Coils at their GPU PrecisionArguments for the given group PrecisionFactor."""
+    c = _c_coil()
+    rows = [coil_row(k, c.coil.PrecisionArguments.get_coil_precision_arguments_GPU(k, c.coil.PrecisionFactor(pf)))
+            for k in coils]
+    return ccoil_abi.CoilSet.from_rows(rows)
+
+
+def single_rect_workload(n_points=65536, seed=0):
+    rng = np.random.default_rng(seed)
+    pts = cylinder_points(rng, n_points)
+    params = [dict(type=RECTANGULAR, R=0.1, a=0.1, b=0.1, N=100, I=1.0, pos=np.zeros(3), alpha=0.0, beta=0.0)]
+    return params, pts
+
+
+def group_workload(n_coils=128, n_points=1 << 20, seed=0):
+    rng = np.random.default_rng(seed)
+    params = random_coil_params(rng, n_coils)
+    pts = cube_points(rng, n_points)
+    return params, pts
+
+
+def pair_workload(n_pairs=65536, seed=0):
+    """Rows for ccoil_pairs: every pair its own primary and secondary, increments ~ U{4,...,80} per active
+    dimension (primary: angular, thickness; secondary: angular, thickness, length)."""
+    rng = np.random.default_rng(seed)
+    prim = random_coil_params(rng, n_pairs)
+    sec = random_coil_params(rng, n_pairs)
+    incs = lambda: int(4 * rng.integers(1, 21))
+    prim_rows, sec_rows = [], []
+    for p, s in zip(prim, sec):
+        cp, cs = make_coil(p), make_coil(s)
+        pa = (incs(), incs() if p["type"] in (FLAT, RECTANGULAR) else 1, 1)
+        sa = (incs(), incs() if s["type"] in (FLAT, RECTANGULAR) else 1, incs() if s["type"] in (THIN, RECTANGULAR) else 1)
+        rp = coil_row(cp)
+        rp.update(n_ang=pa[0], n_thk=pa[1], n_len=pa[2], blocks=(1, 1, 1))
+        rs = coil_row(cs)
+        rs.update(n_ang=sa[0], n_thk=sa[1], n_len=sa[2], blocks=(1, 1, 1))
+        prim_rows.append(rp)
+        sec_rows.append(rs)
+    return prim_rows, sec_rows

@@ -1,0 +1,140 @@
+/*
+ * ccoil_hip.h -- C ABI of the MI355X (gfx950) compute path for C-Coil.
+ *
+ * This is the drop-in boundary that replaces C-Coil's CUDA kernel interface:
+ *   - Calculate_hardware_accelerated_{a,b,g}        src/Coil/CUDAKernels/CoilAcceleration.h:8-18
+ *   - Calculate_hardware_accelerated_{a,b,e,g}_group src/CoilGroup/CUDAKernels/CoilGroupAcceleration.h:7-21
+ *   - Calculate_{mutual_inductance,force_and_torque}_configurations
+ *                                                   src/Coil/CUDAKernels/CoilAcceleration.h:21-29
+ *   - Calculate_{mutual_inductance,force_and_torque}_configurations_group
+ *                                                   src/CoilGroup/CUDAKernels/CoilGroupAcceleration.h:23-37
+ * Differences from that interface, all deliberate:
+ *   - C linkage, plain pointers and sizes, status codes (the reference exits the process on any
+ *     device error, src/CUDAUtils/ErrorCheck/ErrorCheck.cu:5-17);
+ *   - a context owns the device, stream and buffers (the reference keeps process globals,
+ *     src/CUDAUtils/MemoryManagement/GPUMemoryManagement.cu:10-13);
+ *   - coils are passed as SoA FP64 physical attributes; the library derives the per-coil constants
+ *     (constFactor, transformation matrices) exactly as generateCoilData does
+ *     (src/Coil/Utils/SupportFunctions.cxx:167-223) and generates the Gauss-Legendre tables on the device;
+ *   - one call evaluates all coils and every requested quantity (A, B, E, dB) in one launch.
+ *
+ * Threading: a context may be used by one thread at a time. Calls on one context are ordered on its stream.
+ */
+#ifndef CCOIL_HIP_H
+#define CCOIL_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ccoil_ctx_s *ccoil_ctx;
+typedef int32_t ccoil_status;
+
+enum {
+    CCOIL_OK = 0,
+    CCOIL_ERR_ARGUMENT = -1,      /* invalid sizes, pointers or precision values; nothing was launched */
+    CCOIL_ERR_OUT_OF_MEMORY = -2, /* device or host allocation failed (reference: std::bad_alloc) */
+    CCOIL_ERR_DEVICE = -3,        /* a HIP runtime call failed; see ccoil_last_error */
+    CCOIL_ERR_NO_DEVICE = -4      /* no HIP device available (reference: "GPU functions are disabled") */
+};
+
+/* Quantity mask for ccoil_fields. */
+enum {
+    CCOIL_A = 1,          /* vector potential A */
+    CCOIL_B = 2,          /* flux density B */
+    CCOIL_E = 4,          /* E = 2*pi*f*A per coil (ComputeFieldsMethods.cxx:27-32, FieldEGroup.cu:105-107) */
+    CCOIL_G = 8,          /* gradient in the reference convention T*G_local (CalculateFieldsMethods.cxx:83) */
+    CCOIL_G_JACOBIAN = 16 /* true Jacobian dB_i/dx_j = T*G_local*T^T (no reference counterpart) */
+};
+
+/* Arithmetic of the quadrature. */
+enum {
+    CCOIL_FP64 = 0,           /* cancellation-free integrands in double: the accurate mode */
+    CCOIL_FP32C = 1,          /* the same integrands with FP32 per-node arithmetic, FP64 per-row work and row
+                                 accumulation, FP64 re-evaluation of near-winding (coil, point) pairs */
+    CCOIL_FP32 = 2,           /* the reference's integrands in float (what a TYPE=float build computes) */
+    CCOIL_FP64_REFERENCE = 3  /* the reference's integrands verbatim in double (a TYPE=double build): the
+                                 parity mode and the plain-FP64 throughput comparator; its gradient terms
+                                 lose up to ~3e-5 relative to cancellation near the winding */
+};
+
+/* Flags. */
+enum {
+    CCOIL_DEVICE_POINTERS = 1, /* points/outputs are device pointers on the context's device */
+    CCOIL_NO_NEAR_FP64 = 2,    /* FP32C: do not re-evaluate near-winding pairs in FP64 */
+    CCOIL_FORCE_REFERENCE_SCALE = 4 /* ccoil_pairs: force/torque scaled by 2*pi*N2/I1 as the reference's
+                                       single-pair arrangement path does (CalculateForceArrangements.cxx:125,
+                                       SupportFunctions.cxx:238-241); default is the physical 2*pi*N2*I2 */
+};
+
+/* Pair quantity mask for ccoil_pairs. */
+enum { CCOIL_PAIR_M = 1, CCOIL_PAIR_FT = 2 };
+
+/* Coils, SoA, host memory, FP64. Lengths are n. Increments are the already-planned quadrature orders
+ * (1..100) from PrecisionArguments; blocks apply to pair secondaries only. */
+typedef struct {
+    int64_t n;
+    const int32_t *use_fast;   /* 1: THIN/RECTANGULAR (analytic length integral), 0: FLAT/FILAMENT */
+    const int32_t *n_ang;      /* angular increments */
+    const int32_t *n_thk;      /* thickness increments */
+    const int32_t *n_len;      /* length increments (pair secondaries; NULL for fields) */
+    const int32_t *blocks;     /* 3 per coil: angular, thickness, length blocks (pair secondaries; NULL = 1) */
+    const double *R;           /* inner radius */
+    const double *a;           /* thickness after the thin clamp (Coil.cxx:27-28) */
+    const double *b;           /* length after the thin clamp */
+    const double *J;           /* current density N*I/(a*b) */
+    const double *current;     /* I */
+    const double *turns;       /* N */
+    const double *frequency;   /* sine frequency for E (NULL: E not requested) */
+    const double *pos;         /* 3 per coil */
+    const double *alpha;       /* rotation about y (yAxisAngle) */
+    const double *beta;        /* rotation about z (zAxisAngle) */
+} ccoil_coils;
+
+/* Per-call report (any pointer argument may be NULL). */
+typedef struct {
+    int64_t node_evaluations;  /* sum over (coil, point) of quadrature nodes, or (pair) node-pairs */
+    int64_t near_pairs;        /* FP32C: (coil, point) pairs re-evaluated in FP64 */
+    double kernel_ms;          /* device time of the compute kernels of this call (hipEvent) */
+    int32_t block_size;        /* threads per workgroup of the main kernel */
+    int32_t split;             /* lanes cooperating on one point (field) */
+} ccoil_report;
+
+/* Context lifecycle. ccoil_last_error(NULL) returns the last error of ccoil_create on this thread. */
+ccoil_status ccoil_create(int device, ccoil_ctx *out);
+ccoil_status ccoil_destroy(ccoil_ctx ctx);
+const char *ccoil_last_error(ccoil_ctx ctx);
+const char *ccoil_version(void);
+/* Use an external stream (hipStream_t cast to void*); NULL restores the context's own stream. */
+ccoil_status ccoil_set_stream(ccoil_ctx ctx, void *stream);
+ccoil_status ccoil_synchronize(ccoil_ctx ctx);
+
+/* Gauss-Legendre rule of order n (1..100) as generated in FP64 on the device; replaces the host table
+ * src/LegendreMatrix/LegendreMatrix.h:9-10. x, w: host arrays of n doubles, ascending nodes. */
+ccoil_status ccoil_gauss_legendre(ccoil_ctx ctx, int32_t n, double *x, double *w);
+
+/* Fields of all coils summed at n_points points (AoS xyz). Outputs AoS: A, B, E are 3 per point, G is 9
+ * per point row-major; each requested output is overwritten. Replaces the per-coil launch loops of
+ * FieldBGroup.cu:142-208, FieldGGroup.cu:212-278, FieldAGroup.cu, FieldEGroup.cu and the single-coil
+ * FieldA/B/G.cu launchers (a one-coil ccoil_coils). */
+ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, const double *points, int64_t n_points,
+                          uint32_t quantities, int32_t precision, uint32_t flags,
+                          double *A, double *B, double *E, double *G, ccoil_report *report);
+
+/* Mutual inductance M and force/torque (F, T about the secondary centre) for n_pairs (primary, secondary)
+ * pairs, each pair in its own pose. prim_index/sec_index select coils of prim/sec (NULL: identity).
+ * ring_offset/half_ring (per pair, NULL: 0) reproduce the coplanar improved-precision rule of
+ * Coil::improvedPrecisionCase (SupportFunctions.cxx:124-161). M is 1 per pair, F and T 3 per pair.
+ * Replaces MInductance.cu:184-248, Force.cu:206-270 and the *Group.cu launchers. */
+ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coils *sec,
+                         const int64_t *prim_index, const int64_t *sec_index, int64_t n_pairs,
+                         const double *ring_offset, const int32_t *half_ring,
+                         uint32_t what, int32_t precision, uint32_t flags,
+                         double *M, double *F, double *T, ccoil_report *report);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

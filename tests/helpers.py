@@ -1,0 +1,54 @@
+"""Shared test helpers: coil construction for product and oracle from one parameter dict, error metrics."""
+import numpy as np
+
+import oracle as O
+import workloads as W
+
+ORACLE_TYPE = {"RECTANGULAR": O.RECTANGULAR, "THIN": O.THIN, "FLAT": O.FLAT, "FILAMENT": O.FILAMENT}
+
+
+def oracle_coil(p):
+    return O.make_coil(p["R"], p["a"], p["b"], int(p["N"]), p.get("I", 1.0), p.get("freq", 50.0),
+                       pos=tuple(float(x) for x in p["pos"]), alpha=p["alpha"], beta=p["beta"])
+
+
+def gpu_args_rows(params, pf=5.0):
+    """Product planning (c_coil) -> C ABI rows, and the same args as oracle int32[6]."""
+    import c_coil
+    rows, args = [], []
+    for p in params:
+        c = W.make_coil(p, pf)
+        a = c_coil.coil.PrecisionArguments.get_coil_precision_arguments_GPU(c, c_coil.coil.PrecisionFactor(pf))
+        rows.append(W.coil_row(c, a))
+        # The GPU kernels use one block per layer; the oracle integrates the same rule.
+        args.append(np.array([1, 1, 1, a.angular_increment_count, a.thickness_increment_count,
+                              a.length_increment_count], dtype=np.int32))
+    return rows, args
+
+
+def rel_err_vec(x, ref, floor=0.0):
+    """Per-point vector-norm relative error |x - ref| / |ref| (rows), points with |ref| <= floor get NaN."""
+    x = np.asarray(x).reshape(len(ref), -1)
+    ref = np.asarray(ref).reshape(len(ref), -1)
+    den = np.linalg.norm(ref, axis=1)
+    num = np.linalg.norm(x - ref, axis=1)
+    out = np.full(len(ref), np.nan)
+    ok = den > floor
+    out[ok] = num[ok] / den[ok]
+    return out
+
+
+def near_mask(params, pts, width_floor=1e-3):
+    """Points within one cross-section width (max(a, b), floored at 1e-3 R) of any coil's winding."""
+    near = np.zeros(len(pts), dtype=bool)
+    for p in params:
+        c = oracle_coil(p)
+        T = np.array(c.Tinv).reshape(3, 3)
+        l = (pts - np.array(p["pos"])) @ T.T
+        r = np.hypot(l[:, 0], l[:, 1])
+        z = l[:, 2]
+        dr = np.maximum(np.maximum(c.R - r, r - (c.R + c.a)), 0.0)
+        dz = np.maximum(np.abs(z) - 0.5 * c.b, 0.0)
+        w = max(c.a, c.b, width_floor * c.R)
+        near |= dr * dr + dz * dz < w * w
+    return near
