@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X C-Coil field path, one JSON line on stdout.
+
+Timed workload (one step = one pass of the hot path over the whole batch, inputs resident in HBM):
+  B and the gradient tensor of a 128-coil mixed CoilGroup (FILAMENT/THIN/FLAT/RECTANGULAR, R ~ U[0.02, 0.5] m,
+  a, b ~ U[0.005, 0.2] m, N ~ U{1..2000}, I ~ U[-10, 10] A, centres ~ U[-1, 1]^3 m, random orientation) at
+  2^20 points ~ U[-1.5, 1.5]^3 m, PrecisionFactor 5, FP32C precision (FP32 nodes, FP64 row sums, FP64
+  near-winding re-evaluation), through ccoil_fields with device pointers.
+value = coil·point evaluations of B + gradient per second (coils x points / seconds per step).
+
+Beside it, untimed by the contract and reported in the same line:
+  fp64        the same pass in CCOIL_FP64 (stable integrands in double) and CCOIL_FP64_REFERENCE (the
+              reference's integrands in double), and the FP32C speed-up over each;
+  accuracy    FP32C against CCOIL_FP64 at every point: max and p99 of the per-point relative error of B and of
+              the gradient, split into points inside the near-winding shell of any coil and the rest;
+  pairs       mutual inductance + force + torque of 65,536 random coil pairs with per-dimension increments
+              ~ U{4, 8, ..., 80} (one ccoil_pairs call), FP32C and FP64, pairs/s and accuracy;
+  roofline    the field kernel's algorithmic FP32 flops (frozen per-node model, roofline_model.py) per launch
+              over its hipEvent-timed duration on the library's stream, against the MI355X FP32 vector peak;
+  cpu_baseline the FP64 CPU restatement of the reference's CPU_MT field path (-O3 -mavx2 -ffast-math, point
+              chunks on a thread pool) on a bounded sample of the same workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "c-coil_amd")
+if PKG not in sys.path:
+    sys.path.insert(0, PKG)
+
+METRIC = "coil·point B+∇B evals/s and M-pairs/s, FP32-comp vs FP64 at ≤1e-6 rel err"
+PMC_FILE = os.path.join(ROOT, "profiles", "field_kernel_group128_pmc.txt")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rel_err(x, ref):
+    x = np.asarray(x).reshape(len(ref), -1)
+    ref = np.asarray(ref).reshape(len(ref), -1)
+    den = np.linalg.norm(ref, axis=1)
+    num = np.linalg.norm(x - ref, axis=1)
+    ok = den > 0
+    return num[ok] / den[ok], ok
+
+
+def stats(e):
+    if e.size == 0:
+        return {"max": None, "p99": None, "n": 0}
+    return {"max": float(np.max(e)), "p99": float(np.percentile(e, 99)), "n": int(e.size)}
+
+
+def pmc_traffic():
+    """HBM bytes per launch of the field kernel from the committed rocprofv3 --pmc summary, if present."""
+    if not os.path.exists(PMC_FILE):
+        return None
+    with open(PMC_FILE) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch"):
+                return float(line.split("=")[1].split()[0])
+    return None
+
+
+def cpu_baseline(params, pts, rows, n_sample):
+    """FP64 CPU restatement of the reference's CPU_MT path, timed on this host (bounded sample)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = min(16, len(os.sched_getaffinity(0)))
+    coils = [O.make_coil(p["R"], p["a"], p["b"], int(p["N"]), p["I"], 50.0, tuple(float(x) for x in p["pos"]),
+                         p["alpha"], p["beta"]) for p in params]
+    args = [np.array([1, 1, 1, r["n_ang"], r["n_thk"], r["n_len"]], dtype=np.int32) for r in rows]
+    sample = np.ascontiguousarray(pts[:n_sample])
+    O.group_fields(coils[:2], args[:2], sample[:256], False, True, True, axis_ratio=O.AXIS_RATIO_GPU,
+                   threads=threads, fast=True)
+    t0 = time.perf_counter()
+    O.group_fields(coils, args, sample, False, True, True, axis_ratio=O.AXIS_RATIO_GPU, threads=threads, fast=True)
+    dt = time.perf_counter() - t0
+    return {"value": len(coils) * len(sample) / dt, "unit": "coil·point B+∇B evals/s", "cores": threads,
+            "kind": "port", "seconds": dt,
+            "sample": f"all {len(coils)} coils x the first {len(sample)} points of the timed workload, FP64, "
+                      f"oracle/liboracle_fast.so (-O3 -mavx2 -ffast-math), point chunks on {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--coils", type=int, default=128)
+    ap.add_argument("--points", type=int, default=1 << 20)
+    ap.add_argument("--pairs", type=int, default=65536)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--cpu-points", type=int, default=65536)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 legs and the accuracy report")
+    ap.add_argument("--no-pairs", action="store_true", help="skip the pair leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        log("bench.py measures one GPU: the field and pair paths shard by points / pairs with no exchange, and "
+            "multi-GPU runs are not part of this build. Run with --gpus 1.")
+        sys.exit(2)
+
+    import torch
+    import ccoil_abi as abi
+    import roofline_model as RM
+    import workloads as W
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    params, pts = W.group_workload(args.coils, args.points, seed=0)
+    rows = W.group_rows(params)
+    cs = abi.CoilSet.from_rows(rows)
+    n = len(pts)
+    log(f"workload: {len(rows)} coils x {n} points, mean nodes/coil "
+        f"{np.mean([r['n_ang'] * r['n_thk'] for r in rows]):.1f}")
+
+    ctx = abi.Context(0)
+    stream = torch.cuda.current_stream(dev)
+    ctx.set_stream(stream.cuda_stream)
+    d_pts = torch.from_numpy(np.ascontiguousarray(pts)).to(dev)
+    out = {"B": torch.empty((n, 3), dtype=torch.float64, device=dev),
+           "G": torch.empty((n, 9), dtype=torch.float64, device=dev)}
+    q = abi.B | abi.G
+
+    def step(prec, o=out):
+        _, rep = ctx.fields(cs, d_pts, q, prec, abi.DEVICE_POINTERS, out=o)
+        return rep
+
+    for _ in range(args.warmup):
+        step(abi.FP32C)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = [step(abi.FP32C) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = len(rows) * n / ((t1 - t0) / args.steps)
+    main_ms = float(np.mean([r.main_kernel_ms for r in reps]))
+    all_ms = float(np.mean([r.kernel_ms for r in reps]))
+    log(f"FP32C: {ms_per_step:.2f} ms/step, field kernel {main_ms:.2f} ms, all kernels {all_ms:.2f} ms, "
+        f"near pairs {reps[-1].near_pairs}")
+
+    flops, trans = RM.field_work(rows, n, with_gradient=True)
+    achieved = flops / (main_ms * 1e-3) / 1e12
+    roofline = {"bound": "valu", "achieved": achieved, "peak": RM.PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / RM.PEAK_FP32_TFLOPS, "traffic": pmc_traffic(), "kernel": "field_kernel",
+                "kernel_ms": main_ms, "algorithmic_flops_per_launch": flops,
+                "transcendental": {"achieved_ops": trans / (main_ms * 1e-3), "peak_ops": RM.PEAK_TRANS_OPS,
+                                   "frac": trans / (main_ms * 1e-3) / RM.PEAK_TRANS_OPS},
+                "hbm_algorithmic_bytes_per_launch": n * (24 + 24 + 72)}
+
+    result = {"metric": METRIC, "value": value, "unit": "coil·point evals/s", "n_gpus": 1,
+              "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+              "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+              "config": {"workload": f"B+gradient of a {len(rows)}-coil mixed CoilGroup at {n} points, FP32C",
+                         "coils": len(rows), "points": n, "precision_factor": 5.0, "quantities": "B+G",
+                         "precision": "FP32C", "seed": 0},
+              "roofline": roofline, "kernel_ms": {"field_kernel": main_ms, "all": all_ms},
+              "near_pairs": int(reps[-1].near_pairs)}
+
+    if not args.no_fp64:
+        ref = {"B": torch.empty_like(out["B"]), "G": torch.empty_like(out["G"])}
+        fp64 = {}
+        # CCOIL_FP64 last, so that ref holds its output for the accuracy report; out still holds FP32C
+        for name, prec in (("fp64_reference", abi.FP64_REFERENCE), ("fp64", abi.FP64)):
+            step(prec, ref)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r64 = [step(prec, ref) for _ in range(args.fp64_steps)]
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / args.fp64_steps
+            fp64[name] = {"value": len(rows) * n / dt, "ms_per_step": dt * 1e3,
+                          "kernel_ms": float(np.mean([r.main_kernel_ms for r in r64]))}
+            log(f"{name}: {dt * 1e3:.1f} ms/step")
+        fp64["fp32c_speedup_vs_fp64"] = value / fp64["fp64"]["value"]
+        fp64["fp32c_speedup_vs_fp64_reference"] = value / fp64["fp64_reference"]["value"]
+        result["fp64"] = fp64
+        b32, g32 = out["B"].cpu().numpy(), out["G"].cpu().numpy()
+        b64, g64 = ref["B"].cpu().numpy(), ref["G"].cpu().numpy()
+        near = W.near_mask(rows, pts)
+        eb, okb = rel_err(b32, b64)
+        eg, okg = rel_err(g32, g64)
+        result["accuracy"] = {
+            "against": "CCOIL_FP64 on the same points", "near_points": int(near.sum()),
+            "B": {"far": stats(eb[~near[okb]]), "near": stats(eb[near[okb]])},
+            "G": {"far": stats(eg[~near[okg]]), "near": stats(eg[near[okg]])}}
+        log(f"accuracy: {json.dumps(result['accuracy'])}")
+
+    if not args.no_pairs:
+        prow, srow = W.pair_workload(args.pairs, seed=0)
+        P, S = abi.CoilSet.from_rows(prow), abi.CoilSet.from_rows(srow)
+        what = abi.PAIR_M | abi.PAIR_FT
+        pf, ptrans, pnodes = RM.pair_work(prow, srow)
+        pres = {}
+        for name, prec, steps in (("fp32c", abi.FP32C, 2), ("fp64", abi.FP64, 1)):
+            o, rep = ctx.pairs(P, S, len(prow), what, prec)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                o, rep = ctx.pairs(P, S, len(prow), what, prec)
+            dt = (time.perf_counter() - t0) / steps
+            pres[name] = {"pairs_per_s": len(prow) / dt, "ms_per_step": dt * 1e3, "kernel_ms": rep.main_kernel_ms,
+                          "node_pairs_per_s": pnodes / (rep.main_kernel_ms * 1e-3),
+                          "tflops_algorithmic": pf / (rep.main_kernel_ms * 1e-3) / 1e12, "out": o}
+            log(f"pairs {name}: {dt * 1e3:.1f} ms/step, kernel {rep.main_kernel_ms:.1f} ms")
+        o32, o64 = pres["fp32c"].pop("out"), pres["fp64"].pop("out")
+        em = np.abs(o32["M"] - o64["M"]) / np.abs(o64["M"])
+        ef, _ = rel_err(o32["F"], o64["F"])
+        et, _ = rel_err(o32["T"], o64["T"])
+        pres["accuracy_fp32c_vs_fp64"] = {"M": stats(em), "F": stats(ef), "T": stats(et)}
+        pres["workload"] = (f"{len(prow)} random mixed pairs, increments ~ U{{4,...,80}} per active dimension, "
+                            f"{pnodes:.3g} node-pairs, M+F+T")
+        pres["fp32c_speedup_vs_fp64"] = pres["fp32c"]["pairs_per_s"] / pres["fp64"]["pairs_per_s"]
+        result["pairs"] = pres
+
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(params, pts, rows, min(args.cpu_points, n))
+        log(f"cpu baseline: {result['cpu_baseline']['value']:.3g} coil-points/s")
+    else:
+        result["cpu_baseline"] = None
+    ctx.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -166,6 +166,8 @@ ccoil_status ccoil_create(int device, ccoil_ctx *out)
     ctx->stream = ctx->own_stream;
     if ((e = hipEventCreate(&ctx->ev0)) != hipSuccess) return bail(e, "hipEventCreate");
     if ((e = hipEventCreate(&ctx->ev1)) != hipSuccess) return bail(e, "hipEventCreate");
+    if ((e = hipEventCreate(&ctx->evm0)) != hipSuccess) return bail(e, "hipEventCreate");
+    if ((e = hipEventCreate(&ctx->evm1)) != hipSuccess) return bail(e, "hipEventCreate");
     const size_t nd = sizeof(double) * kTableSize, nf = sizeof(float) * kTableSize;
     double **dp[6] = {&ctx->gl_x, &ctx->gl_w, &ctx->gl_cos, &ctx->gl_omc, &ctx->gl_rcos, &ctx->gl_rsin};
     float **fp[4] = {&ctx->gl_xf, &ctx->gl_wf, &ctx->gl_cosf, &ctx->gl_omcf};
@@ -198,6 +200,8 @@ ccoil_status ccoil_destroy(ccoil_ctx ctx)
     for (auto b : bufs) free_buf(*b);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->evm0) (void)hipEventDestroy(ctx->evm0);
+    if (ctx->evm1) (void)hipEventDestroy(ctx->evm1);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return CCOIL_OK;
@@ -327,8 +331,10 @@ ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, const double 
             cap = near + 4096;
         }
     }
+    CC_HIP(ctx, hipEventRecord(ctx->evm0, ctx->stream));
     launch_field(ctx->stream, p, precision, split);
     CC_HIP(ctx, hipGetLastError());
+    CC_HIP(ctx, hipEventRecord(ctx->evm1, ctx->stream));
     if (near_fp64 && near > 0) {
         launch_near_eval(ctx->stream, p, near);
         CC_HIP(ctx, hipGetLastError());
@@ -342,13 +348,15 @@ ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, const double 
     }
     CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (report) {
-        float ms = 0.f;
+        float ms = 0.f, mms = 0.f;
         CC_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        CC_HIP(ctx, hipEventElapsedTime(&mms, ctx->evm0, ctx->evm1));
         int64_t nodes = 0;
         for (const auto &c : hc) nodes += (int64_t)c.n_ang * c.n_thk;
         report->node_evaluations = nodes * n;
         report->near_pairs = near;
         report->kernel_ms = ms;
+        report->main_kernel_ms = mms;
         report->block_size = kFieldBlock;
         report->split = split;
     }

@@ -36,7 +36,8 @@ struct ccoil_ctx_s {
     double *gl_rcos = nullptr, *gl_rsin = nullptr;
     ccoil::DevBuf coils, pts, outA, outB, outE, outG, queue, counter;
     ccoil::DevBuf pprim, psec, ppairs, ppartial, pcounter, pM, pF, pT;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;   // around all compute kernels of a call
+    hipEvent_t evm0 = nullptr, evm1 = nullptr; // around the main kernel (field_kernel / pair_kernel)
 };
 
 namespace ccoil {

@@ -225,10 +225,8 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                     const T invM = same ? gs * Rr : Rr;
                     const T dgM = same ? tP2 * (M * Rr) : (g1 - g2) * Rr;   // (z1 K1 - z2 K2) / M
                     const T rmc = fma_<T>(rf, s, dm);                       // rho - r cos
-                    if (WB) {
-                        rBh = fma_<T>(w, -cp * dK, rBh);
-                        rBz = fma_<T>(w, rmc * dgM, rBz);
-                    }
+                    rBh = fma_<T>(w, -cp * dK, rBh);   // B_rho, also the gradient's rho-phi term
+                    if (WB) rBz = fma_<T>(w, rmc * dgM, rBz);
                     if (WG) {
                         const T K1s = K1 * K1, K2s = K2 * K2;
                         const T dL = dK * (K1s + K12 + K2s);                 // K1^3 - K2^3
@@ -292,10 +290,8 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                     const T K2 = K * K;
                     const T K3 = K2 * K;
                     const T rmc = fma_<T>(rf, s, dm);     // rho - r cos
-                    if (WB) {
-                        rBh = fma_<T>(w, cp * K3, rBh);
-                        rBz = fma_<T>(w, rmc * K3, rBz);
-                    }
+                    rBh = fma_<T>(w, cp * K3, rBh);    // B_rho, also the gradient's rho-phi term
+                    if (WB) rBz = fma_<T>(w, rmc * K3, rBz);
                     if (WG) {
                         const T K5 = K3 * K2;
                         const T rcm = fma_<T>(rho, s, -dm); // r - rho cos

@@ -132,6 +132,7 @@ extern "C" ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, cons
     CC_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     launch_pairs(ctx->stream, pp);
     CC_HIP(ctx, hipGetLastError());
+    CC_HIP(ctx, hipEventRecord(ctx->evm1, ctx->stream));
     launch_pair_reduce(ctx->stream, pp);
     CC_HIP(ctx, hipGetLastError());
     CC_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
@@ -144,10 +145,12 @@ extern "C" ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, cons
     }
     CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (report) {
-        float ms = 0.f;
+        float ms = 0.f, mms = 0.f;
         CC_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        CC_HIP(ctx, hipEventElapsedTime(&mms, ctx->ev0, ctx->evm1));
         report->node_evaluations = node_pairs;
         report->kernel_ms = ms;
+        report->main_kernel_ms = mms;
         report->block_size = 64;
         report->split = 1;
     }

@@ -123,3 +123,41 @@ def pair_workload(n_pairs=65536, seed=0):
         prim_rows.append(rp)
         sec_rows.append(rs)
     return prim_rows, sec_rows
+
+
+def group_rows(params, pf=5.0):
+    """C ABI rows of coils at their GPU PrecisionArguments for a group PrecisionFactor (the product's planning)."""
+    c = _c_coil()
+    rows = []
+    for p in params:
+        k = make_coil(p, pf)
+        a = c.coil.PrecisionArguments.get_coil_precision_arguments_GPU(k, c.coil.PrecisionFactor(pf))
+        rows.append(coil_row(k, a))
+    return rows
+
+
+def rotation(alpha, beta):
+    """The coil's transformation matrix (CalculateAttributes.cxx:65-78) as a 3x3 numpy array. Its entries
+    equal the product Rz(beta) Ry(alpha) Rz(beta); T e_z = (cos b sin a, sin b sin a, cos a) is the coil axis."""
+    ca, sa, cb, sb = np.cos(alpha), np.sin(alpha), np.cos(beta), np.sin(beta)
+    rz = np.array([[cb, -sb, 0.0], [sb, cb, 0.0], [0.0, 0.0, 1.0]])
+    ry = np.array([[ca, 0.0, sa], [0.0, 1.0, 0.0], [-sa, 0.0, ca]])
+    return rz @ ry @ rz
+
+
+def near_mask(rows, pts, width_floor=1e-3, chunk=1 << 18):
+    """Points within one cross-section width (max(a, b), floored at 1e-3 R) of any coil's winding rectangle, the
+    shell where the FP32 path hands (coil, point) pairs to the FP64 evaluator."""
+    near = np.zeros(len(pts), dtype=bool)
+    for s in range(0, len(pts), chunk):
+        P = pts[s:s + chunk]
+        m = near[s:s + chunk]
+        for r in rows:
+            T = rotation(r["alpha"], r["beta"])
+            loc = (P - np.asarray(r["pos"])) @ T      # T^T (x - pos), row vectors
+            rr = np.hypot(loc[:, 0], loc[:, 1])
+            dr = np.maximum(np.maximum(r["R"] - rr, rr - (r["R"] + r["a"])), 0.0)
+            dz = np.maximum(np.abs(loc[:, 2]) - 0.5 * r["b"], 0.0)
+            w = max(r["a"], r["b"], width_floor * r["R"])
+            m |= dr * dr + dz * dz < w * w
+    return near

@@ -100,6 +100,7 @@ typedef struct {
     double kernel_ms;          /* device time of the compute kernels of this call (hipEvent) */
     int32_t block_size;        /* threads per workgroup of the main kernel */
     int32_t split;             /* lanes cooperating on one point (field) */
+    double main_kernel_ms;     /* device time of the main kernel alone (field_kernel or pair_kernel; hipEvent) */
 } ccoil_report;
 
 /* Context lifecycle. ccoil_last_error(NULL) returns the last error of ccoil_create on this thread. */

@@ -3,7 +3,7 @@
 The oracle integrates the same quadrature rule (the coil's GPU PrecisionArguments, one block per layer).
 Accuracy is measured against the oracle's long double evaluation of that rule (the truth), because the
 reference's own FP64 gradient integrands lose up to ~3e-5 to cancellation near the winding
-(test_reference_fp64_gradient_conditioning pins that). Metric: per-point vector-norm relative error
+(test_oracle_fields.py::test_reference_fp64_gradient_conditioning pins that). Metric: per-point vector-norm relative error
 |x - ref| / |ref|. Tolerances:
   CCOIL_FP64           A, B <= 1e-11 at every point; gradient <= 1e-7 (far from the winding <= 1e-9)
   CCOIL_FP64_REFERENCE A, B <= 1e-10 against the oracle's double restatement of the same integrands
@@ -88,18 +88,6 @@ def test_fp64_reference_matches_oracle_double(ctx):
     assert np.nanmax(eg) < 1e-4
 
 
-def test_reference_fp64_gradient_conditioning():
-    """CPU-side fact the tolerances above rely on: the reference's own double gradient integrands differ from
-    the long double evaluation of the same rule by up to ~1e-5 near the winding."""
-    params, pts = W.single_rect_workload(65536, seed=0)
-    pts = pts[:2048]
-    rows, args = H.gpu_args_rows(params)
-    _, _, g = _double(params, args, pts, A=False, B=False)
-    _, _, gt = _truth(params, args, pts, A=False, B=False)
-    e = H.rel_err_vec(g.reshape(len(pts), 9), gt.reshape(len(pts), 9))
-    assert 1e-7 < np.nanmax(e) < 1e-3
-
-
 def test_single_rect_fp32c_and_fp32(ctx):
     params, pts = W.single_rect_workload(65536, seed=0)
     pts = pts[:4096]
@@ -162,6 +150,19 @@ def test_gradient_jacobian_convention(ctx):
     Jm = o2["G"].reshape(-1, 3, 3)
     tr = np.abs(np.trace(Jm, axis1=1, axis2=2)) / np.linalg.norm(Jm.reshape(-1, 9), axis=1)
     assert np.median(tr) < 1e-6
+
+
+@pytest.mark.parametrize("prec", [abi.FP64, abi.FP32C, abi.FP32, abi.FP64_REFERENCE])
+def test_quantity_subsets_agree(ctx, prec):
+    """Each quantity is the same whether requested alone or together with the others (the gradient's
+    rho-phi term reuses the B_rho sum, which must be formed also when B is not requested)."""
+    params, pts = W.group_workload(8, 512, seed=12)
+    rows, _ = H.gpu_args_rows(params)
+    full, _ = _run(ctx, rows, pts, abi.A | abi.B | abi.G, prec)
+    for key, bit in (("A", abi.A), ("B", abi.B), ("G", abi.G)):
+        alone, _ = _run(ctx, rows, pts, bit, prec)
+        e = H.rel_err_vec(alone[key], full[key])
+        assert np.nanmax(e) < (1e-12 if prec in (abi.FP64, abi.FP64_REFERENCE) else 1e-5), (key, np.nanmax(e))
 
 
 def test_device_pointers_with_torch(ctx):

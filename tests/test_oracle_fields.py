@@ -180,3 +180,22 @@ def test_dipole_far_field():
         mv = np.array([0, 0, m])
         ref = MU0 / (4 * np.pi) * (3 * np.dot(mv, rhat) * rhat - mv) / d ** 3
         assert np.linalg.norm(B[0] - ref) <= 5 * (R / d) ** 2 * np.linalg.norm(ref)
+
+
+def test_reference_fp64_gradient_conditioning():
+    """The reference's own double gradient integrands (CalculateFieldsMethods.cxx, restated by the oracle in
+    double) differ from the long double evaluation of the same rule by well over 1e-7 near the winding of a
+    rectangular coil; the accuracy claims of the device path are therefore made against the long double truth."""
+    import helpers as H
+    import workloads as W
+    params, pts = W.single_rect_workload(65536, seed=0)
+    pts = pts[:2048]
+    _, args = H.gpu_args_rows(params)
+    c = H.oracle_coil(params[0])
+    _, _, g = O.coil_fields(c, args[0], pts, False, False, True, axis_ratio=O.AXIS_RATIO_GPU, threads=8)
+    _, _, gt = O.coil_fields(c, args[0], pts, False, False, True, axis_ratio=O.AXIS_RATIO_GPU, threads=8,
+                             extended=True)
+    num = np.linalg.norm((g - gt).reshape(len(pts), 9), axis=1)
+    e = num / np.linalg.norm(gt.reshape(len(pts), 9), axis=1)
+    assert 1e-7 < np.max(e) < 1e-3
+    assert np.median(e) < 1e-12
