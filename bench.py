@@ -167,12 +167,15 @@ def main():
     if not args.no_fp64:
         ref = {"B": torch.empty_like(out["B"]), "G": torch.empty_like(out["G"])}
         fp64 = {}
-        # CCOIL_FP64 last, so that ref holds its output for the accuracy report; out still holds FP32C
-        for name, prec in (("fp64_reference", abi.FP64_REFERENCE), ("fp64", abi.FP64)):
-            step(prec, ref)
+        # CCOIL_FP64 last, so that ref holds its output for the accuracy report; out still holds FP32C and
+        # ieee the CCOIL_FP64_IEEE output
+        ieee = {"B": torch.empty_like(out["B"]), "G": torch.empty_like(out["G"])}
+        for name, prec, o in (("fp64_reference", abi.FP64_REFERENCE, ref), ("fp64_ieee", abi.FP64_IEEE, ieee),
+                              ("fp64", abi.FP64, ref)):
+            step(prec, o)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            r64 = [step(prec, ref) for _ in range(args.fp64_steps)]
+            r64 = [step(prec, o) for _ in range(args.fp64_steps)]
             torch.cuda.synchronize()
             dt = (time.perf_counter() - t0) / args.fp64_steps
             fp64[name] = {"value": len(rows) * n / dt, "ms_per_step": dt * 1e3,
@@ -180,6 +183,7 @@ def main():
             log(f"{name}: {dt * 1e3:.1f} ms/step")
         fp64["fp32c_speedup_vs_fp64"] = value / fp64["fp64"]["value"]
         fp64["fp32c_speedup_vs_fp64_reference"] = value / fp64["fp64_reference"]["value"]
+        fp64["fp32c_speedup_vs_fp64_ieee"] = value / fp64["fp64_ieee"]["value"]
         result["fp64"] = fp64
         b32, g32 = out["B"].cpu().numpy(), out["G"].cpu().numpy()
         b64, g64 = ref["B"].cpu().numpy(), ref["G"].cpu().numpy()
@@ -190,6 +194,9 @@ def main():
             "against": "CCOIL_FP64 on the same points", "near_points": int(near.sum()),
             "B": {"far": stats(eb[~near[okb]]), "near": stats(eb[near[okb]])},
             "G": {"far": stats(eg[~near[okg]]), "near": stats(eg[near[okg]])}}
+        e1, _ = rel_err(ieee["B"].cpu().numpy(), b64)
+        e2, _ = rel_err(ieee["G"].cpu().numpy(), g64)
+        result["accuracy"]["fp64_vs_fp64_ieee"] = {"B": stats(e1), "G": stats(e2)}
         log(f"accuracy: {json.dumps(result['accuracy'])}")
 
     if not args.no_pairs:

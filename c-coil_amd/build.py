@@ -3,7 +3,7 @@
 Outputs (git-ignored, they travel with the source tree):
   c-coil_amd/libccoil_hip.so   HIP kernels + the C ABI of include/ccoil_hip.h
   c-coil_amd/libccoil.so       the C++ Coil / CoilGroup / vec3 API (links libccoil_hip.so)
-  c-coil_amd/c_coil*.so        pybind11 module c_coil (tensor, coil, coil_group submodules)
+  c-coil_amd/_c_coil*.so       pybind11 module behind c_coil.py (tensor, coil, coil_group submodules)
 """
 import concurrent.futures as cf
 import os
@@ -61,7 +61,7 @@ def build(jobs=8, verbose=False):
     _run(["g++", "-shared", "-fPIC", "-o", lib_host] + [_obj(s) for s in HOST_SOURCES] +
          ["-L" + HERE, "-lccoil_hip", "-Wl,-rpath,$ORIGIN"])
     ext = sysconfig.get_config_var("EXT_SUFFIX")
-    mod = os.path.join(HERE, "c_coil" + ext)
+    mod = os.path.join(HERE, "_c_coil" + ext)
     _run(["g++", "-shared", "-fPIC", "-o", mod] + [_obj(s) for s in PY_SOURCES] +
          ["-L" + HERE, "-lccoil", "-lccoil_hip", "-Wl,-rpath,$ORIGIN"])
     return lib_hip, lib_host, mod

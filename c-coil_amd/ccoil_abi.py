@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libccoil_hip.so")
 
 OK, ERR_ARGUMENT, ERR_OUT_OF_MEMORY, ERR_DEVICE, ERR_NO_DEVICE = 0, -1, -2, -3, -4
 A, B, E, G, G_JACOBIAN = 1, 2, 4, 8, 16
-FP64, FP32C, FP32, FP64_REFERENCE = 0, 1, 2, 3
+FP64, FP32C, FP32, FP64_REFERENCE, FP64_IEEE = 0, 1, 2, 3, 4
 DEVICE_POINTERS, NO_NEAR_FP64, FORCE_REFERENCE_SCALE = 1, 2, 4
 PAIR_M, PAIR_FT = 1, 2
 
@@ -46,6 +46,8 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
+        import hip_runtime
+        hip_runtime.preload()
         L = ctypes.CDLL(LIB_PATH)
         vp = ctypes.c_void_p
         dp = ctypes.POINTER(ctypes.c_double)

@@ -243,7 +243,7 @@ ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, const double 
     if (n_points < 0) return fail(ctx, CCOIL_ERR_ARGUMENT, "n_points is negative");
     if (n_points > ((int64_t)1 << 31) - 1) return fail(ctx, CCOIL_ERR_ARGUMENT, "n_points exceeds 2^31 - 1");
     if (precision != CCOIL_FP64 && precision != CCOIL_FP32C && precision != CCOIL_FP32 &&
-        precision != CCOIL_FP64_REFERENCE)
+        precision != CCOIL_FP64_REFERENCE && precision != CCOIL_FP64_IEEE)
         return fail(ctx, CCOIL_ERR_ARGUMENT, "unknown precision");
     const uint32_t known = CCOIL_A | CCOIL_B | CCOIL_E | CCOIL_G | CCOIL_G_JACOBIAN;
     if (quantities == 0 || (quantities & ~known)) return fail(ctx, CCOIL_ERR_ARGUMENT, "bad quantity mask");

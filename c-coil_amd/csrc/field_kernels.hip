@@ -108,6 +108,8 @@ __global__ __launch_bounds__(kFieldBlock) void field_kernel(FieldParams p)
         to_local(c, px, py, pz, z, r, cph, sph);
         Local L = {0, 0, 0, 0, 0, 0, 0};
         if constexpr (PREC == kFP64) {
+            eval_stable<f64s, WA, WB, WG>(c, p.tab, z, r, lane, SPLIT, L);
+        } else if constexpr (PREC == kFP64IEEE) {
             eval_stable<double, WA, WB, WG>(c, p.tab, z, r, lane, SPLIT, L);
         } else if constexpr (PREC == kFP64Reference) {
             eval_plain<double, WA, WB, WG>(c, p.tab, z, r, lane, SPLIT, L);
@@ -221,6 +223,7 @@ void launch_field(hipStream_t s, const FieldParams &p, int precision, int split)
 {
     if (p.n_pts <= 0) return;
     if (precision == kFP64) launch_field_p<kFP64>(s, p, split);
+    else if (precision == kFP64IEEE) launch_field_p<kFP64IEEE>(s, p, split);
     else if (precision == kFP64Reference) launch_field_p<kFP64Reference>(s, p, split);
     else if (precision == kFP32) launch_field_p<kFP32>(s, p, split);
     else launch_field_p<kFP32C>(s, p, split);

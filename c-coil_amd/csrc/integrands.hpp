@@ -168,14 +168,42 @@ template <typename T> __device__ inline T log1p_(T x);
 template <> __device__ inline double log1p_<double>(double x) { return log1p(x); }
 template <> __device__ inline float log1p_<float>(float x) { return log1pf(x); }
 
+// Arithmetic policies of eval_stable: float (FP32C), double (IEEE division and square root), and f64s:
+// double arithmetic whose reciprocals and reciprocal square roots come from the hardware FP64
+// approximations (v_rcp_f64, v_rsq_f64) refined by one Newton step carrying the second-order term.
+// With a seed error e the refined value errs by O(e^3), below the double rounding of the residual, so f64s
+// is accurate to a few ulp per operation while avoiding the scaled division and square-root sequences
+// (FP64 FMA issues at the FP32 rate on CDNA3/4; the IEEE sequences dominate the double kernel's time).
+// Its domain is that of the approximations: normal, finite operands.
+struct f64s {};
+template <typename P> struct Real { using type = P; };
+template <> struct Real<f64s> { using type = double; };
+template <typename P> using real_t = typename Real<P>::type;
+
+template <typename P> __device__ inline real_t<P> prsq(real_t<P> x) { return rsq<real_t<P>>(x); }
+template <> __device__ inline double prsq<f64s>(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-(x * y), y, 1.0);               // 1 - x y^2
+    return fma(y * e, fma(0.375, e, 0.5), y);             // y (1 + e/2 + 3 e^2 / 8)
+}
+template <typename P> __device__ inline real_t<P> prcp(real_t<P> x) { return rcp_<real_t<P>>(x); }
+template <> __device__ inline double prcp<f64s>(double x)
+{
+    const double y = __builtin_amdgcn_rcp(x);
+    const double e = fma(-x, y, 1.0);                      // 1 - x y
+    return fma(y, fma(e, e, e), y);                        // y (1 + e + e^2)
+}
+
 template <typename T> __device__ inline const T *tab_omc(const Tables &t);
 template <> __device__ inline const double *tab_omc<double>(const Tables &t) { return t.omc; }
 template <> __device__ inline const float *tab_omc<float>(const Tables &t) { return t.omcf; }
 
-template <typename T, bool WA, bool WB, bool WG>
+template <typename P, bool WA, bool WB, bool WG>
 __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double zd, double rd, int row0, int rstep,
                                    Local &L)
 {
+    using T = real_t<P>;
     const double *xt = tab.x + table_offset(c.n_thk);
     const double *wt = tab.w + table_offset(c.n_thk);
     const T *ct = tab_cos<T>(tab) + table_offset(c.n_ang);
@@ -213,15 +241,15 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                 const T M = fma_<T>(q, s, base);
                 if (WB || WG) {
                     const T J1 = M + z1s, J2 = M + z2s;
-                    const T K1 = rsq<T>(J1), K2 = rsq<T>(J2);
-                    const T S = rcp_<T>(K1 + K2);
+                    const T K1 = prsq<P>(J1), K2 = prsq<P>(J2);
+                    const T S = prcp<P>(K1 + K2);
                     const T K12 = K1 * K2;
                     const T tP2 = tbz * (K12 * K12);
                     const T dK = -tP2 * S;                              // K1 - K2
                     const T g1 = z1 * K1, g2 = z2 * K2;
                     const T gs = g1 + g2;
                     const T Y = same ? M * gs : M;
-                    const T Rr = rcp_<T>(Y);
+                    const T Rr = prcp<P>(Y);
                     const T invM = same ? gs * Rr : Rr;
                     const T dgM = same ? tP2 * (M * Rr) : (g1 - g2) * Rr;   // (z1 K1 - z2 K2) / M
                     const T rmc = fma_<T>(rf, s, dm);                       // rho - r cos
@@ -243,14 +271,14 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                 }
                 if (WA) {
                     const T J1 = M + za1s, J2 = M + za2s;
-                    const T K1 = rsq<T>(J1), K2 = rsq<T>(J2);
+                    const T K1 = prsq<P>(J1), K2 = prsq<P>(J2);
                     const T sJ1 = J1 * K1, sJ2 = J2 * K2;                   // sqrt(J)
                     // asinh(z1 / sqrt(M)) - asinh(z2 / sqrt(M)) = asinh(t)
                     const T num = samea ? tbza : (za1 * sJ2 - za2 * sJ1);
                     const T den = samea ? (za1 * sJ2 + za2 * sJ1) : M;
-                    const T t = num * rcp_<T>(den);
+                    const T t = num * prcp<P>(den);
                     const T y = sqrt_<T>(fma_<T>(t, t, (T)1));
-                    const T as = log1p_<T>(t + t * t * rcp_<T>((T)1 + y));
+                    const T as = log1p_<T>(t + t * t * prcp<P>((T)1 + y));
                     rA = fma_<T>(w, cp * as, rA);
                 }
             }
@@ -285,7 +313,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             for (int i = 0; i < na; ++i) {
                 const T cp = ct[i], s = st[i], w = wa[i];
                 const T J = fma_<T>(q, s, baseJ);
-                const T K = rsq<T>(J);
+                const T K = prsq<P>(J);
                 if (WB || WG) {
                     const T K2 = K * K;
                     const T K3 = K2 * K;

@@ -6,7 +6,7 @@
 namespace ccoil {
 
 // Arithmetic of the quadrature (values of the C ABI precision argument).
-enum { kFP64 = 0, kFP32C = 1, kFP32 = 2, kFP64Reference = 3 };
+enum { kFP64 = 0, kFP32C = 1, kFP32 = 2, kFP64Reference = 3, kFP64IEEE = 4 };
 constexpr int kFieldBlock = 256;
 
 struct FieldParams {

@@ -20,12 +20,15 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
-// PREC: 0 FP64 (stable integrands), 1 FP32C (stable in FP32, near-winding nodes in FP64),
-// 2 FP32 (reference integrands in float), 3 FP64 reference integrands.
+// PREC: 0 FP64 (stable integrands, refined hardware reciprocals), 1 FP32C (stable in FP32, near-winding
+// nodes in FP64), 2 FP32 (reference integrands in float), 3 FP64 reference integrands, 4 FP64 with IEEE
+// division and square root.
 template <int PREC, bool WM, bool WF>
 __device__ inline void eval_prim(const CoilDev &pc, const Tables &tab, double z, double r, Local &L)
 {
     if constexpr (PREC == 0) {
+        eval_stable<f64s, WM, WF, false>(pc, tab, z, r, 0, 1, L);
+    } else if constexpr (PREC == 4) {
         eval_stable<double, WM, WF, false>(pc, tab, z, r, 0, 1, L);
     } else if constexpr (PREC == 2) {
         eval_plain<float, WM, WF, false>(pc, tab, z, r, 0, 1, L);
@@ -156,6 +159,7 @@ void launch_pairs(hipStream_t s, const PairParams &p)
 {
     if (p.n_items <= 0) return;
     if (p.precision == 0) launch_pairs_p<0>(s, p);
+    else if (p.precision == 4) launch_pairs_p<4>(s, p);
     else if (p.precision == 2) launch_pairs_p<2>(s, p);
     else if (p.precision == 3) launch_pairs_p<3>(s, p);
     else launch_pairs_p<1>(s, p);

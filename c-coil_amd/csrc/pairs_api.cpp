@@ -33,7 +33,7 @@ extern "C" ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, cons
     if (what == 0 || (what & ~(uint32_t)(CCOIL_PAIR_M | CCOIL_PAIR_FT)))
         return fail(ctx, CCOIL_ERR_ARGUMENT, "bad pair quantity mask");
     if (precision != CCOIL_FP64 && precision != CCOIL_FP32C && precision != CCOIL_FP32 &&
-        precision != CCOIL_FP64_REFERENCE)
+        precision != CCOIL_FP64_REFERENCE && precision != CCOIL_FP64_IEEE)
         return fail(ctx, CCOIL_ERR_ARGUMENT, "unknown precision");
     if ((what & CCOIL_PAIR_M) && !M) return fail(ctx, CCOIL_ERR_ARGUMENT, "M requested but M is NULL");
     if ((what & CCOIL_PAIR_FT) && (!F || !T)) return fail(ctx, CCOIL_ERR_ARGUMENT, "F/T requested but F or T is NULL");

@@ -53,9 +53,10 @@ int precision() { return t_precision; }
 
 void setPrecision(int p)
 {
-    if (p != CCOIL_FP64 && p != CCOIL_FP32C && p != CCOIL_FP32 && p != CCOIL_FP64_REFERENCE)
-        throw std::invalid_argument("GPU precision must be CCOIL_FP64, CCOIL_FP32C, CCOIL_FP32 or "
-                                    "CCOIL_FP64_REFERENCE");
+    if (p != CCOIL_FP64 && p != CCOIL_FP32C && p != CCOIL_FP32 && p != CCOIL_FP64_REFERENCE &&
+        p != CCOIL_FP64_IEEE)
+        throw std::invalid_argument("GPU precision must be CCOIL_FP64, CCOIL_FP32C, CCOIL_FP32, "
+                                    "CCOIL_FP64_REFERENCE or CCOIL_FP64_IEEE");
     t_precision = p;
 }
 

@@ -1,4 +1,4 @@
-// pybind11 module c_coil: submodules tensor, coil, coil_group with the reference's Python names
+// pybind11 module _c_coil (imported as c_coil through c_coil.py): submodules tensor, coil, coil_group with the reference's Python names
 // (src/PyMain.cxx:36-43, src/Tensor/PyBindings.cxx, src/Coil/PyBindings.cxx:11-364,
 // src/CoilGroup/PyBindings.cxx:13-113). The benchmark and compare submodules of the reference are not
 // part of this GPU library.
@@ -354,6 +354,7 @@ static void init_coil(py::module_ &m)
     c.attr("GPU_FP32C") = (int)CCOIL_FP32C;
     c.attr("GPU_FP32") = (int)CCOIL_FP32;
     c.attr("GPU_FP64_REFERENCE") = (int)CCOIL_FP64_REFERENCE;
+    c.attr("GPU_FP64_IEEE") = (int)CCOIL_FP64_IEEE;
     c.def("set_gpu_increments", &ccoil::setGpuIncrements, py::arg("increments"));
     c.def("get_gpu_increments", &ccoil::gpuIncrements);
 }
@@ -414,7 +415,7 @@ static void init_coil_group(py::module_ &m)
         .def("__repr__", &CoilGroup::operator std::string);
 }
 
-PYBIND11_MODULE(c_coil, m)
+PYBIND11_MODULE(_c_coil, m)
 {
     m.doc() = "C-Coil (MI355X build): circular-coil magnetostatics on gfx950 through the ccoil_hip C ABI";
     init_tensor(m);

@@ -51,13 +51,17 @@ enum {
 
 /* Arithmetic of the quadrature. */
 enum {
-    CCOIL_FP64 = 0,           /* cancellation-free integrands in double: the accurate mode */
+    CCOIL_FP64 = 0,           /* cancellation-free integrands in double; reciprocals and reciprocal square roots
+                                 from the hardware FP64 approximations refined by one Newton step (a few ulp):
+                                 the accurate mode */
     CCOIL_FP32C = 1,          /* the same integrands with FP32 per-node arithmetic, FP64 per-row work and row
                                  accumulation, FP64 re-evaluation of near-winding (coil, point) pairs */
     CCOIL_FP32 = 2,           /* the reference's integrands in float (what a TYPE=float build computes) */
-    CCOIL_FP64_REFERENCE = 3  /* the reference's integrands verbatim in double (a TYPE=double build): the
+    CCOIL_FP64_REFERENCE = 3, /* the reference's integrands verbatim in double (a TYPE=double build): the
                                  parity mode and the plain-FP64 throughput comparator; its gradient terms
                                  lose up to ~3e-5 relative to cancellation near the winding */
+    CCOIL_FP64_IEEE = 4       /* CCOIL_FP64's integrands with IEEE division and square root (also the
+                                 evaluator of FP32C's near-winding pairs) */
 };
 
 /* Flags. */

@@ -45,8 +45,9 @@ def test_device_gauss_legendre_matches_leggauss(ctx):
         assert np.max(np.abs(w - wr)) < 1e-14, n
 
 
+@pytest.mark.parametrize("prec", [abi.FP64, abi.FP64_IEEE])
 @pytest.mark.parametrize("case", ["rect", "mixed"])
-def test_fp64_matches_truth(ctx, case):
+def test_fp64_matches_truth(ctx, case, prec):
     if case == "rect":
         params, pts = W.single_rect_workload(65536, seed=0)
         pts = pts[:4096]
@@ -55,13 +56,13 @@ def test_fp64_matches_truth(ctx, case):
     rows, args = H.gpu_args_rows(params)
     if case == "rect":
         assert list(args[0][3:]) == [59, 27, 27]
-    out, rep = _run(ctx, rows, pts, abi.A | abi.B | abi.G, abi.FP64)
+    out, rep = _run(ctx, rows, pts, abi.A | abi.B | abi.G, prec)
     assert rep.near_pairs == 0
     a, b, g = _truth(params, args, pts)
     near = H.near_mask(params, pts)
     ea, eb = H.rel_err_vec(out["A"], a), H.rel_err_vec(out["B"], b)
     eg = H.rel_err_vec(out["G"], g.reshape(len(pts), 9))
-    print(f"\nFP64 {case}: A max {np.nanmax(ea):.3g} B max {np.nanmax(eb):.3g} G max {np.nanmax(eg):.3g} "
+    print(f"\nprecision {prec} {case}: A max {np.nanmax(ea):.3g} B max {np.nanmax(eb):.3g} G max {np.nanmax(eg):.3g} "
           f"G far max {np.nanmax(eg[~near]):.3g}")
     assert np.nanmax(ea) < 1e-11
     assert np.nanmax(eb) < 1e-11
@@ -152,7 +153,7 @@ def test_gradient_jacobian_convention(ctx):
     assert np.median(tr) < 1e-6
 
 
-@pytest.mark.parametrize("prec", [abi.FP64, abi.FP32C, abi.FP32, abi.FP64_REFERENCE])
+@pytest.mark.parametrize("prec", [abi.FP64, abi.FP32C, abi.FP32, abi.FP64_REFERENCE, abi.FP64_IEEE])
 def test_quantity_subsets_agree(ctx, prec):
     """Each quantity is the same whether requested alone or together with the others (the gradient's
     rho-phi term reuses the B_rho sum, which must be formed also when B is not requested)."""
@@ -162,7 +163,8 @@ def test_quantity_subsets_agree(ctx, prec):
     for key, bit in (("A", abi.A), ("B", abi.B), ("G", abi.G)):
         alone, _ = _run(ctx, rows, pts, bit, prec)
         e = H.rel_err_vec(alone[key], full[key])
-        assert np.nanmax(e) < (1e-12 if prec in (abi.FP64, abi.FP64_REFERENCE) else 1e-5), (key, np.nanmax(e))
+        assert np.nanmax(e) < (1e-12 if prec in (abi.FP64, abi.FP64_REFERENCE, abi.FP64_IEEE) else 1e-5), \
+            (key, np.nanmax(e))
 
 
 def test_device_pointers_with_torch(ctx):

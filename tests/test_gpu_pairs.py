@@ -31,7 +31,7 @@ def _small_incs(rng, params, sec):
     return out
 
 
-@pytest.mark.parametrize("prec,tol", [(abi.FP64, 1e-10), (abi.FP32C, 1e-6)])
+@pytest.mark.parametrize("prec,tol", [(abi.FP64, 1e-10), (abi.FP64_IEEE, 1e-10), (abi.FP32C, 1e-6)])
 def test_gpu_rule_pairs_match_oracle(ctx, prec, tol):
     rng = np.random.default_rng(7)
     n = 24
