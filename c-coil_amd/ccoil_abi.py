@@ -19,7 +19,8 @@ DEVICE_POINTERS, NO_NEAR_FP64, FORCE_REFERENCE_SCALE = 1, 2, 4
 PAIR_M, PAIR_FT = 1, 2
 
 EXPORTED = ["ccoil_create", "ccoil_destroy", "ccoil_last_error", "ccoil_version", "ccoil_set_stream",
-            "ccoil_synchronize", "ccoil_gauss_legendre", "ccoil_fields", "ccoil_pairs"]
+            "ccoil_synchronize", "ccoil_gauss_legendre", "ccoil_fields", "ccoil_pairs", "ccoil_field_gradient"]
+GRAD_TERMS = ("current", "R", "a", "b", "px", "py", "pz", "alpha", "beta")
 
 
 class Coils(ctypes.Structure):
@@ -63,8 +64,10 @@ def lib():
                                    ctypes.c_uint32, vp, vp, vp, vp, ctypes.POINTER(Report)]
         L.ccoil_pairs.argtypes = [vp, ctypes.POINTER(Coils), ctypes.POINTER(Coils), vp, vp, ctypes.c_int64, vp, vp,
                                   ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32, vp, vp, vp, ctypes.POINTER(Report)]
+        L.ccoil_field_gradient.argtypes = [vp, ctypes.POINTER(Coils), vp, ctypes.c_int64, vp, ctypes.c_int32,
+                                           ctypes.c_uint32, vp, ctypes.POINTER(Report)]
         for f in ("ccoil_create", "ccoil_destroy", "ccoil_set_stream", "ccoil_synchronize", "ccoil_gauss_legendre",
-                  "ccoil_fields", "ccoil_pairs"):
+                  "ccoil_fields", "ccoil_pairs", "ccoil_field_gradient"):
             getattr(L, f).restype = ctypes.c_int32
         _lib = L
     return _lib
@@ -165,6 +168,23 @@ class Context:
         self._check(lib().ccoil_fields(self.h, ctypes.byref(coils.struct), _ptr(points), n, quantities, precision,
                                        flags, _ptr(out.get("A")), _ptr(out.get("B")), _ptr(out.get("E")),
                                        _ptr(out.get("G")), ctypes.byref(rep)))
+        return out, rep
+
+    def field_gradient(self, coils, points, dL_dB, precision=FP64, flags=0, out=None):
+        """Adjoint of the summed B: (n_coils, 9) gradients in GRAD_TERMS order. Host numpy arrays, or torch tensors
+        with DEVICE_POINTERS and a preallocated `out`."""
+        rep = Report()
+        dev = flags & DEVICE_POINTERS
+        n = points.shape[0]
+        if not dev:
+            points = np.ascontiguousarray(points, dtype=np.float64)
+            dL_dB = np.ascontiguousarray(dL_dB, dtype=np.float64)
+            if dL_dB.shape != points.shape:
+                raise ValueError("dL_dB must have the shape of points")
+        if out is None:
+            out = np.zeros((coils.n, len(GRAD_TERMS)))
+        self._check(lib().ccoil_field_gradient(self.h, ctypes.byref(coils.struct), _ptr(points), n, _ptr(dL_dB),
+                                               precision, flags, _ptr(out), ctypes.byref(rep)))
         return out, rep
 
     def pairs(self, prim, sec, n_pairs, what, precision=FP32C, flags=0, prim_index=None, sec_index=None,

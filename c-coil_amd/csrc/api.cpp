@@ -65,6 +65,32 @@ void rotation(double alpha, double beta, double *T, double *Ti)
 
 namespace ccoil {
 
+// Axial vectors of Omega = T^T dT/dtheta for theta = alpha, beta, with T of rotation() differentiated entry by
+// entry (the antisymmetric part is read off; T^T dT is antisymmetric because T is orthogonal).
+void rotation_generators(double alpha, double beta, double *om_a, double *om_b)
+{
+    const double cY = std::cos(alpha), sY = std::sin(alpha), cZ = std::cos(beta), sZ = std::sin(beta);
+    double T[9], Ti[9];
+    rotation(alpha, beta, T, Ti);
+    const double dA[9] = {-cZ * cZ * sY, sZ * cZ * sY, cZ * cY,
+                          -sZ * cZ * sY, sZ * sZ * sY, sZ * cY,
+                          -cY * cZ, cY * sZ, -sY};
+    const double dB[9] = {-2 * cZ * sZ * cY - 2 * sZ * cZ, -(cZ * cZ - sZ * sZ) * cY - (cZ * cZ - sZ * sZ), -sZ * sY,
+                          (cZ * cZ - sZ * sZ) * cY + (cZ * cZ - sZ * sZ), -2 * cZ * sZ - 2 * sZ * cZ * cY, cZ * sY,
+                          sY * sZ, sY * cZ, 0.0};
+    double *om[2] = {om_a, om_b};
+    const double *dT[2] = {dA, dB};
+    for (int k = 0; k < 2; ++k) {
+        double O[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                O[3 * i + j] = T[i] * dT[k][j] + T[3 + i] * dT[k][3 + j] + T[6 + i] * dT[k][6 + j];
+        om[k][0] = 0.5 * (O[7] - O[5]);
+        om[k][1] = 0.5 * (O[2] - O[6]);
+        om[k][2] = 0.5 * (O[3] - O[1]);
+    }
+}
+
 // Validates a coil descriptor and packs coil i into the device layout. constFactor follows
 // generateCoilData (SupportFunctions.cxx:169-172): 1e-7 J a pi/2, times b for the slow method.
 std::string pack_coils(const ccoil_coils *c, bool need_len, bool need_freq, std::vector<CoilDev> &out)
@@ -113,6 +139,9 @@ std::string pack_coils(const ccoil_coils *c, bool need_len, bool need_freq, std:
         d.ca = std::cos(c->alpha[i]);
         d.sb = std::sin(c->beta[i]);
         d.cb = std::cos(c->beta[i]);
+        rotation_generators(c->alpha[i], c->beta[i], d.om_a, d.om_b);
+        d.cur_used = d.current;
+        d.adj_scale = 1.0;
         out[(size_t)i] = d;
     }
     return "";
@@ -196,7 +225,8 @@ ccoil_status ccoil_destroy(ccoil_ctx ctx)
         if (p) (void)hipFree(p);
     DevBuf *bufs[] = {&ctx->coils, &ctx->pts,   &ctx->outA,  &ctx->outB,     &ctx->outE,     &ctx->outG,
                       &ctx->queue, &ctx->counter, &ctx->pprim, &ctx->psec, &ctx->ppairs, &ctx->ppartial,
-                      &ctx->pcounter, &ctx->pM, &ctx->pF, &ctx->pT};
+                      &ctx->pcounter, &ctx->pM, &ctx->pF, &ctx->pT, &ctx->acoils, &ctx->apts, &ctx->agB,
+                      &ctx->apartial, &ctx->agrad};
     for (auto b : bufs) free_buf(*b);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);

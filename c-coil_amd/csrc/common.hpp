@@ -27,6 +27,9 @@ struct alignas(64) CoilDev {
     double turns;     // N
     double near_w2;   // squared near-winding shell width
     double sa, ca, sb, cb;   // sin/cos of alpha (y-axis angle) and beta (z-axis angle)
+    double om_a[3], om_b[3]; // T^T dT/dalpha and T^T dT/dbeta as axial vectors (Omega v = om x v)
+    double cur_used;         // current the constFactor was formed with (1 for a zero-current coil)
+    double adj_scale;        // I / cur_used: scales every adjoint term except d/dI
     int32_t use_fast;
     int32_t n_ang, n_thk, n_len;
     int32_t blk_ang, blk_thk, blk_len;

@@ -20,6 +20,7 @@ struct DevBuf {
 hipError_t ensure(DevBuf &b, size_t bytes);
 void free_buf(DevBuf &b);
 void rotation(double alpha, double beta, double *T, double *Ti);
+void rotation_generators(double alpha, double beta, double *om_a, double *om_b);
 std::string pack_coils(const ccoil_coils *c, bool need_len, bool need_freq, std::vector<CoilDev> &out);
 
 }  // namespace ccoil
@@ -36,6 +37,7 @@ struct ccoil_ctx_s {
     double *gl_rcos = nullptr, *gl_rsin = nullptr;
     ccoil::DevBuf coils, pts, outA, outB, outE, outG, queue, counter;
     ccoil::DevBuf pprim, psec, ppairs, ppartial, pcounter, pM, pF, pT;
+    ccoil::DevBuf acoils, apts, agB, apartial, agrad;   // field adjoint
     hipEvent_t ev0 = nullptr, ev1 = nullptr;   // around all compute kernels of a call
     hipEvent_t evm0 = nullptr, evm1 = nullptr; // around the main kernel (field_kernel / pair_kernel)
 };

@@ -199,10 +199,22 @@ template <typename T> __device__ inline const T *tab_omc(const Tables &t);
 template <> __device__ inline const double *tab_omc<double>(const Tables &t) { return t.omc; }
 template <> __device__ inline const float *tab_omc<float>(const Tables &t) { return t.omcf; }
 
-template <typename P, bool WA, bool WB, bool WG>
+// Adjoint extras of one (coil, point): derivatives of the local (B_rho, B_z) with respect to the coil's inner
+// radius R, thickness a and length b ([0] rho component, [1] z component). Each quadrature row j is the field
+// f_j(rho_j, b, r, z) of a current sheet (fast method) or ring (slow method) of radius rho_j = R + a/2 (1 + x_j);
+// f_j is homogeneous in (rho, b, r, z) of degree 0 (sheet) or -1 (ring), so
+//   rho df/drho = -(deg f + r df/dr + z df/dz + b df/db),
+// with df/dr, df/dz the row's own gradient terms and df/db = (ring at z + b/2 + ring at z - b/2) / 2. The
+// constant factors hold J a = N I / b (sheet: dB/db gains -B/b) and J a b = N I (ring: no a or b).
+struct AdjLocal {
+    double dR[2], da[2], db[2];
+};
+
+template <typename P, bool WA, bool WB, bool WG, bool AD = false>
 __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double zd, double rd, int row0, int rstep,
-                                   Local &L)
+                                   Local &L, AdjLocal *ad = nullptr)
 {
+    static_assert(!AD || (WB && WG), "the adjoint needs B and the gradient terms");
     using T = real_t<P>;
     const double *xt = tab.x + table_offset(c.n_thk);
     const double *wt = tab.w + table_offset(c.n_thk);
@@ -211,6 +223,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
     const T *wa = tab_w<T>(tab) + table_offset(c.n_ang);
     const int na = c.n_ang, nt = c.n_thk;
     double aA = 0, aBh = 0, aBz = 0, aRR = 0, aRZ = 0, aZZ = 0;
+    double hR0 = 0, hR1 = 0, ha0 = 0, ha1 = 0, hb0 = 0, hb1 = 0;   // adjoint row sums
 
     if (c.use_fast) {
         const double z1d = zd + 0.5 * c.b, z2d = zd - 0.5 * c.b;
@@ -236,6 +249,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             const T n2 = (T)(-3.0 * rho_d * rd * dd * dd);
             const T n3 = (T)(-2.0 * rho_d * rho_d * rd * rd);
             T rBh = 0, rBz = 0, rRR = 0, rRZ = 0, rZZ = 0, rA = 0;
+            T eR = 0, eZ = 0;   // adjoint: end rings, sum w cos (z1 K1^3 + z2 K2^3), sum w (rho - r cos)(K1^3 + K2^3)
             for (int i = 0; i < na; ++i) {
                 const T cp = ct[i], s = st[i], w = wa[i];
                 const T M = fma_<T>(q, s, base);
@@ -267,6 +281,11 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                         const T N = fma_<T>(s, fma_<T>(s, fma_<T>(s, n3, n2), n1), base2);
                         const T t = fma_<T>(O, dg, N * dh);
                         rRZ = fma_<T>(w, t * (invM * invM), rRZ);
+                        if constexpr (AD) {
+                            const T K13 = K1s * K1, K23 = K2s * K2;
+                            eR = fma_<T>(w, cp * fma_<T>(z1, K13, z2 * K23), eR);
+                            eZ = fma_<T>(w, rmc * (K13 + K23), eZ);
+                        }
                     }
                 }
                 if (WA) {
@@ -289,6 +308,26 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             aZZ += W * (double)rZZ;
             aRZ += W * (double)rRZ;
             aA += W * (double)rA;
+            if constexpr (AD) {
+                const double Wr = c.cf * wt[j], hx = 0.5 * (1.0 + xt[j]);
+                const double e0 = 0.5 * (double)eR, e1 = 0.5 * (double)eZ;
+                const double g0 = rd * (double)rRR + zd * (double)rRZ + c.b * e0;   // -d f_rho / d rho
+                const double g1 = rd * (double)rRZ + zd * (double)rZZ + c.b * e1;   // -d f_z / d rho
+                hR0 += Wr * g0;
+                hR1 += Wr * g1;
+                ha0 += Wr * hx * g0;
+                ha1 += Wr * hx * g1;
+                hb0 += W * e0;
+                hb1 += W * e1;
+            }
+        }
+        if constexpr (AD) {
+            ad->dR[0] = -hR0;
+            ad->dR[1] = -hR1;
+            ad->da[0] = -ha0;
+            ad->da[1] = -ha1;
+            ad->db[0] = hb0 - aBh / c.b;
+            ad->db[1] = hb1 - aBz / c.b;
         }
         L.A = aA;
         L.Bh = aBh;
@@ -338,6 +377,23 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             aZZ += W * (double)rZZ;
             aRZ += W * (double)rRZ;
             aA += W * (double)rA;
+            if constexpr (AD) {
+                const double Wr = c.cf * wt[j], hx = 0.5 * (1.0 + xt[j]);
+                const double g0 = zd * ((double)rBh - 3.0 * rd * (double)rRR + (double)rRZ);     // -d f_rho / d rho
+                const double g1 = (double)rBz + rd * (double)rRZ - 3.0 * zd * zd * (double)rZZ;  // -d f_z / d rho
+                hR0 += Wr * g0;
+                hR1 += Wr * g1;
+                ha0 += Wr * hx * g0;
+                ha1 += Wr * hx * g1;
+            }
+        }
+        if constexpr (AD) {
+            ad->dR[0] = -hR0;
+            ad->dR[1] = -hR1;
+            ad->da[0] = -ha0;
+            ad->da[1] = -ha1;
+            ad->db[0] = 0.0;
+            ad->db[1] = 0.0;
         }
         L.A = aA;
         L.Bh = aBh * zd;

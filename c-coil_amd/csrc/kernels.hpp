@@ -23,6 +23,22 @@ struct FieldParams {
     int64_t near_capacity;
 };
 
+// Adjoint of the summed B field: per-coil gradients of sum_x dL/dB(x) . B(x), 9 per coil in the order
+// current, R, a, b, px, py, pz, alpha, beta. Grid: (point blocks, coils); partial sums per (coil, block).
+constexpr int kGradTerms = 9;
+struct GradParams {
+    const CoilDev *coils;
+    int32_t n_coils;
+    const double *pts;     // AoS xyz
+    int64_t n_pts;
+    const double *gB;      // dL/dB, AoS (global frame)
+    double *partial;       // [n_coils][n_blocks][kGradTerms]
+    int32_t n_blocks;
+    double *grad;          // [n_coils][kGradTerms]
+    Tables tab;
+};
+void launch_field_grad(hipStream_t s, const GradParams &p, int precision);
+
 void launch_gl_tables(hipStream_t s, double *x, double *w, double *cosv, double *omc, float *xf, float *wf,
                       float *cosf, float *omcf, double *rcos, double *rsin);
 // split: 1 or 4 lanes per point.

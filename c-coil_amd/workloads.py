@@ -161,3 +161,28 @@ def near_mask(rows, pts, width_floor=1e-3, chunk=1 << 18):
             w = max(r["a"], r["b"], width_floor * r["R"])
             m |= dr * dr + dz * dz < w * w
     return near
+
+
+def adjoint_workload(n_coils=64, n_points=4096, seed=0):
+    """Coil-design workload: THIN / RECTANGULAR coils near the z axis and a target region on the axis.
+    type ~ U{THIN, RECTANGULAR}; R ~ U[0.05, 0.3]; b ~ U[0.01, 0.1]; a ~ U[0.01, 0.1] for RECTANGULAR (0 for THIN);
+    N = 500; I ~ U[0.5, 5]; x, y ~ U[-0.05, 0.05], z ~ U[-0.5, 0.5]; tilt alpha ~ U[0, 5 deg], beta ~ U[0, 2 pi).
+    Points: z ~ U[-0.2, 0.2], uniform in the disk r <= 0.02."""
+    rng = np.random.default_rng(seed)
+    params = []
+    for _ in range(n_coils):
+        t = (THIN, RECTANGULAR)[rng.integers(2)]
+        R = rng.uniform(0.05, 0.3)
+        b = rng.uniform(0.01, 0.1)
+        a = rng.uniform(0.01, 0.1) if t == RECTANGULAR else 0.0
+        I = rng.uniform(0.5, 5.0)
+        pos = np.array([rng.uniform(-0.05, 0.05), rng.uniform(-0.05, 0.05), rng.uniform(-0.5, 0.5)])
+        alpha = rng.uniform(0.0, np.deg2rad(5.0))
+        beta = rng.uniform(0.0, 2 * np.pi)
+        params.append(dict(type=t, R=R, a=a, b=b, N=500, I=I, pos=pos, alpha=alpha, beta=beta))
+    u = rng.random((n_points, 3))
+    r = 0.02 * np.sqrt(u[:, 0])
+    phi = 2 * np.pi * u[:, 1]
+    z = 0.4 * u[:, 2] - 0.2
+    pts = np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)
+    return params, pts

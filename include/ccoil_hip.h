@@ -139,6 +139,17 @@ ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coi
                          uint32_t what, int32_t precision, uint32_t flags,
                          double *M, double *F, double *T, ccoil_report *report);
 
+/* Adjoint of the summed B field (no counterpart in the reference's CUDA interface; it backs the gradient of
+ * scalar objectives of the field, e.g. a torch.autograd.Function over ccoil_fields). Given dL_dB (3 per point,
+ * global frame) for a scalar L of B = sum_c B_c at n_points points, writes per coil c the 9 values
+ *   dL/d(current, R, a, b, px, py, pz, alpha, beta)   (grad, 9 per coil, coil-major)
+ * of sum_x dL_dB(x) . B_c(x), with each coil's quadrature orders (n_ang, n_thk) held fixed. Needs coil current
+ * and turns (B is linear in the current through J = N I / (a b)). precision: CCOIL_FP64, CCOIL_FP64_IEEE or
+ * CCOIL_FP32C (near-winding pairs in FP64). Deterministic: partial sums are reduced in a fixed order. */
+ccoil_status ccoil_field_gradient(ccoil_ctx ctx, const ccoil_coils *coils, const double *points, int64_t n_points,
+                                  const double *dL_dB, int32_t precision, uint32_t flags, double *grad,
+                                  ccoil_report *report);
+
 #ifdef __cplusplus
 }
 #endif
