@@ -4,21 +4,24 @@
 Timed workload (one step = one pass of the hot path over the whole batch, inputs resident in HBM):
   B and the gradient tensor of a 128-coil mixed CoilGroup (FILAMENT/THIN/FLAT/RECTANGULAR, R ~ U[0.02, 0.5] m,
   a, b ~ U[0.005, 0.2] m, N ~ U{1..2000}, I ~ U[-10, 10] A, centres ~ U[-1, 1]^3 m, random orientation) at
-  2^20 points ~ U[-1.5, 1.5]^3 m, PrecisionFactor 5, FP32C precision (FP32 nodes, FP64 row sums, FP64
-  near-winding re-evaluation), through ccoil_fields with device pointers.
+  2^20 points ~ U[-1.5, 1.5]^3 m, PrecisionFactor 5, through one ccoil_fields call with device pointers, in
+  CCOIL_FP64 (the cancellation-free integrands in double with Newton-refined hardware reciprocals: the mode that
+  meets the 1e-6 relative-error bar; it agrees with IEEE double to ~1e-14).
 value = coil·point evaluations of B + gradient per second (coils x points / seconds per step).
 
-Beside it, untimed by the contract and reported in the same line:
-  fp64        the same pass in CCOIL_FP64 (stable integrands in double) and CCOIL_FP64_REFERENCE (the
-              reference's integrands in double), and the FP32C speed-up over each;
-  accuracy    FP32C against CCOIL_FP64 at every point: max and p99 of the per-point relative error of B and of
-              the gradient, split into points inside the near-winding shell of any coil and the rest;
-  pairs       mutual inductance + force + torque of 65,536 random coil pairs with per-dimension increments
-              ~ U{4, 8, ..., 80} (one ccoil_pairs call), FP32C and FP64, pairs/s and accuracy;
-  roofline    the field kernel's algorithmic FP32 flops (frozen per-node model, roofline_model.py) per launch
-              over its hipEvent-timed duration on the library's stream, against the MI355X FP32 vector peak;
-  cpu_baseline the FP64 CPU restatement of the reference's CPU_MT field path (-O3 -mavx2 -ffast-math, point
-              chunks on a thread pool) on a bounded sample of the same workload on this host.
+Reported beside it in the same line:
+  side         the same pass in CCOIL_FP32C (FP32 nodes, FP64 row sums, FP64 near-winding pairs), CCOIL_FP64_IEEE
+               (same integrands, IEEE division and sqrt) and CCOIL_FP64_REFERENCE (the reference's integrands in
+               double: the plain-FP64 comparator): speed and roofline fraction;
+  accuracy     FP32C and FP64_IEEE against CCOIL_FP64 at every point (max / p99 / p99.9, points inside the
+               near-winding shell of any coil and the rest);
+  pairs        mutual inductance + force + torque of 65,536 random coil pairs with per-dimension increments
+               ~ U{4, 8, ..., 80} (one ccoil_pairs call) in CCOIL_FP64 and CCOIL_FP32C: pairs/s, node-pairs/s and the
+               FP32C error;
+  roofline     field_kernel's algorithmic flops per launch (frozen per-node model, roofline_model.py) over its
+               hipEvent-timed duration on the library's stream, against the MI355X vector peak of its type;
+  cpu_baseline the FP64 CPU restatement of the reference's CPU_MT field path (-O3 -mavx2 -ffast-math, point chunks on
+               a thread pool), timed on a bounded sample of the same workload on this host.
 """
 import argparse
 import json
@@ -34,7 +37,7 @@ if PKG not in sys.path:
     sys.path.insert(0, PKG)
 
 METRIC = "coil·point B+∇B evals/s and M-pairs/s, FP32-comp vs FP64 at ≤1e-6 rel err"
-PMC_FILE = os.path.join(ROOT, "profiles", "field_kernel_group128_pmc.txt")
+PMC_FILE = os.path.join(ROOT, "profiles", "field_kernel_fp64_group128_points1M_pmc.md")
 
 
 def log(*a):
@@ -52,8 +55,9 @@ def rel_err(x, ref):
 
 def stats(e):
     if e.size == 0:
-        return {"max": None, "p99": None, "n": 0}
-    return {"max": float(np.max(e)), "p99": float(np.percentile(e, 99)), "n": int(e.size)}
+        return {"max": None, "p99": None, "p999": None, "n": 0}
+    return {"max": float(np.max(e)), "p99": float(np.percentile(e, 99)), "p999": float(np.percentile(e, 99.9)),
+            "n": int(e.size)}
 
 
 def pmc_traffic():
@@ -95,9 +99,10 @@ def main():
     ap.add_argument("--coils", type=int, default=128)
     ap.add_argument("--points", type=int, default=1 << 20)
     ap.add_argument("--pairs", type=int, default=65536)
-    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--side-steps", type=int, default=3, help="timed steps of each secondary field mode")
     ap.add_argument("--cpu-points", type=int, default=65536)
-    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 legs and the accuracy report")
+    ap.add_argument("--precision", choices=("fp64", "fp32c"), default="fp64", help="mode of the timed steps")
+    ap.add_argument("--no-side", action="store_true", help="skip the secondary field modes and the accuracy report")
     ap.add_argument("--no-pairs", action="store_true", help="skip the pair leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args()
@@ -124,80 +129,76 @@ def main():
     stream = torch.cuda.current_stream(dev)
     ctx.set_stream(stream.cuda_stream)
     d_pts = torch.from_numpy(np.ascontiguousarray(pts)).to(dev)
-    out = {"B": torch.empty((n, 3), dtype=torch.float64, device=dev),
-           "G": torch.empty((n, 9), dtype=torch.float64, device=dev)}
     q = abi.B | abi.G
-
-    def step(prec, o=out):
-        _, rep = ctx.fields(cs, d_pts, q, prec, abi.DEVICE_POINTERS, out=o)
-        return rep
-
-    for _ in range(args.warmup):
-        step(abi.FP32C)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    reps = [step(abi.FP32C) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    value = len(rows) * n / ((t1 - t0) / args.steps)
-    main_ms = float(np.mean([r.main_kernel_ms for r in reps]))
-    all_ms = float(np.mean([r.kernel_ms for r in reps]))
-    log(f"FP32C: {ms_per_step:.2f} ms/step, field kernel {main_ms:.2f} ms, all kernels {all_ms:.2f} ms, "
-        f"near pairs {reps[-1].near_pairs}")
-
     flops, trans = RM.field_work(rows, n, with_gradient=True)
-    achieved = flops / (main_ms * 1e-3) / 1e12
-    roofline = {"bound": "valu", "achieved": achieved, "peak": RM.PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / RM.PEAK_FP32_TFLOPS, "traffic": pmc_traffic(), "kernel": "field_kernel",
-                "kernel_ms": main_ms, "algorithmic_flops_per_launch": flops,
+
+    def new_out():
+        return {"B": torch.empty((n, 3), dtype=torch.float64, device=dev),
+                "G": torch.empty((n, 9), dtype=torch.float64, device=dev)}
+
+    def timed(prec, steps, warmup, out):
+        for _ in range(warmup):
+            ctx.fields(cs, d_pts, q, prec, abi.DEVICE_POINTERS, out=out)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        reps = [ctx.fields(cs, d_pts, q, prec, abi.DEVICE_POINTERS, out=out)[1] for _ in range(steps)]
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        main_ms = float(np.mean([r.main_kernel_ms for r in reps]))
+        all_ms = float(np.mean([r.kernel_ms for r in reps]))
+        return dt, main_ms, all_ms, reps[-1]
+
+    def roofline(main_ms, peak):
+        achieved = flops / (main_ms * 1e-3) / 1e12
+        return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                "kernel": "field_kernel", "kernel_ms": main_ms, "algorithmic_flops_per_launch": flops,
                 "transcendental": {"achieved_ops": trans / (main_ms * 1e-3), "peak_ops": RM.PEAK_TRANS_OPS,
                                    "frac": trans / (main_ms * 1e-3) / RM.PEAK_TRANS_OPS},
                 "hbm_algorithmic_bytes_per_launch": n * (24 + 24 + 72)}
 
+    fp64_head = args.precision == "fp64"
+    head_prec = abi.FP64 if fp64_head else abi.FP32C
+    head = new_out()
+    dt, main_ms, all_ms, rep = timed(head_prec, args.steps, args.warmup, head)
+    value = len(rows) * n / dt
+    log(f"{args.precision}: {dt * 1e3:.2f} ms/step, field kernel {main_ms:.2f} ms, all kernels {all_ms:.2f} ms")
+    rl = roofline(main_ms, RM.PEAK_FP64_TFLOPS if fp64_head else RM.PEAK_FP32_TFLOPS)
+    rl["traffic"] = pmc_traffic() if fp64_head else None
     result = {"metric": METRIC, "value": value, "unit": "coil·point evals/s", "n_gpus": 1,
-              "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
-              "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-              "config": {"workload": f"B+gradient of a {len(rows)}-coil mixed CoilGroup at {n} points, FP32C",
+              "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
+              "scaling": "weak", "vs_baseline": None, "dtype": "f64" if fp64_head else "f32", "data": "synthetic",
+              "config": {"workload": f"B+gradient of a {len(rows)}-coil mixed CoilGroup at {n} points",
                          "coils": len(rows), "points": n, "precision_factor": 5.0, "quantities": "B+G",
-                         "precision": "FP32C", "seed": 0},
-              "roofline": roofline, "kernel_ms": {"field_kernel": main_ms, "all": all_ms},
-              "near_pairs": int(reps[-1].near_pairs)}
+                         "precision": "CCOIL_FP64" if fp64_head else "CCOIL_FP32C", "seed": 0},
+              "roofline": rl, "kernel_ms": {"field_kernel": main_ms, "all": all_ms}}
 
-    if not args.no_fp64:
-        ref = {"B": torch.empty_like(out["B"]), "G": torch.empty_like(out["G"])}
-        fp64 = {}
-        # CCOIL_FP64 last, so that ref holds its output for the accuracy report; out still holds FP32C and
-        # ieee the CCOIL_FP64_IEEE output
-        ieee = {"B": torch.empty_like(out["B"]), "G": torch.empty_like(out["G"])}
-        for name, prec, o in (("fp64_reference", abi.FP64_REFERENCE, ref), ("fp64_ieee", abi.FP64_IEEE, ieee),
-                              ("fp64", abi.FP64, ref)):
-            step(prec, o)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            r64 = [step(prec, o) for _ in range(args.fp64_steps)]
-            torch.cuda.synchronize()
-            dt = (time.perf_counter() - t0) / args.fp64_steps
-            fp64[name] = {"value": len(rows) * n / dt, "ms_per_step": dt * 1e3,
-                          "kernel_ms": float(np.mean([r.main_kernel_ms for r in r64]))}
-            log(f"{name}: {dt * 1e3:.1f} ms/step")
-        fp64["fp32c_speedup_vs_fp64"] = value / fp64["fp64"]["value"]
-        fp64["fp32c_speedup_vs_fp64_reference"] = value / fp64["fp64_reference"]["value"]
-        fp64["fp32c_speedup_vs_fp64_ieee"] = value / fp64["fp64_ieee"]["value"]
-        result["fp64"] = fp64
-        b32, g32 = out["B"].cpu().numpy(), out["G"].cpu().numpy()
-        b64, g64 = ref["B"].cpu().numpy(), ref["G"].cpu().numpy()
-        near = W.near_mask(rows, pts)
-        eb, okb = rel_err(b32, b64)
-        eg, okg = rel_err(g32, g64)
-        result["accuracy"] = {
-            "against": "CCOIL_FP64 on the same points", "near_points": int(near.sum()),
-            "B": {"far": stats(eb[~near[okb]]), "near": stats(eb[near[okb]])},
-            "G": {"far": stats(eg[~near[okg]]), "near": stats(eg[near[okg]])}}
-        e1, _ = rel_err(ieee["B"].cpu().numpy(), b64)
-        e2, _ = rel_err(ieee["G"].cpu().numpy(), g64)
-        result["accuracy"]["fp64_vs_fp64_ieee"] = {"B": stats(e1), "G": stats(e2)}
-        log(f"accuracy: {json.dumps(result['accuracy'])}")
+    if not args.no_side:
+        side, outs = {}, {}
+        for name, prec, peak in (("fp32c", abi.FP32C, RM.PEAK_FP32_TFLOPS),
+                                 ("fp64_ieee", abi.FP64_IEEE, RM.PEAK_FP64_TFLOPS),
+                                 ("fp64_reference", abi.FP64_REFERENCE, RM.PEAK_FP64_TFLOPS)):
+            if prec == head_prec:
+                continue
+            outs[name] = new_out()
+            sdt, sms, sall, srep = timed(prec, args.side_steps, 1, outs[name])
+            side[name] = {"value": len(rows) * n / sdt, "ms_per_step": sdt * 1e3, "kernel_ms": sms, "all_ms": sall,
+                          "roofline_frac": roofline(sms, peak)["frac"], "near_pairs": int(srep.near_pairs),
+                          "value_over_headline": (len(rows) * n / sdt) / value}
+            log(f"{name}: {sdt * 1e3:.1f} ms/step, field kernel {sms:.2f} ms")
+        result["side"] = side
+        if fp64_head:
+            b64, g64 = head["B"].cpu().numpy(), head["G"].cpu().numpy()
+            near = W.near_mask(rows, pts)
+            acc = {"against": "CCOIL_FP64 on the same points", "near_points": int(near.sum())}
+            for name in ("fp32c", "fp64_ieee"):
+                eb, okb = rel_err(outs[name]["B"].cpu().numpy(), b64)
+                eg, okg = rel_err(outs[name]["G"].cpu().numpy(), g64)
+                acc[name] = {"B": {"far": stats(eb[~near[okb]]), "near": stats(eb[near[okb]])},
+                             "G": {"far": stats(eg[~near[okg]]), "near": stats(eg[near[okg]])}}
+            result["accuracy"] = acc
+            log(f"accuracy: {json.dumps(acc)}")
+        if "fp64_reference" in side:
+            result["speedup_vs_plain_fp64"] = value / side["fp64_reference"]["value"]
 
     if not args.no_pairs:
         prow, srow = W.pair_workload(args.pairs, seed=0)
@@ -205,24 +206,23 @@ def main():
         what = abi.PAIR_M | abi.PAIR_FT
         pf, ptrans, pnodes = RM.pair_work(prow, srow)
         pres = {}
-        for name, prec, steps in (("fp32c", abi.FP32C, 2), ("fp64", abi.FP64, 1)):
-            o, rep = ctx.pairs(P, S, len(prow), what, prec)
+        for name, prec in (("fp64", abi.FP64), ("fp32c", abi.FP32C)):
+            ctx.pairs(P, S, len(prow), what, prec)
             t0 = time.perf_counter()
-            for _ in range(steps):
-                o, rep = ctx.pairs(P, S, len(prow), what, prec)
-            dt = (time.perf_counter() - t0) / steps
-            pres[name] = {"pairs_per_s": len(prow) / dt, "ms_per_step": dt * 1e3, "kernel_ms": rep.main_kernel_ms,
-                          "node_pairs_per_s": pnodes / (rep.main_kernel_ms * 1e-3),
-                          "tflops_algorithmic": pf / (rep.main_kernel_ms * 1e-3) / 1e12, "out": o}
-            log(f"pairs {name}: {dt * 1e3:.1f} ms/step, kernel {rep.main_kernel_ms:.1f} ms")
-        o32, o64 = pres["fp32c"].pop("out"), pres["fp64"].pop("out")
+            o, prep = ctx.pairs(P, S, len(prow), what, prec)
+            dt_p = time.perf_counter() - t0
+            pres[name] = {"pairs_per_s": len(prow) / dt_p, "ms_per_step": dt_p * 1e3,
+                          "kernel_ms": prep.main_kernel_ms,
+                          "node_pairs_per_s": pnodes / (prep.main_kernel_ms * 1e-3),
+                          "tflops_algorithmic": pf / (prep.main_kernel_ms * 1e-3) / 1e12, "out": o}
+            log(f"pairs {name}: {dt_p * 1e3:.1f} ms/step, kernel {prep.main_kernel_ms:.1f} ms")
+        o64, o32 = pres["fp64"].pop("out"), pres["fp32c"].pop("out")
         em = np.abs(o32["M"] - o64["M"]) / np.abs(o64["M"])
         ef, _ = rel_err(o32["F"], o64["F"])
         et, _ = rel_err(o32["T"], o64["T"])
         pres["accuracy_fp32c_vs_fp64"] = {"M": stats(em), "F": stats(ef), "T": stats(et)}
         pres["workload"] = (f"{len(prow)} random mixed pairs, increments ~ U{{4,...,80}} per active dimension, "
                             f"{pnodes:.3g} node-pairs, M+F+T")
-        pres["fp32c_speedup_vs_fp64"] = pres["fp32c"]["pairs_per_s"] / pres["fp64"]["pairs_per_s"]
         result["pairs"] = pres
 
     if not args.no_cpu:
