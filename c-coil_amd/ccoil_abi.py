@@ -16,7 +16,7 @@ OK, ERR_ARGUMENT, ERR_OUT_OF_MEMORY, ERR_DEVICE, ERR_NO_DEVICE = 0, -1, -2, -3, 
 A, B, E, G, G_JACOBIAN = 1, 2, 4, 8, 16
 FP64, FP32C, FP32, FP64_REFERENCE, FP64_IEEE = 0, 1, 2, 3, 4
 DEVICE_POINTERS, NO_NEAR_FP64, FORCE_REFERENCE_SCALE = 1, 2, 4
-PAIR_M, PAIR_FT = 1, 2
+PAIR_M, PAIR_FT, PAIR_DM = 1, 2, 4
 
 EXPORTED = ["ccoil_create", "ccoil_destroy", "ccoil_last_error", "ccoil_version", "ccoil_set_stream",
             "ccoil_synchronize", "ccoil_gauss_legendre", "ccoil_fields", "ccoil_pairs", "ccoil_field_gradient"]
@@ -63,7 +63,7 @@ def lib():
         L.ccoil_fields.argtypes = [vp, ctypes.POINTER(Coils), vp, ctypes.c_int64, ctypes.c_uint32, ctypes.c_int32,
                                    ctypes.c_uint32, vp, vp, vp, vp, ctypes.POINTER(Report)]
         L.ccoil_pairs.argtypes = [vp, ctypes.POINTER(Coils), ctypes.POINTER(Coils), vp, vp, ctypes.c_int64, vp, vp,
-                                  ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32, vp, vp, vp, ctypes.POINTER(Report)]
+                                  ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32, vp, vp, vp, vp, ctypes.POINTER(Report)]
         L.ccoil_field_gradient.argtypes = [vp, ctypes.POINTER(Coils), vp, ctypes.c_int64, vp, ctypes.c_int32,
                                            ctypes.c_uint32, vp, ctypes.POINTER(Report)]
         for f in ("ccoil_create", "ccoil_destroy", "ccoil_set_stream", "ccoil_synchronize", "ccoil_gauss_legendre",
@@ -197,11 +197,14 @@ class Context:
             if what & PAIR_FT:
                 out["F"] = np.zeros((n_pairs, 3))
                 out["T"] = np.zeros((n_pairs, 3))
+            if what & PAIR_DM:
+                out["dM"] = np.zeros((n_pairs, 3))
         pi = None if prim_index is None else np.ascontiguousarray(prim_index, dtype=np.int64)
         si = None if sec_index is None else np.ascontiguousarray(sec_index, dtype=np.int64)
         ro = None if ring_offset is None else np.ascontiguousarray(ring_offset, dtype=np.float64)
         hr = None if half_ring is None else np.ascontiguousarray(half_ring, dtype=np.int32)
         self._check(lib().ccoil_pairs(self.h, ctypes.byref(prim.struct), ctypes.byref(sec.struct), _ptr(pi), _ptr(si),
                                       n_pairs, _ptr(ro), _ptr(hr), what, precision, flags, _ptr(out.get("M")),
-                                      _ptr(out.get("F")), _ptr(out.get("T")), ctypes.byref(rep)))
+                                      _ptr(out.get("F")), _ptr(out.get("T")), _ptr(out.get("dM")),
+                                      ctypes.byref(rep)))
         return out, rep

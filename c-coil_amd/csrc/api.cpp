@@ -225,7 +225,7 @@ ccoil_status ccoil_destroy(ccoil_ctx ctx)
         if (p) (void)hipFree(p);
     DevBuf *bufs[] = {&ctx->coils, &ctx->pts,   &ctx->outA,  &ctx->outB,     &ctx->outE,     &ctx->outG,
                       &ctx->queue, &ctx->counter, &ctx->pprim, &ctx->psec, &ctx->ppairs, &ctx->ppartial,
-                      &ctx->pcounter, &ctx->pM, &ctx->pF, &ctx->pT, &ctx->acoils, &ctx->apts, &ctx->agB,
+                      &ctx->pcounter, &ctx->pM, &ctx->pF, &ctx->pT, &ctx->pdM, &ctx->acoils, &ctx->apts, &ctx->agB,
                       &ctx->apartial, &ctx->agrad};
     for (auto b : bufs) free_buf(*b);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

@@ -195,6 +195,37 @@ template <> __device__ inline double prcp<f64s>(double x)
     return fma(y, fma(e, e, e), y);                        // y (1 + e + e^2)
 }
 
+// log(1 + x) for x >= 0: 2 atanh(u), u = x / (2 + x), by its odd series while u <= 0.2 (x <= 0.5), else log.
+// The series is exact to the type's precision there (u^21 / 21 < 1e-16 in double, u^11 / 11 < 1e-8 in float) and
+// keeps full relative accuracy for small x, where log(1 + x) would lose it.
+template <typename P> __device__ inline real_t<P> log1p_pos(real_t<P> x)
+{
+    using T = real_t<P>;
+    if (x > (T)0.5) return log_<T>((T)1 + x);
+    const T u = x * prcp<P>((T)2 + x), u2 = u * u;
+    T s;
+    if constexpr (sizeof(T) == 8) {
+        s = (T)(1.0 / 21);
+        s = fma_<T>(s, u2, (T)(1.0 / 19));
+        s = fma_<T>(s, u2, (T)(1.0 / 17));
+        s = fma_<T>(s, u2, (T)(1.0 / 15));
+        s = fma_<T>(s, u2, (T)(1.0 / 13));
+        s = fma_<T>(s, u2, (T)(1.0 / 11));
+        s = fma_<T>(s, u2, (T)(1.0 / 9));
+        s = fma_<T>(s, u2, (T)(1.0 / 7));
+        s = fma_<T>(s, u2, (T)(1.0 / 5));
+        s = fma_<T>(s, u2, (T)(1.0 / 3));
+    } else {
+        s = (T)(1.0 / 11);
+        s = fma_<T>(s, u2, (T)(1.0 / 9));
+        s = fma_<T>(s, u2, (T)(1.0 / 7));
+        s = fma_<T>(s, u2, (T)(1.0 / 5));
+        s = fma_<T>(s, u2, (T)(1.0 / 3));
+    }
+    s = fma_<T>(s, u2, (T)1);
+    return (T)2 * u * s;
+}
+
 template <typename T> __device__ inline const T *tab_omc(const Tables &t);
 template <> __device__ inline const double *tab_omc<double>(const Tables &t) { return t.omc; }
 template <> __device__ inline const float *tab_omc<float>(const Tables &t) { return t.omcf; }
@@ -237,6 +268,8 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
         const T za1s = (T)((zad + 0.5 * c.b) * (zad + 0.5 * c.b)), za2s = (T)((zad - 0.5 * c.b) * (zad - 0.5 * c.b));
         const T tbza = (T)(2.0 * c.b * zad);
         const bool samea = (zad - 0.5 * c.b) > 0.0;
+        const bool zpos = zd >= 0.0;
+        const T bf = (T)c.b;
         for (int j = row0; j < nt; j += rstep) {
             const double rho_d = c.R + 0.5 * c.a * (1.0 + xt[j]);
             const double dd = rho_d - rd;
@@ -289,16 +322,24 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                     }
                 }
                 if (WA) {
-                    const T J1 = M + za1s, J2 = M + za2s;
-                    const T K1 = prsq<P>(J1), K2 = prsq<P>(J2);
-                    const T sJ1 = J1 * K1, sJ2 = J2 * K2;                   // sqrt(J)
-                    // asinh(z1 / sqrt(M)) - asinh(z2 / sqrt(M)) = asinh(t)
-                    const T num = samea ? tbza : (za1 * sJ2 - za2 * sJ1);
-                    const T den = samea ? (za1 * sJ2 + za2 * sJ1) : M;
-                    const T t = num * prcp<P>(den);
-                    const T y = sqrt_<T>(fma_<T>(t, t, (T)1));
-                    const T as = log1p_<T>(t + t * t * prcp<P>((T)1 + y));
-                    rA = fma_<T>(w, cp * as, rA);
+                    // asinh(za1 / sqrt(M)) - asinh(za2 / sqrt(M)) = log(P1 / P2), P_k = za_k + sqrt(M + za_k^2),
+                    // = log1p(x) with x = (P1 - P2) / P2 = (b (sJ1 + sJ2) + 2 b |z|) / ((sJ1 + sJ2) P2) >= 0;
+                    // for za2 < 0, P2 = M / (sJ2 - za2). {za1^2, za2^2} = {z1^2, z2^2}, so the B terms' K serve.
+                    T sJa1, sJa2;
+                    if (WB || WG) {
+                        const T J1 = M + z1s, J2 = M + z2s;
+                        const T K1 = prsq<P>(J1), K2 = prsq<P>(J2);   // the same values as above (CSE)
+                        sJa1 = zpos ? J1 * K1 : J2 * K2;
+                        sJa2 = zpos ? J2 * K2 : J1 * K1;
+                    } else {
+                        const T J1 = M + za1s, J2 = M + za2s;
+                        sJa1 = J1 * prsq<P>(J1);
+                        sJa2 = J2 * prsq<P>(J2);
+                    }
+                    const T sum = sJa1 + sJa2;
+                    const T numr = fma_<T>(bf, sum, tbza);
+                    const T x = samea ? numr * prcp<P>(sum * (za2 + sJa2)) : numr * (sJa2 - za2) * prcp<P>(sum * M);
+                    rA = fma_<T>(w, cp * log1p_pos<P>(x), rA);
                 }
             }
             const double W = c.cf * wt[j] * rho_d;

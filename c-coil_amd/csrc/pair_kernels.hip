@@ -4,7 +4,8 @@
 // item = blockIdx.x. Items are laid out longest-first on the host, and workgroups are dispatched in index
 // order, so the launch is an LPT schedule without a shared cursor. Each lane evaluates the primary's A or B
 // at one secondary node at a time (the primary coil is wave-uniform, so its loops and tables are scalar),
-// the 7 weighted sums are reduced across the wave in FP64 and one partial is written per item. A second
+// the weighted sums (M, F, T and the gradient of M) are reduced across the wave in FP64 and one partial is
+// written per item. A second
 // kernel adds the partials of each pair in item order, so results are deterministic. The reference instead
 // runs one thread per (configuration, secondary node) and FP32 atomicAdds per configuration
 // (MInductance.cu:157, Force.cu:172-178).
@@ -41,7 +42,31 @@ __device__ inline void eval_prim(const CoilDev &pc, const Tables &tab, double z,
     }
 }
 
-template <int PREC, bool WM, bool WF>
+// Gradient of M with respect to the secondary's position: M = sum_n w A(x_n) . t_n over the secondary nodes x_n,
+// so dM/dp = sum_n w J_A(x_n)^T t_n with J_A the Jacobian of the primary's A. With A = A_phi(r, z) e_phi and the
+// discrete B = curl A of the same rule, dA_phi/dz = -B_rho and dA_phi/dr = B_z - A_phi / r, so J_A follows from
+// (A_phi, B_rho, B_z) at the node (on the axis, A_phi / r and dA_phi/dr tend to B_z / 2).
+__device__ inline void grad_m_term(const CoilDev &pc, const Local &L, double r, double c, double s,
+                                   const double *tan, double *out)
+{
+    double tl[3];
+    mat_vec(pc.Tinv, tan, tl);
+    double aor, dra;
+    if (r / pc.R <= kGpuAxisRatio) {
+        aor = 0.5 * L.Bz;
+        dra = aor;
+    } else {
+        aor = L.A / r;
+        dra = L.Bz - aor;
+    }
+    const double dd = dra - aor;
+    const double g[3] = {tl[0] * (-c * s * dd) + tl[1] * (c * c * dra + s * s * aor),
+                         tl[0] * (-s * s * dra - c * c * aor) + tl[1] * (c * s * dd),
+                         (tl[0] * s - tl[1] * c) * L.Bh};
+    mat_vec(pc.T, g, out);
+}
+
+template <int PREC, bool WM, bool WF, bool WD>
 __global__ __launch_bounds__(64) void pair_kernel(PairParams p)
 {
     const int lane = threadIdx.x;
@@ -66,7 +91,7 @@ __global__ __launch_bounds__(64) void pair_kernel(PairParams p)
         const double u[3] = {sc.cb * sc.ca, sc.sb * sc.ca, -sc.sa};
         const double v[3] = {-sc.sb, sc.cb, 0.0};
         const int oa = table_offset(sc.n_ang), ot = table_offset(sc.n_thk), ol = table_offset(sc.n_len);
-        double m = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0;
+        double m = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0;
         for (int64_t k = n0 + lane; k < n1; k += 64) {
             const int ia = (int)(k % NA);
             const int64_t rest = k / NA;
@@ -91,7 +116,7 @@ __global__ __launch_bounds__(64) void pair_kernel(PairParams p)
             double z, r, pcph, psph;
             to_local(pc, sc.pos[0] + arm[0], sc.pos[1] + arm[1], sc.pos[2] + arm[2], z, r, pcph, psph);
             Local L = {0, 0, 0, 0, 0, 0, 0};
-            eval_prim<PREC, WM, WF>(pc, p.tab, z, r, L);
+            eval_prim<PREC, WM || WD, WF || WD>(pc, p.tab, z, r, L);
             if (WM) {
                 const double vl[3] = {-L.A * psph, L.A * pcph, 0.0};
                 double ag[3];
@@ -112,8 +137,15 @@ __global__ __launch_bounds__(64) void pair_kernel(PairParams p)
                 t1 += arm[2] * df0 - arm[0] * df2;
                 t2 += arm[0] * df1 - arm[1] * df0;
             }
+            if (WD) {
+                double gd[3];
+                grad_m_term(pc, L, r, pcph, psph, tan, gd);
+                d0 += w * gd[0];
+                d1 += w * gd[1];
+                d2 += w * gd[2];
+            }
         }
-        double *out = p.partial + 7 * item;
+        double *out = p.partial + kPairTerms * item;
         if (WM) {
             m = wave_sum(m);
             if (lane == 0) out[0] = m;
@@ -126,6 +158,12 @@ __global__ __launch_bounds__(64) void pair_kernel(PairParams p)
                 out[4] = t0; out[5] = t1; out[6] = t2;
             }
         }
+        if (WD) {
+            d0 = wave_sum(d0); d1 = wave_sum(d1); d2 = wave_sum(d2);
+            if (lane == 0) {
+                out[7] = d0; out[8] = d1; out[9] = d2;
+            }
+        }
     }
 }
 
@@ -135,24 +173,34 @@ __global__ __launch_bounds__(256) void pair_reduce_kernel(PairParams p)
     if (k >= p.n_pairs) return;
     const PairDev &pd = p.pairs[k];
     const int64_t end = (k + 1 < p.n_pairs) ? p.pairs[k + 1].first_item : p.n_items;
-    double s[7] = {0, 0, 0, 0, 0, 0, 0};
+    double s[kPairTerms];
+    for (int q = 0; q < kPairTerms; ++q) s[q] = 0.0;
     for (int64_t it = pd.first_item; it < end; ++it)
-        for (int q = 0; q < 7; ++q) s[q] += p.partial[7 * it + q];
+        for (int q = 0; q < kPairTerms; ++q) s[q] += p.partial[kPairTerms * it + q];
     if (p.what & 1) p.M[pd.out] = s[0] * pd.m_scale;
     if (p.what & 2) {
         for (int q = 0; q < 3; ++q) p.F[3 * pd.out + q] = s[1 + q] * pd.f_scale;
         for (int q = 0; q < 3; ++q) p.T[3 * pd.out + q] = s[4 + q] * pd.f_scale;
     }
+    if (p.what & 4)
+        for (int q = 0; q < 3; ++q) p.dM[3 * pd.out + q] = s[7 + q] * pd.m_scale;
 }
 
 template <int PREC>
 static void launch_pairs_p(hipStream_t s, const PairParams &p)
 {
-    const bool wm = p.what & 1, wf = p.what & 2;
+    const bool wm = p.what & 1, wf = p.what & 2, wd = p.what & 4;
     const dim3 grid((unsigned)p.n_items);
-    if (wm && wf) hipLaunchKernelGGL((pair_kernel<PREC, true, true>), grid, dim3(64), 0, s, p);
-    else if (wm) hipLaunchKernelGGL((pair_kernel<PREC, true, false>), grid, dim3(64), 0, s, p);
-    else hipLaunchKernelGGL((pair_kernel<PREC, false, true>), grid, dim3(64), 0, s, p);
+    if (wd) {
+        // the gradient needs A and B at every node; M, F and T come with it at little extra cost
+        hipLaunchKernelGGL((pair_kernel<PREC, true, true, true>), grid, dim3(64), 0, s, p);
+    } else if (wm && wf) {
+        hipLaunchKernelGGL((pair_kernel<PREC, true, true, false>), grid, dim3(64), 0, s, p);
+    } else if (wm) {
+        hipLaunchKernelGGL((pair_kernel<PREC, true, false, false>), grid, dim3(64), 0, s, p);
+    } else {
+        hipLaunchKernelGGL((pair_kernel<PREC, false, true, false>), grid, dim3(64), 0, s, p);
+    }
 }
 
 void launch_pairs(hipStream_t s, const PairParams &p)

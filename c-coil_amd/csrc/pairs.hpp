@@ -26,12 +26,13 @@ struct PairParams {
     const PairDev *pairs;
     int32_t n_pairs;
     int64_t n_items;
-    double *partial;                 // 7 doubles per item: M, F[3], T[3]
-    uint32_t what;                   // 1: M, 2: F and T
+    double *partial;                 // kPairTerms doubles per item: M, F[3], T[3], dM/dp[3]
+    uint32_t what;                   // 1: M, 2: F and T, 4: dM/d(secondary position)
     int32_t precision;
     Tables tab;
-    double *M, *F, *T;               // outputs (device), indexed by PairDev::out
+    double *M, *F, *T, *dM;          // outputs (device), indexed by PairDev::out
 };
+constexpr int kPairTerms = 10;
 
 // One 64-lane workgroup per work item (grid = n_items, at most 2^31 - 1).
 void launch_pairs(hipStream_t s, const PairParams &p);

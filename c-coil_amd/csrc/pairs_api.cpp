@@ -25,18 +25,21 @@ extern "C" ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, cons
                                     const int64_t *prim_index, const int64_t *sec_index, int64_t n_pairs,
                                     const double *ring_offset, const int32_t *half_ring, uint32_t what,
                                     int32_t precision, uint32_t flags, double *M, double *F, double *T,
-                                    ccoil_report *report)
+                                    double *dM, ccoil_report *report)
 {
     if (!ctx) return fail(nullptr, CCOIL_ERR_ARGUMENT, "ccoil_pairs: ctx is NULL");
     if (n_pairs < 0) return fail(ctx, CCOIL_ERR_ARGUMENT, "n_pairs is negative");
     if (n_pairs > INT32_MAX) return fail(ctx, CCOIL_ERR_ARGUMENT, "n_pairs exceeds 2^31 - 1");
-    if (what == 0 || (what & ~(uint32_t)(CCOIL_PAIR_M | CCOIL_PAIR_FT)))
+    if (what == 0 || (what & ~(uint32_t)(CCOIL_PAIR_M | CCOIL_PAIR_FT | CCOIL_PAIR_DM)))
         return fail(ctx, CCOIL_ERR_ARGUMENT, "bad pair quantity mask");
     if (precision != CCOIL_FP64 && precision != CCOIL_FP32C && precision != CCOIL_FP32 &&
         precision != CCOIL_FP64_REFERENCE && precision != CCOIL_FP64_IEEE)
         return fail(ctx, CCOIL_ERR_ARGUMENT, "unknown precision");
     if ((what & CCOIL_PAIR_M) && !M) return fail(ctx, CCOIL_ERR_ARGUMENT, "M requested but M is NULL");
     if ((what & CCOIL_PAIR_FT) && (!F || !T)) return fail(ctx, CCOIL_ERR_ARGUMENT, "F/T requested but F or T is NULL");
+    if ((what & CCOIL_PAIR_DM) && !dM) return fail(ctx, CCOIL_ERR_ARGUMENT, "dM requested but dM is NULL");
+    if ((what & CCOIL_PAIR_DM) && (precision == CCOIL_FP32 || precision == CCOIL_FP64_REFERENCE))
+        return fail(ctx, CCOIL_ERR_ARGUMENT, "CCOIL_PAIR_DM needs CCOIL_FP64, CCOIL_FP64_IEEE or CCOIL_FP32C");
     std::vector<CoilDev> hp, hs;
     std::string msg = pack_coils(prim, false, false, hp);
     if (msg.empty()) msg = pack_coils(sec, true, false, hs);
@@ -66,7 +69,7 @@ extern "C" ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, cons
         d.ring_offset = ring_offset ? ring_offset[k] : 0.0;
         d.half_ring = half_ring ? (half_ring[k] ? 1 : 0) : 0;
         const double blocks = (double)s.blk_ang * s.blk_thk * s.blk_len;
-        if (p.current == 0.0 && (what & CCOIL_PAIR_M))
+        if (p.current == 0.0 && (what & (CCOIL_PAIR_M | CCOIL_PAIR_DM)))
             return fail(ctx, CCOIL_ERR_ARGUMENT, "pair " + std::to_string(k) + ": primary current is zero");
         d.m_scale = 2.0 * kPi * s.turns / (p.current * blocks);
         d.f_scale = ref_scale ? 2.0 * kPi * s.turns / (p.current * blocks) : 2.0 * kPi * s.turns * s.current / blocks;
@@ -99,7 +102,7 @@ extern "C" ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, cons
     CC_HIP(ctx, ensure(ctx->pprim, sizeof(CoilDev) * hp.size()));
     CC_HIP(ctx, ensure(ctx->psec, sizeof(CoilDev) * hs.size()));
     CC_HIP(ctx, ensure(ctx->ppairs, sizeof(PairDev) * sorted.size()));
-    CC_HIP(ctx, ensure(ctx->ppartial, sizeof(double) * 7 * (size_t)items));
+    CC_HIP(ctx, ensure(ctx->ppartial, sizeof(double) * kPairTerms * (size_t)items));
     CC_HIP(ctx, hipMemcpyAsync(ctx->pprim.p, hp.data(), sizeof(CoilDev) * hp.size(), hipMemcpyHostToDevice, ctx->stream));
     CC_HIP(ctx, hipMemcpyAsync(ctx->psec.p, hs.data(), sizeof(CoilDev) * hs.size(), hipMemcpyHostToDevice, ctx->stream));
     CC_HIP(ctx, hipMemcpyAsync(ctx->ppairs.p, sorted.data(), sizeof(PairDev) * sorted.size(), hipMemcpyHostToDevice,
@@ -120,7 +123,9 @@ extern "C" ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, cons
         pp.M = M;
         pp.F = F;
         pp.T = T;
+        pp.dM = dM;
     } else {
+        if (what & CCOIL_PAIR_DM) { CC_HIP(ctx, ensure(ctx->pdM, s3)); pp.dM = (double *)ctx->pdM.p; }
         if (what & CCOIL_PAIR_M) { CC_HIP(ctx, ensure(ctx->pM, s1)); pp.M = (double *)ctx->pM.p; }
         if (what & CCOIL_PAIR_FT) {
             CC_HIP(ctx, ensure(ctx->pF, s3));
@@ -142,6 +147,7 @@ extern "C" ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, cons
             CC_HIP(ctx, hipMemcpyAsync(F, pp.F, s3, hipMemcpyDeviceToHost, ctx->stream));
             CC_HIP(ctx, hipMemcpyAsync(T, pp.T, s3, hipMemcpyDeviceToHost, ctx->stream));
         }
+        if (what & CCOIL_PAIR_DM) CC_HIP(ctx, hipMemcpyAsync(dM, pp.dM, s3, hipMemcpyDeviceToHost, ctx->stream));
     }
     CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (report) {

@@ -390,7 +390,7 @@ void pairGPU(const Coil &prim, const Coil &sec, const CoilPairArguments &args, u
     }
     ccoil_ctx ctx = ccoil::bridge::context();
     ccoil::bridge::check(ccoil_pairs(ctx, &pv, &sv, nullptr, nullptr, 1, &offset, &half, what,
-                                     ccoil::bridge::precision(), 0, M, F, T, nullptr),
+                                     ccoil::bridge::precision(), 0, M, F, T, nullptr, nullptr),
                          ctx, "ccoil_pairs");
 }
 
@@ -468,7 +468,7 @@ void arrangementsGPU(const Coil &prim, const Coil &sec, const vec3::Vector3Array
     // The reference's single-pair force arrangement scales by 2 pi N2 / I1 (CalculateForceArrangements.cxx:125).
     const uint32_t flags = (what & CCOIL_PAIR_FT) ? CCOIL_FORCE_REFERENCE_SCALE : 0;
     ccoil::bridge::check(ccoil_pairs(ctx, &pv, &sv, nullptr, nullptr, (int64_t)pp.size(), nullptr, nullptr, what,
-                                     ccoil::bridge::precision(), flags, M, F, T, nullptr),
+                                     ccoil::bridge::precision(), flags, M, F, T, nullptr, nullptr),
                          ctx, "ccoil_pairs");
 }
 

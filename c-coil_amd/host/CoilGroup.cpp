@@ -153,7 +153,7 @@ void groupPairsGeneral(const std::vector<std::shared_ptr<Coil>> &coils, const Co
     const ccoil_coils pv = p.view(), sv = s.view();
     ccoil_ctx ctx = ccoil::bridge::context();
     ccoil::bridge::check(ccoil_pairs(ctx, &pv, &sv, nullptr, nullptr, n, offsets.data(), halves.data(), what,
-                                     ccoil::bridge::precision(), 0, m.data(), f.data(), t.data(), nullptr),
+                                     ccoil::bridge::precision(), 0, m.data(), f.data(), t.data(), nullptr, nullptr),
                          ctx, "ccoil_pairs");
     for (int64_t i = 0; i < n; ++i) {
         if (what & CCOIL_PAIR_M) *M += m[i];
@@ -216,7 +216,7 @@ void groupArrangements(const std::vector<std::shared_ptr<Coil>> &coils, const Co
     const ccoil_coils pv = p.view(), sv = s.view();
     ccoil_ctx ctx = ccoil::bridge::context();
     ccoil::bridge::check(ccoil_pairs(ctx, &pv, &sv, pi.data(), si.data(), n, nullptr, nullptr, what,
-                                     ccoil::bridge::precision(), 0, m.data(), f.data(), t.data(), nullptr),
+                                     ccoil::bridge::precision(), 0, m.data(), f.data(), t.data(), nullptr, nullptr),
                          ctx, "ccoil_pairs");
     for (int64_t k = 0; k < n; ++k) {
         const size_t c = (size_t)si[k];

@@ -186,3 +186,21 @@ def adjoint_workload(n_coils=64, n_points=4096, seed=0):
     z = 0.4 * u[:, 2] - 0.2
     pts = np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)
     return params, pts
+
+
+def secondary_rows(params, pf=5.0):
+    """C ABI rows of coils as pair secondaries: PrecisionArguments.getSecondaryCoilPrecisionArgumentsGPU."""
+    c = _c_coil()
+    rows = []
+    for p in params:
+        k = make_coil(p, pf)
+        a = c.coil.PrecisionArguments.get_secondary_coil_precision_arguments_GPU(k, c.coil.PrecisionFactor(pf))
+        rows.append(coil_row(k, a))
+    return rows
+
+
+def matrix_pairs(n):
+    """(prim_index, sec_index) of every off-diagonal entry of an n x n matrix, row-major."""
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    off = i != j
+    return i[off].astype(np.int64), j[off].astype(np.int64)

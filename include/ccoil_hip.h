@@ -73,8 +73,10 @@ enum {
                                        SupportFunctions.cxx:238-241); default is the physical 2*pi*N2*I2 */
 };
 
-/* Pair quantity mask for ccoil_pairs. */
-enum { CCOIL_PAIR_M = 1, CCOIL_PAIR_FT = 2 };
+/* Pair quantity mask for ccoil_pairs. CCOIL_PAIR_DM: gradient of M with respect to the secondary's position,
+ * the exact derivative of the same quadrature (no counterpart in the reference; F / (I1 I2) approximates it up to
+ * the quadrature error, and dM/d(primary position) = -dM). */
+enum { CCOIL_PAIR_M = 1, CCOIL_PAIR_FT = 2, CCOIL_PAIR_DM = 4 };
 
 /* Coils, SoA, host memory, FP64. Lengths are n. Increments are the already-planned quadrature orders
  * (1..100) from PrecisionArguments; blocks apply to pair secondaries only. */
@@ -131,13 +133,13 @@ ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, const double 
 /* Mutual inductance M and force/torque (F, T about the secondary centre) for n_pairs (primary, secondary)
  * pairs, each pair in its own pose. prim_index/sec_index select coils of prim/sec (NULL: identity).
  * ring_offset/half_ring (per pair, NULL: 0) reproduce the coplanar improved-precision rule of
- * Coil::improvedPrecisionCase (SupportFunctions.cxx:124-161). M is 1 per pair, F and T 3 per pair.
+ * Coil::improvedPrecisionCase (SupportFunctions.cxx:124-161). M is 1 per pair, F, T and dM 3 per pair.
  * Replaces MInductance.cu:184-248, Force.cu:206-270 and the *Group.cu launchers. */
 ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coils *sec,
                          const int64_t *prim_index, const int64_t *sec_index, int64_t n_pairs,
                          const double *ring_offset, const int32_t *half_ring,
                          uint32_t what, int32_t precision, uint32_t flags,
-                         double *M, double *F, double *T, ccoil_report *report);
+                         double *M, double *F, double *T, double *dM, ccoil_report *report);
 
 /* Adjoint of the summed B field (no counterpart in the reference's CUDA interface; it backs the gradient of
  * scalar objectives of the field, e.g. a torch.autograd.Function over ccoil_fields). Given dL_dB (3 per point,

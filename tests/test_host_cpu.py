@@ -175,5 +175,6 @@ def test_abi_without_device_reports_no_device():
     assert e.value.status == ccoil_abi.ERR_NO_DEVICE
     L = ccoil_abi.lib()
     assert L.ccoil_fields(None, None, None, 0, 0, 0, 0, None, None, None, None, None) == ccoil_abi.ERR_ARGUMENT
-    assert L.ccoil_pairs(None, None, None, None, None, 0, None, None, 0, 0, 0, None, None, None, None) == \
+    assert L.ccoil_pairs(None, None, None, None, None, 0, None, None, 0, 0, 0, None, None, None, None, None) == \
         ccoil_abi.ERR_ARGUMENT
+    assert L.ccoil_field_gradient(None, None, None, 0, None, 0, 0, None, None) == ccoil_abi.ERR_ARGUMENT
