@@ -1,6 +1,8 @@
 // C ABI implementation (include/ccoil_hip.h): context, device tables, buffers, field entry point.
 #include "context.hpp"
 
+#include <algorithm>
+
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -223,7 +225,7 @@ ccoil_status ccoil_destroy(ccoil_ctx ctx)
         if (p) (void)hipFree(p);
     for (auto p : fp)
         if (p) (void)hipFree(p);
-    DevBuf *bufs[] = {&ctx->coils, &ctx->pts,   &ctx->outA,  &ctx->outB,     &ctx->outE,     &ctx->outG,
+    DevBuf *bufs[] = {&ctx->coils, &ctx->fpartial, &ctx->pts, &ctx->outA, &ctx->outB, &ctx->outE, &ctx->outG,
                       &ctx->queue, &ctx->counter, &ctx->pprim, &ctx->psec, &ctx->ppairs, &ctx->ppartial,
                       &ctx->pcounter, &ctx->pM, &ctx->pF, &ctx->pT, &ctx->pdM, &ctx->acoils, &ctx->apts, &ctx->agB,
                       &ctx->apartial, &ctx->agrad};
@@ -336,8 +338,21 @@ ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, const double 
     }
 
     // Launch shape: split the thickness rows over 4 lanes when there are too few points to give every
-    // SIMD of the 256 CUs several waves.
+    // SIMD of the 256 CUs several waves, and split the coils into chunks (grid y, summed in order by
+    // field_reduce_kernel) while the launch still has fewer than about 8 waves per SIMD.
     const int split = n < (int64_t)256 * 4 * 64 * 4 ? 4 : 1;
+    {
+        const int64_t waves = (n * split + 63) / 64, target = (int64_t)256 * 4 * 8;
+        int64_t chunks = waves >= target ? 1 : std::min<int64_t>((int64_t)hc.size(), (target + waves - 1) / waves);
+        chunks = std::max<int64_t>(1, std::min<int64_t>(chunks, 65535));
+        const int64_t cpc = ((int64_t)hc.size() + chunks - 1) / chunks;
+        p.coils_per_chunk = (int32_t)cpc;
+        p.n_chunks = (int32_t)(((int64_t)hc.size() + cpc - 1) / cpc);
+        if (p.n_chunks > 1) {
+            CC_HIP(ctx, ensure(ctx->fpartial, sizeof(double) * kCartTerms * (size_t)n * (size_t)p.n_chunks));
+            p.partial = (double *)ctx->fpartial.p;
+        }
+    }
     int64_t near = 0;
     const bool near_fp64 = precision == CCOIL_FP32C && !(flags & CCOIL_NO_NEAR_FP64);
     CC_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
@@ -365,6 +380,8 @@ ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, const double 
     launch_field(ctx->stream, p, precision, split);
     CC_HIP(ctx, hipGetLastError());
     CC_HIP(ctx, hipEventRecord(ctx->evm1, ctx->stream));
+    launch_field_reduce(ctx->stream, p);
+    CC_HIP(ctx, hipGetLastError());
     if (near_fp64 && near > 0) {
         launch_near_eval(ctx->stream, p, near);
         CC_HIP(ctx, hipGetLastError());

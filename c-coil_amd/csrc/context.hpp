@@ -35,7 +35,7 @@ struct ccoil_ctx_s {
     double *gl_x = nullptr, *gl_w = nullptr, *gl_cos = nullptr, *gl_omc = nullptr;
     float *gl_xf = nullptr, *gl_wf = nullptr, *gl_cosf = nullptr, *gl_omcf = nullptr;
     double *gl_rcos = nullptr, *gl_rsin = nullptr;
-    ccoil::DevBuf coils, pts, outA, outB, outE, outG, queue, counter;
+    ccoil::DevBuf coils, pts, outA, outB, outE, outG, queue, counter, fpartial;
     ccoil::DevBuf pprim, psec, ppairs, ppartial, pcounter, pM, pF, pT, pdM;
     ccoil::DevBuf acoils, apts, agB, apartial, agrad;   // field adjoint
     hipEvent_t ev0 = nullptr, ev1 = nullptr;   // around all compute kernels of a call

@@ -102,7 +102,8 @@ __global__ __launch_bounds__(kFieldBlock) void field_kernel(FieldParams p)
     for (int k = 0; k < 3; ++k) { acc.A[k] = 0.0; acc.E[k] = 0.0; acc.B[k] = 0.0; }
     for (int k = 0; k < 9; ++k) acc.G[k] = 0.0;
 
-    for (int ci = 0; ci < p.n_coils; ++ci) {
+    const int c0 = blockIdx.y * p.coils_per_chunk, c1 = min(c0 + p.coils_per_chunk, p.n_coils);
+    for (int ci = c0; ci < c1; ++ci) {
         const CoilDev &c = p.coils[ci];
         double z, r, cph, sph;
         to_local(c, px, py, pz, z, r, cph, sph);
@@ -123,10 +124,33 @@ __global__ __launch_bounds__(kFieldBlock) void field_kernel(FieldParams p)
         accumulate_global(c, L, r, cph, sph, p.q, acc);
     }
     if (!active || lane != 0) return;
+    if (p.n_chunks > 1) {
+        double *o = p.partial + ((int64_t)blockIdx.y * p.n_pts + pi) * kCartTerms;
+        for (int k = 0; k < 3; ++k) { o[k] = acc.A[k]; o[3 + k] = acc.E[k]; o[6 + k] = acc.B[k]; }
+        for (int k = 0; k < 9; ++k) o[9 + k] = acc.G[k];
+        return;
+    }
     if (p.A) for (int k = 0; k < 3; ++k) p.A[3 * pi + k] = acc.A[k];
     if (p.E) for (int k = 0; k < 3; ++k) p.E[3 * pi + k] = acc.E[k];
     if (p.B) for (int k = 0; k < 3; ++k) p.B[3 * pi + k] = acc.B[k];
     if (p.G) for (int k = 0; k < 9; ++k) p.G[9 * pi + k] = acc.G[k];
+}
+
+// Sum of the coil chunks of each point, in chunk order (deterministic), into the requested outputs.
+__global__ __launch_bounds__(256) void field_reduce_kernel(FieldParams p)
+{
+    const int64_t pi = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (pi >= p.n_pts) return;
+    double s[kCartTerms];
+    for (int k = 0; k < kCartTerms; ++k) s[k] = 0.0;
+    for (int ch = 0; ch < p.n_chunks; ++ch) {
+        const double *o = p.partial + ((int64_t)ch * p.n_pts + pi) * kCartTerms;
+        for (int k = 0; k < kCartTerms; ++k) s[k] += o[k];
+    }
+    if (p.A) for (int k = 0; k < 3; ++k) p.A[3 * pi + k] = s[k];
+    if (p.E) for (int k = 0; k < 3; ++k) p.E[3 * pi + k] = s[3 + k];
+    if (p.B) for (int k = 0; k < 3; ++k) p.B[3 * pi + k] = s[6 + k];
+    if (p.G) for (int k = 0; k < 9; ++k) p.G[9 * pi + k] = s[9 + k];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -198,9 +222,9 @@ template <int PREC, bool WA, bool WB, bool WG>
 static void launch_field_q(hipStream_t s, const FieldParams &p, int split)
 {
     const int64_t threads = p.n_pts * split;
-    const unsigned blocks = (unsigned)((threads + kFieldBlock - 1) / kFieldBlock);
-    if (split == 1) hipLaunchKernelGGL((field_kernel<PREC, WA, WB, WG, 1>), dim3(blocks), dim3(kFieldBlock), 0, s, p);
-    else hipLaunchKernelGGL((field_kernel<PREC, WA, WB, WG, 4>), dim3(blocks), dim3(kFieldBlock), 0, s, p);
+    const dim3 grid((unsigned)((threads + kFieldBlock - 1) / kFieldBlock), (unsigned)p.n_chunks);
+    if (split == 1) hipLaunchKernelGGL((field_kernel<PREC, WA, WB, WG, 1>), grid, dim3(kFieldBlock), 0, s, p);
+    else hipLaunchKernelGGL((field_kernel<PREC, WA, WB, WG, 4>), grid, dim3(kFieldBlock), 0, s, p);
 }
 
 template <int PREC>
@@ -227,6 +251,12 @@ void launch_field(hipStream_t s, const FieldParams &p, int precision, int split)
     else if (precision == kFP64Reference) launch_field_p<kFP64Reference>(s, p, split);
     else if (precision == kFP32) launch_field_p<kFP32>(s, p, split);
     else launch_field_p<kFP32C>(s, p, split);
+}
+
+void launch_field_reduce(hipStream_t s, const FieldParams &p)
+{
+    if (p.n_pts <= 0 || p.n_chunks <= 1) return;
+    hipLaunchKernelGGL(field_reduce_kernel, dim3((unsigned)((p.n_pts + 255) / 256)), dim3(256), 0, s, p);
 }
 
 void launch_near_classify(hipStream_t s, const FieldParams &p)
