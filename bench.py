@@ -20,8 +20,13 @@ Reported beside it in the same line:
                FP32C error;
   roofline     field_kernel's algorithmic flops per launch (frozen per-node model, roofline_model.py) over its
                hipEvent-timed duration on the library's stream, against the MI355X vector peak of its type;
+  adjoint      the coil-design workload (64 THIN/RECTANGULAR coils near the axis, 4,096 points in a small
+               cylinder): forward B (ccoil_fields) and the adjoint of a field loss (ccoil_field_gradient, all 9
+               parameters of every coil), FP64 and FP32C, and the adjoint / forward time ratio;
   cpu_baseline the FP64 CPU restatement of the reference's CPU_MT field path (-O3 -mavx2 -ffast-math, point chunks on
                a thread pool), timed on a bounded sample of the same workload on this host.
+With --large (not part of the default run): B+gradient of a 1,024-coil group at 2^22 points, and the full
+1,024 x 1,024 mutual-inductance matrix with its position gradient (CCOIL_PAIR_DM) checked against F / (I_i I_j).
 """
 import argparse
 import json
@@ -105,6 +110,10 @@ def main():
     ap.add_argument("--no-side", action="store_true", help="skip the secondary field modes and the accuracy report")
     ap.add_argument("--no-pairs", action="store_true", help="skip the pair leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--large", action="store_true", help="also run the 1,024-coil field and M-matrix legs")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 512 MiB device-to-device copy 3 times (calibrates the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         log("bench.py measures one GPU: the field and pair paths shard by points / pairs with no exchange, and "
@@ -118,6 +127,14 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8, "launches": 3}))
+        return
     params, pts = W.group_workload(args.coils, args.points, seed=0)
     rows = W.group_rows(params)
     cs = abi.CoilSet.from_rows(rows)
@@ -224,6 +241,61 @@ def main():
         pres["workload"] = (f"{len(prow)} random mixed pairs, increments ~ U{{4,...,80}} per active dimension, "
                             f"{pnodes:.3g} node-pairs, M+F+T")
         result["pairs"] = pres
+
+    if not args.no_adjoint:
+        aparams, apts = W.adjoint_workload(64, 4096, seed=0)
+        arows = W.group_rows(aparams)
+        acs = abi.CoilSet.from_rows(arows)
+        ad_pts = torch.from_numpy(apts).to(dev)
+        aB = torch.empty((len(apts), 3), dtype=torch.float64, device=dev)
+        agrad = torch.empty((len(arows), len(abi.GRAD_TERMS)), dtype=torch.float64, device=dev)
+        gB = torch.from_numpy(np.random.default_rng(1).standard_normal(apts.shape)).to(dev)
+        adj = {"workload": "64 THIN/RECTANGULAR coils x 4,096 points, B and the 9-parameter adjoint of a field loss"}
+        for name, prec in (("fp64", abi.FP64), ("fp32c", abi.FP32C)):
+            fw, bw = [], []
+            for it in range(4):
+                _, r1 = ctx.fields(acs, ad_pts, abi.B, prec, abi.DEVICE_POINTERS, out={"B": aB})
+                _, r2 = ctx.field_gradient(acs, ad_pts, gB, prec, abi.DEVICE_POINTERS, out=agrad)
+                if it:
+                    fw.append(r1.kernel_ms)
+                    bw.append(r2.kernel_ms)
+            adj[name] = {"forward_ms": float(np.mean(fw)), "adjoint_ms": float(np.mean(bw)),
+                         "adjoint_over_forward": float(np.mean(bw) / np.mean(fw))}
+        result["adjoint"] = adj
+        log(f"adjoint: {json.dumps(adj)}")
+
+    if args.large:
+        lparams, lpts = W.group_workload(1024, 1 << 22, seed=0)
+        lrows = W.group_rows(lparams)
+        lcs = abi.CoilSet.from_rows(lrows)
+        lp = torch.from_numpy(lpts).to(dev)
+        lout = {"B": torch.empty((len(lpts), 3), dtype=torch.float64, device=dev),
+                "G": torch.empty((len(lpts), 9), dtype=torch.float64, device=dev)}
+        large = {}
+        for name, prec in (("fp64", abi.FP64), ("fp32c", abi.FP32C)):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, lrep = ctx.fields(lcs, lp, q, prec, abi.DEVICE_POINTERS, out=lout)
+            torch.cuda.synchronize()
+            ldt = time.perf_counter() - t0
+            large["fields_" + name] = {"value": len(lrows) * len(lpts) / ldt, "ms": ldt * 1e3,
+                                       "kernel_ms": lrep.main_kernel_ms}
+            log(f"large fields {name}: {ldt * 1e3:.0f} ms")
+        prim = abi.CoilSet.from_rows(lrows)
+        sec = abi.CoilSet.from_rows(W.secondary_rows(lparams))
+        pi, si = W.matrix_pairs(len(lrows))
+        t0 = time.perf_counter()
+        mo, mrep = ctx.pairs(prim, sec, len(pi), abi.PAIR_M | abi.PAIR_FT | abi.PAIR_DM, abi.FP64,
+                             prim_index=pi, sec_index=si)
+        mdt = time.perf_counter() - t0
+        I = np.array([p["I"] for p in lparams])
+        force = mo["F"] / (I[pi] * I[si])[:, None]
+        e = np.linalg.norm(mo["dM"] - force, axis=1) / np.linalg.norm(force, axis=1)
+        large["m_matrix_fp64"] = {"pairs": int(len(pi)), "ms": mdt * 1e3, "kernel_ms": mrep.main_kernel_ms,
+                                  "pairs_per_s": len(pi) / mdt, "node_pairs": int(mrep.node_evaluations),
+                                  "dM_vs_F_over_II": stats(e)}
+        log(f"large M matrix: {mdt * 1e3:.0f} ms, dM vs F/(II) {json.dumps(stats(e))}")
+        result["large"] = large
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(params, pts, rows, min(args.cpu_points, n))

@@ -67,6 +67,7 @@ __device__ inline void eval_plain(const CoilDev &c, const Tables &tab, double zd
             const T gB = r * r, gC = rho * r, gD = A_ + gB, gE = A_ * A_ + gB * gB, gF = gC * gC;
             // A: FieldAGroup.cu (tempConstA/B)
             const T aA = A_ + r * r, aB = (T)2 * rho * r;
+#pragma unroll 4
             for (int i = 0; i < na; ++i) {
                 const T cp = ct[i];
                 const T g = cf * wtj * wa[i];
@@ -107,6 +108,7 @@ __device__ inline void eval_plain(const CoilDev &c, const Tables &tab, double zd
             // G: FieldGGroup.cu:310-319
             const T gB = r * r, gC = z * z, gD = r * rho, gE = rho * z;
             const T gF = (T)2 * A_ + (T)2 * gB - gC, gG = A_ + gB + gC;
+#pragma unroll 4
             for (int i = 0; i < na; ++i) {
                 const T cp = ct[i];
                 const T h = cf * wtj * wa[i];
@@ -152,7 +154,9 @@ __device__ inline void eval_plain(const CoilDev &c, const Tables &tab, double zd
 //                 = |z1| K1 + |z2| K2 (direct)                      otherwise
 //   z1 K1^3 - z2 K2^3 = (z1 K1 - z2 K2) K1^2 - 2 b z z2 K2 (K1 K2)^2  when z1 z2 > 0, direct otherwise
 //   N (gradient RZ term) as a polynomial in s with coefficients in base and q.
-// One reciprocal serves both 1/M and 1/(z1 K1 + z2 K2): Y = M (z1 K1 + z2 K2) (or M), R = 1/Y.
+// One reciprocal serves 1/(K1 + K2), 1/M and 1/(z1 K1 + z2 K2): with Y = M (z1 K1 + z2 K2) (or M) and
+// Rb = 1 / ((K1 + K2) Y), 1/(K1 + K2) = Y Rb and 1/Y = (K1 + K2) Rb. The angular loops are unrolled by 4 so
+// that the scalar table loads of several nodes are in flight together.
 // The factor rho common to every integrand and the thickness weight are applied per row in FP64, and row
 // sums are accumulated in FP64, so T = float computes only the per-node work in FP32 (FP32C).
 // ---------------------------------------------------------------------------------------------
@@ -283,20 +287,22 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             const T n3 = (T)(-2.0 * rho_d * rho_d * rd * rd);
             T rBh = 0, rBz = 0, rRR = 0, rRZ = 0, rZZ = 0, rA = 0;
             T eR = 0, eZ = 0;   // adjoint: end rings, sum w cos (z1 K1^3 + z2 K2^3), sum w (rho - r cos)(K1^3 + K2^3)
+#pragma unroll 4
             for (int i = 0; i < na; ++i) {
                 const T cp = ct[i], s = st[i], w = wa[i];
                 const T M = fma_<T>(q, s, base);
                 if (WB || WG) {
                     const T J1 = M + z1s, J2 = M + z2s;
                     const T K1 = prsq<P>(J1), K2 = prsq<P>(J2);
-                    const T S = prcp<P>(K1 + K2);
                     const T K12 = K1 * K2;
                     const T tP2 = tbz * (K12 * K12);
-                    const T dK = -tP2 * S;                              // K1 - K2
                     const T g1 = z1 * K1, g2 = z2 * K2;
                     const T gs = g1 + g2;
                     const T Y = same ? M * gs : M;
-                    const T Rr = prcp<P>(Y);
+                    const T Ks = K1 + K2;
+                    const T Rb = prcp<P>(Ks * Y);
+                    const T S = Y * Rb, Rr = Ks * Rb;
+                    const T dK = -tP2 * S;                              // K1 - K2
                     const T invM = same ? gs * Rr : Rr;
                     const T dgM = same ? tP2 * (M * Rr) : (g1 - g2) * Rr;   // (z1 K1 - z2 K2) / M
                     const T rmc = fma_<T>(rf, s, dm);                       // rho - r cos
@@ -390,6 +396,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             const T k1 = (T)(2.0 * dd * dd + 2.0 * rho_d * rd - zz);
             const T k2 = (T)(rho_d * rd);
             T rBh = 0, rBz = 0, rRR = 0, rRZ = 0, rZZ = 0, rA = 0;
+#pragma unroll 4
             for (int i = 0; i < na; ++i) {
                 const T cp = ct[i], s = st[i], w = wa[i];
                 const T J = fma_<T>(q, s, baseJ);
