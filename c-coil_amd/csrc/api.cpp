@@ -339,17 +339,34 @@ ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, const double 
 
     // Launch shape: split the thickness rows over 4 lanes when there are too few points to give every
     // SIMD of the 256 CUs several waves, and split the coils into chunks (grid y, summed in order by
-    // field_reduce_kernel) while the launch still has fewer than about 8 waves per SIMD.
+    // field_reduce_kernel) while the launch still has fewer than about 8 waves per SIMD. Large launches take
+    // 1-4 chunks, whichever makes the last round of resident waves fullest: every wave does the same work, so
+    // with R rounds of resident waves the launch runs ceil(R) rounds.
     const int split = n < (int64_t)256 * 4 * 64 * 4 ? 4 : 1;
     {
         const int64_t waves = (n * split + 63) / 64, target = (int64_t)256 * 4 * 8;
         int64_t chunks = waves >= target ? 1 : std::min<int64_t>((int64_t)hc.size(), (target + waves - 1) / waves);
+        if (waves >= target) {
+            const int bpc = field_blocks_per_cu(quantities, precision, split);
+            const double slots = (double)ctx->cu_count * bpc * (kFieldBlock / 64);
+            if (slots > 0) {
+                double best = 0.0;
+                for (int64_t c = 1; c <= std::min<int64_t>(4, (int64_t)hc.size()); ++c) {
+                    const double R = (double)waves * c / slots, eff = R / std::ceil(R);
+                    if (eff > best + 0.02) {
+                        best = eff;
+                        chunks = c;
+                    }
+                }
+            }
+        }
         chunks = std::max<int64_t>(1, std::min<int64_t>(chunks, 65535));
         const int64_t cpc = ((int64_t)hc.size() + chunks - 1) / chunks;
         p.coils_per_chunk = (int32_t)cpc;
         p.n_chunks = (int32_t)(((int64_t)hc.size() + cpc - 1) / cpc);
         if (p.n_chunks > 1) {
-            CC_HIP(ctx, ensure(ctx->fpartial, sizeof(double) * kCartTerms * (size_t)n * (size_t)p.n_chunks));
+            p.partial_width = (p.A ? 3 : 0) + (p.E ? 3 : 0) + (p.B ? 3 : 0) + (p.G ? 9 : 0);
+            CC_HIP(ctx, ensure(ctx->fpartial, sizeof(double) * p.partial_width * (size_t)n * (size_t)p.n_chunks));
             p.partial = (double *)ctx->fpartial.p;
         }
     }

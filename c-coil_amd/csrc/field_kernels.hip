@@ -125,9 +125,11 @@ __global__ __launch_bounds__(kFieldBlock) void field_kernel(FieldParams p)
     }
     if (!active || lane != 0) return;
     if (p.n_chunks > 1) {
-        double *o = p.partial + ((int64_t)blockIdx.y * p.n_pts + pi) * kCartTerms;
-        for (int k = 0; k < 3; ++k) { o[k] = acc.A[k]; o[3 + k] = acc.E[k]; o[6 + k] = acc.B[k]; }
-        for (int k = 0; k < 9; ++k) o[9 + k] = acc.G[k];
+        double *o = p.partial + ((int64_t)blockIdx.y * p.n_pts + pi) * p.partial_width;
+        if (p.A) { for (int k = 0; k < 3; ++k) o[k] = acc.A[k]; o += 3; }
+        if (p.E) { for (int k = 0; k < 3; ++k) o[k] = acc.E[k]; o += 3; }
+        if (p.B) { for (int k = 0; k < 3; ++k) o[k] = acc.B[k]; o += 3; }
+        if (p.G) for (int k = 0; k < 9; ++k) o[k] = acc.G[k];
         return;
     }
     if (p.A) for (int k = 0; k < 3; ++k) p.A[3 * pi + k] = acc.A[k];
@@ -141,16 +143,19 @@ __global__ __launch_bounds__(256) void field_reduce_kernel(FieldParams p)
 {
     const int64_t pi = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (pi >= p.n_pts) return;
-    double s[kCartTerms];
-    for (int k = 0; k < kCartTerms; ++k) s[k] = 0.0;
+    constexpr int kMaxWidth = 18;
+    double s[kMaxWidth];
+    for (int k = 0; k < kMaxWidth; ++k) s[k] = 0.0;
+    const int w = p.partial_width;
     for (int ch = 0; ch < p.n_chunks; ++ch) {
-        const double *o = p.partial + ((int64_t)ch * p.n_pts + pi) * kCartTerms;
-        for (int k = 0; k < kCartTerms; ++k) s[k] += o[k];
+        const double *o = p.partial + ((int64_t)ch * p.n_pts + pi) * w;
+        for (int k = 0; k < w; ++k) s[k] += o[k];
     }
-    if (p.A) for (int k = 0; k < 3; ++k) p.A[3 * pi + k] = s[k];
-    if (p.E) for (int k = 0; k < 3; ++k) p.E[3 * pi + k] = s[3 + k];
-    if (p.B) for (int k = 0; k < 3; ++k) p.B[3 * pi + k] = s[6 + k];
-    if (p.G) for (int k = 0; k < 9; ++k) p.G[9 * pi + k] = s[9 + k];
+    int off = 0;
+    if (p.A) { for (int k = 0; k < 3; ++k) p.A[3 * pi + k] = s[off + k]; off += 3; }
+    if (p.E) { for (int k = 0; k < 3; ++k) p.E[3 * pi + k] = s[off + k]; off += 3; }
+    if (p.B) { for (int k = 0; k < 3; ++k) p.B[3 * pi + k] = s[off + k]; off += 3; }
+    if (p.G) for (int k = 0; k < 9; ++k) p.G[9 * pi + k] = s[off + k];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -241,6 +246,40 @@ static void launch_field_p(hipStream_t s, const FieldParams &p, int split)
         case 6: launch_field_q<PREC, true, true, false>(s, p, split); break;
         default: launch_field_q<PREC, true, true, true>(s, p, split); break;
     }
+}
+
+template <int PREC, bool WA, bool WB, bool WG>
+static int blocks_per_cu_q(int split)
+{
+    int b = 0;
+    hipError_t e = split == 1
+                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, field_kernel<PREC, WA, WB, WG, 1>, kFieldBlock, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, field_kernel<PREC, WA, WB, WG, 4>, kFieldBlock, 0);
+    return e == hipSuccess ? b : 0;
+}
+
+template <int PREC>
+static int blocks_per_cu_p(uint32_t q, int split)
+{
+    const bool wa = q & (Q_A | Q_E), wb = q & Q_B, wg = q & (Q_G | Q_GJ);
+    switch ((wa ? 4 : 0) | (wb ? 2 : 0) | (wg ? 1 : 0)) {
+        case 1: return blocks_per_cu_q<PREC, false, false, true>(split);
+        case 2: return blocks_per_cu_q<PREC, false, true, false>(split);
+        case 3: return blocks_per_cu_q<PREC, false, true, true>(split);
+        case 4: return blocks_per_cu_q<PREC, true, false, false>(split);
+        case 5: return blocks_per_cu_q<PREC, true, false, true>(split);
+        case 6: return blocks_per_cu_q<PREC, true, true, false>(split);
+        default: return blocks_per_cu_q<PREC, true, true, true>(split);
+    }
+}
+
+int field_blocks_per_cu(uint32_t q, int precision, int split)
+{
+    if (precision == kFP64) return blocks_per_cu_p<kFP64>(q, split);
+    if (precision == kFP64IEEE) return blocks_per_cu_p<kFP64IEEE>(q, split);
+    if (precision == kFP64Reference) return blocks_per_cu_p<kFP64Reference>(q, split);
+    if (precision == kFP32) return blocks_per_cu_p<kFP32>(q, split);
+    return blocks_per_cu_p<kFP32C>(q, split);
 }
 
 void launch_field(hipStream_t s, const FieldParams &p, int precision, int split)

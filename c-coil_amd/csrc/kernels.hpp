@@ -21,13 +21,14 @@ struct FieldParams {
     int2 *near_queue;      // (point, coil)
     unsigned long long *near_count;
     int64_t near_capacity;
-    // Coil chunks (grid y): with few points, each block row sums its chunk of coils into partial
-    // ([chunk][point][kCartTerms]) and field_reduce_kernel adds the chunks in order.
+    // Coil chunks (grid y): each block row sums its chunk of coils into partial ([chunk][point][width]: the
+    // requested outputs A, E, B, G in that order, 3 or 9 doubles each) and field_reduce_kernel adds the chunks in
+    // order.
     int32_t coils_per_chunk;
     int32_t n_chunks;
+    int32_t partial_width;
     double *partial;
 };
-constexpr int kCartTerms = 18;   // A[3], E[3], B[3], G[9]
 
 // Adjoint of the summed B field: per-coil gradients of sum_x dL/dB(x) . B(x), 9 per coil in the order
 // current, R, a, b, px, py, pz, alpha, beta. Grid: (point blocks, coils); partial sums per (coil, block).
@@ -52,6 +53,8 @@ void launch_gl_tables(hipStream_t s, double *x, double *w, double *cosv, double 
 void launch_fill_zero(hipStream_t s, void *p, int64_t n_words);
 void launch_field(hipStream_t s, const FieldParams &p, int precision, int split);
 void launch_field_reduce(hipStream_t s, const FieldParams &p);   // after launch_field when n_chunks > 1
+// Resident 256-thread blocks per CU of the field kernel instance (0 if the runtime cannot tell).
+int field_blocks_per_cu(uint32_t q, int precision, int split);
 void launch_near_classify(hipStream_t s, const FieldParams &p);
 void launch_near_eval(hipStream_t s, const FieldParams &p, int64_t count);
 
