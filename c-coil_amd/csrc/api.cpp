@@ -346,7 +346,7 @@ ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, const double 
     {
         const int64_t waves = (n * split + 63) / 64, target = (int64_t)256 * 4 * 8;
         int64_t chunks = waves >= target ? 1 : std::min<int64_t>((int64_t)hc.size(), (target + waves - 1) / waves);
-        if (waves >= target) {
+        if (waves >= target && n * 18 < ((int64_t)1 << 31)) {   // the reduction grid stays below 2^31 threads
             const int bpc = field_blocks_per_cu(quantities, precision, split);
             const double slots = (double)ctx->cu_count * bpc * (kFieldBlock / 64);
             if (slots > 0) {

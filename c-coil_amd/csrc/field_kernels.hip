@@ -138,24 +138,22 @@ __global__ __launch_bounds__(kFieldBlock) void field_kernel(FieldParams p)
     if (p.G) for (int k = 0; k < 9; ++k) p.G[9 * pi + k] = acc.G[k];
 }
 
-// Sum of the coil chunks of each point, in chunk order (deterministic), into the requested outputs.
+// Sum of the coil chunks, in chunk order (deterministic), into the requested outputs. One thread per
+// (point, partial component), so consecutive lanes read consecutive partial words and write consecutive output
+// words.
 __global__ __launch_bounds__(256) void field_reduce_kernel(FieldParams p)
 {
-    const int64_t pi = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (pi >= p.n_pts) return;
-    constexpr int kMaxWidth = 18;
-    double s[kMaxWidth];
-    for (int k = 0; k < kMaxWidth; ++k) s[k] = 0.0;
     const int w = p.partial_width;
-    for (int ch = 0; ch < p.n_chunks; ++ch) {
-        const double *o = p.partial + ((int64_t)ch * p.n_pts + pi) * w;
-        for (int k = 0; k < w; ++k) s[k] += o[k];
-    }
-    int off = 0;
-    if (p.A) { for (int k = 0; k < 3; ++k) p.A[3 * pi + k] = s[off + k]; off += 3; }
-    if (p.E) { for (int k = 0; k < 3; ++k) p.E[3 * pi + k] = s[off + k]; off += 3; }
-    if (p.B) { for (int k = 0; k < 3; ++k) p.B[3 * pi + k] = s[off + k]; off += 3; }
-    if (p.G) for (int k = 0; k < 9; ++k) p.G[9 * pi + k] = s[off + k];
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= p.n_pts * w) return;
+    double s = 0.0;
+    for (int ch = 0; ch < p.n_chunks; ++ch) s += p.partial[(int64_t)ch * p.n_pts * w + t];
+    const int64_t pi = t / w;
+    int k = (int)(t - pi * w);
+    if (p.A) { if (k < 3) { p.A[3 * pi + k] = s; return; } k -= 3; }
+    if (p.E) { if (k < 3) { p.E[3 * pi + k] = s; return; } k -= 3; }
+    if (p.B) { if (k < 3) { p.B[3 * pi + k] = s; return; } k -= 3; }
+    if (p.G) p.G[9 * pi + k] = s;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -295,7 +293,8 @@ void launch_field(hipStream_t s, const FieldParams &p, int precision, int split)
 void launch_field_reduce(hipStream_t s, const FieldParams &p)
 {
     if (p.n_pts <= 0 || p.n_chunks <= 1) return;
-    hipLaunchKernelGGL(field_reduce_kernel, dim3((unsigned)((p.n_pts + 255) / 256)), dim3(256), 0, s, p);
+    const int64_t n = p.n_pts * p.partial_width;
+    hipLaunchKernelGGL(field_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p);
 }
 
 void launch_near_classify(hipStream_t s, const FieldParams &p)
