@@ -161,6 +161,8 @@ Tables ctx_tables(ccoil_ctx ctx)
     t.cosf = ctx->gl_cosf;
     t.omcf = ctx->gl_omcf;
     t.rcos = ctx->gl_rcos;
+    t.wcos = ctx->gl_wcos;
+    t.wcosf = ctx->gl_wcosf;
     t.rsin = ctx->gl_rsin;
     return t;
 }
@@ -200,8 +202,8 @@ ccoil_status ccoil_create(int device, ccoil_ctx *out)
     if ((e = hipEventCreate(&ctx->evm0)) != hipSuccess) return bail(e, "hipEventCreate");
     if ((e = hipEventCreate(&ctx->evm1)) != hipSuccess) return bail(e, "hipEventCreate");
     const size_t nd = sizeof(double) * kTableSize, nf = sizeof(float) * kTableSize;
-    double **dp[6] = {&ctx->gl_x, &ctx->gl_w, &ctx->gl_cos, &ctx->gl_omc, &ctx->gl_rcos, &ctx->gl_rsin};
-    float **fp[4] = {&ctx->gl_xf, &ctx->gl_wf, &ctx->gl_cosf, &ctx->gl_omcf};
+    double **dp[7] = {&ctx->gl_x, &ctx->gl_w, &ctx->gl_cos, &ctx->gl_omc, &ctx->gl_rcos, &ctx->gl_rsin, &ctx->gl_wcos};
+    float **fp[5] = {&ctx->gl_xf, &ctx->gl_wf, &ctx->gl_cosf, &ctx->gl_omcf, &ctx->gl_wcosf};
     for (auto p : dp)
         if ((e = hipMalloc((void **)p, nd)) != hipSuccess) return bail(e, "hipMalloc(tables)");
     for (auto p : fp)
@@ -209,6 +211,8 @@ ccoil_status ccoil_create(int device, ccoil_ctx *out)
     launch_gl_tables(ctx->stream, ctx->gl_x, ctx->gl_w, ctx->gl_cos, ctx->gl_omc, ctx->gl_xf, ctx->gl_wf,
                      ctx->gl_cosf, ctx->gl_omcf, ctx->gl_rcos, ctx->gl_rsin);
     if ((e = hipGetLastError()) != hipSuccess) return bail(e, "gl_table_kernel launch");
+    launch_wcos_table(ctx->stream, ctx->gl_w, ctx->gl_cos, ctx->gl_wcos, ctx->gl_wcosf);
+    if ((e = hipGetLastError()) != hipSuccess) return bail(e, "wcos_table_kernel launch");
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return bail(e, "gl_table_kernel");
     *out = ctx;
     return CCOIL_OK;
@@ -219,8 +223,8 @@ ccoil_status ccoil_destroy(ccoil_ctx ctx)
     if (!ctx) return CCOIL_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->own_stream) (void)hipStreamSynchronize(ctx->own_stream);
-    double *dp[6] = {ctx->gl_x, ctx->gl_w, ctx->gl_cos, ctx->gl_omc, ctx->gl_rcos, ctx->gl_rsin};
-    float *fp[4] = {ctx->gl_xf, ctx->gl_wf, ctx->gl_cosf, ctx->gl_omcf};
+    double *dp[7] = {ctx->gl_x, ctx->gl_w, ctx->gl_cos, ctx->gl_omc, ctx->gl_rcos, ctx->gl_rsin, ctx->gl_wcos};
+    float *fp[5] = {ctx->gl_xf, ctx->gl_wf, ctx->gl_cosf, ctx->gl_omcf, ctx->gl_wcosf};
     for (auto p : dp)
         if (p) (void)hipFree(p);
     for (auto p : fp)

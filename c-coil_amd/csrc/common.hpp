@@ -48,6 +48,8 @@ struct Tables {
     const float *omcf;
     const double *rcos;   // cos(pi (1 + x)): secondary ring nodes of the GPU pair rule (MInductance.cu:34)
     const double *rsin;   // sin(pi (1 + x))
+    const double *wcos;   // w cos(pi/2 (1 + x)): the angular weight times the cosine
+    const float *wcosf;
 };
 
 // Quantity bits (mirror of the C ABI enum).

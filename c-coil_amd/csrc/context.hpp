@@ -34,7 +34,8 @@ struct ccoil_ctx_s {
     // Gauss-Legendre tables (FP64 generated on device, FP32 copies, ring tables)
     double *gl_x = nullptr, *gl_w = nullptr, *gl_cos = nullptr, *gl_omc = nullptr;
     float *gl_xf = nullptr, *gl_wf = nullptr, *gl_cosf = nullptr, *gl_omcf = nullptr;
-    double *gl_rcos = nullptr, *gl_rsin = nullptr;
+    double *gl_rcos = nullptr, *gl_rsin = nullptr, *gl_wcos = nullptr;
+    float *gl_wcosf = nullptr;
     ccoil::DevBuf coils, pts, outA, outB, outE, outG, queue, counter, fpartial;
     ccoil::DevBuf pprim, psec, ppairs, ppartial, pcounter, pM, pF, pT, pdM;
     ccoil::DevBuf acoils, apts, agB, apartial, agrad;   // field adjoint

@@ -208,6 +208,19 @@ void launch_gl_tables(hipStream_t s, double *x, double *w, double *cosv, double 
                        rcos, rsin);
 }
 
+__global__ void wcos_table_kernel(const double *w, const double *cosv, double *wc, float *wcf)
+{
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= kTableSize) return;
+    wc[e] = w[e] * cosv[e];
+    wcf[e] = (float)wc[e];
+}
+
+void launch_wcos_table(hipStream_t s, const double *w, const double *cosv, double *wc, float *wcf)
+{
+    hipLaunchKernelGGL(wcos_table_kernel, dim3((kTableSize + 255) / 256), dim3(256), 0, s, w, cosv, wc, wcf);
+}
+
 __global__ void fill_zero_kernel(unsigned long long *p, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;

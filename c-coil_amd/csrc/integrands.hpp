@@ -230,6 +230,10 @@ template <typename P> __device__ inline real_t<P> log1p_pos(real_t<P> x)
     return (T)2 * u * s;
 }
 
+template <typename T> __device__ inline const T *tab_wcos(const Tables &t);
+template <> __device__ inline const double *tab_wcos<double>(const Tables &t) { return t.wcos; }
+template <> __device__ inline const float *tab_wcos<float>(const Tables &t) { return t.wcosf; }
+
 template <typename T> __device__ inline const T *tab_omc(const Tables &t);
 template <> __device__ inline const double *tab_omc<double>(const Tables &t) { return t.omc; }
 template <> __device__ inline const float *tab_omc<float>(const Tables &t) { return t.omcf; }
@@ -256,6 +260,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
     const T *ct = tab_cos<T>(tab) + table_offset(c.n_ang);
     const T *st = tab_omc<T>(tab) + table_offset(c.n_ang);
     const T *wa = tab_w<T>(tab) + table_offset(c.n_ang);
+    const T *wct = tab_wcos<T>(tab) + table_offset(c.n_ang);   // w cos
     const int na = c.n_ang, nt = c.n_thk;
     double aA = 0, aBh = 0, aBz = 0, aRR = 0, aRZ = 0, aZZ = 0;
     double hR0 = 0, hR1 = 0, ha0 = 0, ha1 = 0, hb0 = 0, hb1 = 0;   // adjoint row sums
@@ -289,8 +294,9 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             T eR = 0, eZ = 0;   // adjoint: end rings, sum w cos (z1 K1^3 + z2 K2^3), sum w (rho - r cos)(K1^3 + K2^3)
 #pragma unroll 4
             for (int i = 0; i < na; ++i) {
-                const T cp = ct[i], s = st[i], w = wa[i];
-                const T M = fma_<T>(q, s, base);
+                const T cp = ct[i], s = st[i], w = wa[i], wc = wct[i];
+                const T qs = q * s;
+                const T M = base + qs;
                 if (WB || WG) {
                     const T J1 = M + z1s, J2 = M + z2s;
                     const T K1 = prsq<P>(J1), K2 = prsq<P>(J2);
@@ -306,23 +312,23 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                     const T invM = same ? gs * Rr : Rr;
                     const T dgM = same ? tP2 * (M * Rr) : (g1 - g2) * Rr;   // (z1 K1 - z2 K2) / M
                     const T rmc = fma_<T>(rf, s, dm);                       // rho - r cos
-                    rBh = fma_<T>(w, -cp * dK, rBh);   // B_rho, also the gradient's rho-phi term
+                    rBh = fma_<T>(wc, -dK, rBh);       // B_rho, also the gradient's rho-phi term
                     if (WB) rBz = fma_<T>(w, rmc * dgM, rBz);
                     if (WG) {
                         const T K1s = K1 * K1, K2s = K2 * K2;
                         const T dL = dK * (K1s + K12 + K2s);                 // K1^3 - K2^3
                         const T rcm = fma_<T>(rho, s, -dm);                 // r - rho cos
-                        rRR = fma_<T>(w, rcm * cp * dL, rRR);
+                        rRR = fma_<T>(wc, rcm * dL, rRR);
                         rZZ = fma_<T>(w, rmc * dL, rZZ);
                         const T dg = dgM * M;                               // z1 K1 - z2 K2
                         const T dh = same ? fma_<T>(dg, K1s, -tP2 * g2) : (g1 * K1s - g2 * K2s);
-                        const T O = fma_<T>(cp, base, -q * s);              // cos (rho^2 + r^2) - 2 rho r
+                        const T O = fma_<T>(cp, base, -qs);                 // cos (rho^2 + r^2) - 2 rho r
                         const T N = fma_<T>(s, fma_<T>(s, fma_<T>(s, n3, n2), n1), base2);
                         const T t = fma_<T>(O, dg, N * dh);
                         rRZ = fma_<T>(w, t * (invM * invM), rRZ);
                         if constexpr (AD) {
                             const T K13 = K1s * K1, K23 = K2s * K2;
-                            eR = fma_<T>(w, cp * fma_<T>(z1, K13, z2 * K23), eR);
+                            eR = fma_<T>(wc, fma_<T>(z1, K13, z2 * K23), eR);
                             eZ = fma_<T>(w, rmc * (K13 + K23), eZ);
                         }
                     }
@@ -345,7 +351,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                     const T sum = sJa1 + sJa2;
                     const T numr = fma_<T>(bf, sum, tbza);
                     const T x = samea ? numr * prcp<P>(sum * (za2 + sJa2)) : numr * (sJa2 - za2) * prcp<P>(sum * M);
-                    rA = fma_<T>(w, cp * log1p_pos<P>(x), rA);
+                    rA = fma_<T>(wc, log1p_pos<P>(x), rA);
                 }
             }
             const double W = c.cf * wt[j] * rho_d;
@@ -398,25 +404,25 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             T rBh = 0, rBz = 0, rRR = 0, rRZ = 0, rZZ = 0, rA = 0;
 #pragma unroll 4
             for (int i = 0; i < na; ++i) {
-                const T cp = ct[i], s = st[i], w = wa[i];
+                const T s = st[i], w = wa[i], wc = wct[i];
                 const T J = fma_<T>(q, s, baseJ);
                 const T K = prsq<P>(J);
                 if (WB || WG) {
                     const T K2 = K * K;
                     const T K3 = K2 * K;
                     const T rmc = fma_<T>(rf, s, dm);     // rho - r cos
-                    rBh = fma_<T>(w, cp * K3, rBh);    // B_rho, also the gradient's rho-phi term
+                    rBh = fma_<T>(wc, K3, rBh);        // B_rho, also the gradient's rho-phi term
                     if (WB) rBz = fma_<T>(w, rmc * K3, rBz);
                     if (WG) {
                         const T K5 = K3 * K2;
                         const T rcm = fma_<T>(rho, s, -dm); // r - rho cos
-                        rRR = fma_<T>(w, rcm * cp * K5, rRR);
+                        rRR = fma_<T>(wc, rcm * K5, rRR);
                         rZZ = fma_<T>(w, rmc * K5, rZZ);
                         const T poly = k0 - s * fma_<T>(k2, s, k1);
                         rRZ = fma_<T>(w, poly * K5, rRZ);
                     }
                 }
-                if (WA) rA = fma_<T>(w, cp * K, rA);
+                if (WA) rA = fma_<T>(wc, K, rA);
             }
             const double W = c.cf * wt[j] * rho_d;
             aBh += W * (double)rBh;
