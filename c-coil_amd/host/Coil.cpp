@@ -335,6 +335,16 @@ vec3::Matrix3Array Coil::computeAllBGradientMatrices(const vec3::Vector3Array &p
     return computeAllBGradientMatrices(pts, m == GPU ? defaultPrecisionGPU : defaultPrecisionCPU, m);
 }
 
+std::vector<std::pair<vec3::Vector3, vec3::Vector3>>
+Coil::computeAllForceOnDipoleMoments(const vec3::Vector3Array &points, const vec3::Vector3Array &moments,
+                                     ComputeMethod m) const
+{
+    requireGpu(m, "Coil::computeAllForceOnDipoleMoments with CPU_ST/CPU_MT");
+    CoilSoA s;
+    s.add(*this, defaultPrecisionGPU);
+    return ccoil::bridge::dipoleForceTorque(s, points, moments);
+}
+
 // --------------------------------------------------------------------------------------------
 // Pairs.
 // --------------------------------------------------------------------------------------------

@@ -219,6 +219,12 @@ class Coil {
     computeForceOnDipoleMoment(vec3::Vector3 pointVector, vec3::Vector3 dipoleMoment,
                                const PrecisionArguments &usedPrecision) const;
 
+    // Force and torque on point dipoles (moments[k] at points[k]) in this coil's field, GPU only: F = J^T m with
+    // the true Jacobian of B (the reference's single-point form multiplies its T*G gradient, ComputeForceMethods.cxx:38-49).
+    [[nodiscard]] std::vector<std::pair<vec3::Vector3, vec3::Vector3>>
+    computeAllForceOnDipoleMoments(const vec3::Vector3Array &points, const vec3::Vector3Array &moments,
+                                   ComputeMethod computeMethod = GPU) const;
+
     static std::vector<double> computeAllMutualInductanceArrangements(
         const Coil &primary, const Coil &secondary, const vec3::Vector3Array &primaryPositions,
         const vec3::Vector3Array &secondaryPositions, const std::vector<double> &primaryYAngles,

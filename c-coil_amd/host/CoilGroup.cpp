@@ -92,6 +92,16 @@ void groupFields(const std::vector<std::shared_ptr<Coil>> &coils, PrecisionFacto
 
 }  // namespace
 
+std::vector<std::pair<vec3::Vector3, vec3::Vector3>>
+CoilGroup::computeAllForceOnDipoleMoments(const vec3::Vector3Array &points, const vec3::Vector3Array &moments,
+                                          ComputeMethod m) const
+{
+    if (m != GPU) cpuOnly("CoilGroup::computeAllForceOnDipoleMoments with CPU_ST/CPU_MT");
+    CoilSoA s;
+    for (auto &c : memberCoils) s.add(*c, PrecisionArguments::getCoilPrecisionArgumentsGPU(*c, defaultPrecisionFactor));
+    return ccoil::bridge::dipoleForceTorque(s, points, moments);
+}
+
 vec3::Vector3Array CoilGroup::computeAllAPotentialVectors(const vec3::Vector3Array &pts, ComputeMethod m) const
 {
     if (m != GPU) cpuOnly("CoilGroup::computeAllAPotentialVectors with CPU_ST/CPU_MT");

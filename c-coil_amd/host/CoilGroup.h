@@ -52,6 +52,10 @@ class CoilGroup {
                        ComputeMethod computeMethod = CPU_ST) const;
     [[nodiscard]] std::pair<vec3::Vector3, vec3::Vector3> computeForceOnDipoleMoment(vec3::Vector3 pointVector,
                                                                                      vec3::Vector3 dipoleMoment) const;
+    // Force and torque on point dipoles in the group's field (GPU, true Jacobian; see Coil).
+    [[nodiscard]] std::vector<std::pair<vec3::Vector3, vec3::Vector3>>
+    computeAllForceOnDipoleMoments(const vec3::Vector3Array &points, const vec3::Vector3Array &moments,
+                                   ComputeMethod computeMethod = GPU) const;
 
     [[nodiscard]] std::vector<double>
     computeAllMutualInductanceArrangements(const Coil &secondary, const vec3::Vector3Array &secondaryPositions,

@@ -112,5 +112,29 @@ ccoil_coils CoilSoA::view() const
     return v;
 }
 
+std::vector<std::pair<vec3::Vector3, vec3::Vector3>> dipoleForceTorque(const CoilSoA &coils,
+                                                                       const vec3::Vector3Array &points,
+                                                                       const vec3::Vector3Array &moments)
+{
+    if (points.size() != moments.size()) throw std::logic_error("Array sizes do not match!");
+    const size_t n = points.size();
+    std::vector<double> B(3 * n), J(9 * n);
+    const ccoil_coils v = coils.view();
+    ccoil_ctx ctx = context();
+    check(ccoil_fields(ctx, &v, points.data(), (int64_t)n, CCOIL_B | CCOIL_G_JACOBIAN, precision(), 0, nullptr,
+                       B.data(), nullptr, J.data(), nullptr),
+          ctx, "ccoil_fields");
+    std::vector<std::pair<vec3::Vector3, vec3::Vector3>> out(n);
+    for (size_t k = 0; k < n; ++k) {
+        const vec3::Vector3 m = moments[k];
+        const double *j = &J[9 * k], *b = &B[3 * k];
+        const vec3::Vector3 F(j[0] * m.x + j[3] * m.y + j[6] * m.z, j[1] * m.x + j[4] * m.y + j[7] * m.z,
+                              j[2] * m.x + j[5] * m.y + j[8] * m.z);
+        const vec3::Vector3 T(m.y * b[2] - m.z * b[1], m.z * b[0] - m.x * b[2], m.x * b[1] - m.y * b[0]);
+        out[k] = {F, T};
+    }
+    return out;
+}
+
 }  // namespace bridge
 }  // namespace ccoil

@@ -32,5 +32,11 @@ struct CoilSoA {
     [[nodiscard]] ccoil_coils view() const;
 };
 
+// Force and torque on point magnetic dipoles in the field of the coils: F = J^T m with J the true Jacobian
+// dB_i/dx_j (CCOIL_G_JACOBIAN), T = m x B. One ccoil_fields call for all points.
+std::vector<std::pair<vec3::Vector3, vec3::Vector3>> dipoleForceTorque(const CoilSoA &coils,
+                                                                       const vec3::Vector3Array &points,
+                                                                       const vec3::Vector3Array &moments);
+
 }  // namespace bridge
 }  // namespace ccoil

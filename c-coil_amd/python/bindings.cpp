@@ -336,6 +336,8 @@ static void init_coil(py::module_ &m)
         .def("compute_force_on_dipole_moment",
              py::overload_cast<V, V, const PrecisionArguments &>(&Coil::computeForceOnDipoleMoment, py::const_),
              py::arg("point_vector"), py::arg("dipole_moment"), py::arg("used_precision"))
+        .def("compute_all_force_on_dipole_moments", &Coil::computeAllForceOnDipoleMoments, py::arg("point_vectors"),
+             py::arg("dipole_moments"), py::arg("compute_method") = GPU)
         .def_static("compute_all_mutual_inductance_arrangements", &Coil::computeAllMutualInductanceArrangements,
                     py::arg("primary"), py::arg("secondary"), py::arg("primary_positions"),
                     py::arg("secondary_positions"), py::arg("primary_Y_angles"), py::arg("primary_Z_angles"),
@@ -404,6 +406,8 @@ static void init_coil_group(py::module_ &m)
              py::arg("precision_factor") = PrecisionFactor(), py::arg("compute_method") = CPU_ST)
         .def("compute_force_on_dipole_moment", &CoilGroup::computeForceOnDipoleMoment, py::arg("point_vector"),
              py::arg("dipole_moment"))
+        .def("compute_all_force_on_dipole_moments", &CoilGroup::computeAllForceOnDipoleMoments,
+             py::arg("point_vectors"), py::arg("dipole_moments"), py::arg("compute_method") = GPU)
         .def("compute_all_mutual_inductance_arrangements", &CoilGroup::computeAllMutualInductanceArrangements,
              py::arg("secondary"), py::arg("secondary_positions"), py::arg("secondary_y_angles"),
              py::arg("secondary_z_angles"), py::arg("precision_factor") = PrecisionFactor(),

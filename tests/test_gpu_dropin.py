@@ -149,3 +149,44 @@ def test_group_arrangements_precision_remap(cc):
             pa = O.coil_precision_args(oc, gpf, True)
             ref += O.mutual_inductance_gpu_rule(oc, os_, pa[3:6], sec_args[3:6])
         assert abs(M[k] - ref) <= 1e-11 * abs(ref), (k, M[k], ref)
+
+
+def test_dipole_force_and_torque(cc):
+    """CoilGroup.compute_all_force_on_dipole_moments: F = grad(m . B) (true Jacobian), T = m x B, against the oracle's
+    long double field and central differences of m . B."""
+    import workloads as W
+    C, V = cc.coil, cc.tensor.Vector3
+    params, _ = W.group_workload(5, 8, seed=46)
+    group = cc.coil_group.CoilGroup()
+    for p in params:
+        group.add_coil(p["R"], p["a"], p["b"], int(p["N"]), p["I"], C.PrecisionFactor(5.0), 8, V(*p["pos"]),
+                       p["alpha"], p["beta"])
+    rng = np.random.default_rng(47)
+    pts = rng.uniform(1.6, 2.2, size=(16, 3)) * rng.choice([-1.0, 1.0], size=(16, 3))
+    mom = rng.standard_normal((16, 3))
+    out = group.compute_all_force_on_dipole_moments(_va(cc, pts), _va(cc, mom), C.GPU)
+    coils = [H.oracle_coil(p) for p in params]
+    args = [_args6(C.PrecisionArguments.get_coil_precision_arguments_GPU(m, C.PrecisionFactor(5.0)))
+            for m in group.get_member_coils()]
+    J = np.zeros((len(pts), 3, 3))
+    for c, a in zip(coils, args):
+        _, _, g = O.coil_fields(c, a, pts, False, False, True, axis_ratio=O.AXIS_RATIO_GPU, threads=4, extended=True)
+        J += g @ np.array(c.T).reshape(3, 3).T
+    _, b, _ = O.group_fields(coils, args, pts, False, True, False, axis_ratio=O.AXIS_RATIO_GPU, threads=4,
+                             extended=True)
+    h = 1e-6
+    for k, (f, t) in enumerate(out):
+        F = np.array([f.x, f.y, f.z])
+        Tq = np.array([t.x, t.y, t.z])
+        assert np.linalg.norm(F - J[k].T @ mom[k]) <= 1e-11 * np.linalg.norm(F)
+        assert np.linalg.norm(Tq - np.cross(mom[k], b[k])) <= 1e-11 * np.linalg.norm(Tq)
+        fd = np.zeros(3)
+        for ax in range(3):
+            e = np.zeros(3)
+            e[ax] = h
+            _, bp, _ = O.group_fields(coils, args, (pts[k] + e)[None], False, True, False,
+                                      axis_ratio=O.AXIS_RATIO_GPU, extended=True)
+            _, bm, _ = O.group_fields(coils, args, (pts[k] - e)[None], False, True, False,
+                                      axis_ratio=O.AXIS_RATIO_GPU, extended=True)
+            fd[ax] = mom[k] @ (bp[0] - bm[0]) / (2 * h)
+        assert np.linalg.norm(F - fd) <= 1e-7 * np.linalg.norm(fd), (k, F, fd)
