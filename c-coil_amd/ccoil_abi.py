@@ -19,7 +19,8 @@ DEVICE_POINTERS, NO_NEAR_FP64, FORCE_REFERENCE_SCALE = 1, 2, 4
 PAIR_M, PAIR_FT, PAIR_DM = 1, 2, 4
 
 EXPORTED = ["ccoil_create", "ccoil_destroy", "ccoil_last_error", "ccoil_version", "ccoil_set_stream",
-            "ccoil_synchronize", "ccoil_gauss_legendre", "ccoil_fields", "ccoil_pairs", "ccoil_field_gradient"]
+            "ccoil_synchronize", "ccoil_gauss_legendre", "ccoil_fields", "ccoil_pairs", "ccoil_field_gradient",
+            "ccoil_pairs_zaxis", "ccoil_self_inductance"]
 GRAD_TERMS = ("current", "R", "a", "b", "px", "py", "pz", "alpha", "beta")
 
 
@@ -66,8 +67,13 @@ def lib():
                                   ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32, vp, vp, vp, vp, ctypes.POINTER(Report)]
         L.ccoil_field_gradient.argtypes = [vp, ctypes.POINTER(Coils), vp, ctypes.c_int64, vp, ctypes.c_int32,
                                            ctypes.c_uint32, vp, ctypes.POINTER(Report)]
+        L.ccoil_pairs_zaxis.argtypes = [vp, ctypes.POINTER(Coils), ctypes.POINTER(Coils), vp, vp, ctypes.c_int64,
+                                        ctypes.c_uint32, ctypes.c_uint32, vp, vp, ctypes.POINTER(Report)]
+        L.ccoil_self_inductance.argtypes = [vp, ctypes.POINTER(Coils), vp, ctypes.c_uint32, vp,
+                                            ctypes.POINTER(Report)]
         for f in ("ccoil_create", "ccoil_destroy", "ccoil_set_stream", "ccoil_synchronize", "ccoil_gauss_legendre",
-                  "ccoil_fields", "ccoil_pairs", "ccoil_field_gradient"):
+                  "ccoil_fields", "ccoil_pairs", "ccoil_field_gradient", "ccoil_pairs_zaxis",
+                  "ccoil_self_inductance"):
             getattr(L, f).restype = ctypes.c_int32
         _lib = L
     return _lib
@@ -207,4 +213,30 @@ class Context:
                                       n_pairs, _ptr(ro), _ptr(hr), what, precision, flags, _ptr(out.get("M")),
                                       _ptr(out.get("F")), _ptr(out.get("T")), _ptr(out.get("dM")),
                                       ctypes.byref(rep)))
+        return out, rep
+
+    def pairs_zaxis(self, prim, sec, n_pairs, what, flags=0, prim_index=None, sec_index=None, out=None):
+        """Coaxial pairs: {"M": (n,), "Fz": (n,)} (axial force on the secondary)."""
+        rep = Report()
+        if out is None:
+            out = {}
+            if what & PAIR_M:
+                out["M"] = np.zeros(n_pairs)
+            if what & PAIR_FT:
+                out["Fz"] = np.zeros(n_pairs)
+        pi = None if prim_index is None else np.ascontiguousarray(prim_index, dtype=np.int64)
+        si = None if sec_index is None else np.ascontiguousarray(sec_index, dtype=np.int64)
+        self._check(lib().ccoil_pairs_zaxis(self.h, ctypes.byref(prim.struct), ctypes.byref(sec.struct), _ptr(pi),
+                                            _ptr(si), n_pairs, what, flags, _ptr(out.get("M")), _ptr(out.get("Fz")),
+                                            ctypes.byref(rep)))
+        return out, rep
+
+    def self_inductance(self, coils, radial, flags=0, out=None):
+        """Self-inductance of every coil; radial: (n, 2) secondary thickness (blocks, increments)."""
+        rep = Report()
+        radial = np.ascontiguousarray(radial, dtype=np.int32).reshape(-1)
+        if out is None:
+            out = np.zeros(coils.n)
+        self._check(lib().ccoil_self_inductance(self.h, ctypes.byref(coils.struct), _ptr(radial), flags, _ptr(out),
+                                                ctypes.byref(rep)))
         return out, rep

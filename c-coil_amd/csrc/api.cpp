@@ -232,7 +232,7 @@ ccoil_status ccoil_destroy(ccoil_ctx ctx)
     DevBuf *bufs[] = {&ctx->coils, &ctx->fpartial, &ctx->pts, &ctx->outA, &ctx->outB, &ctx->outE, &ctx->outG,
                       &ctx->queue, &ctx->counter, &ctx->pprim, &ctx->psec, &ctx->ppairs, &ctx->ppartial,
                       &ctx->pcounter, &ctx->pM, &ctx->pF, &ctx->pT, &ctx->pdM, &ctx->acoils, &ctx->apts, &ctx->agB,
-                      &ctx->apartial, &ctx->agrad};
+                      &ctx->apartial, &ctx->agrad, &ctx->zitems, &ctx->zpartial, &ctx->zout};
     for (auto b : bufs) free_buf(*b);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);

@@ -39,6 +39,7 @@ struct ccoil_ctx_s {
     ccoil::DevBuf coils, pts, outA, outB, outE, outG, queue, counter, fpartial;
     ccoil::DevBuf pprim, psec, ppairs, ppartial, pcounter, pM, pF, pT, pdM;
     ccoil::DevBuf acoils, apts, agB, apartial, agrad;   // field adjoint
+    ccoil::DevBuf zitems, zpartial, zout;               // coaxial rules
     hipEvent_t ev0 = nullptr, ev1 = nullptr;   // around all compute kernels of a call
     hipEvent_t evm0 = nullptr, evm1 = nullptr; // around the main kernel (field_kernel / pair_kernel)
 };

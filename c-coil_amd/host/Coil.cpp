@@ -256,7 +256,30 @@ std::pair<vec3::Vector3, vec3::Vector3> Coil::computeForceOnDipoleMoment(vec3::V
 {
     cpuOnly("Coil::computeForceOnDipoleMoment");
 }
-double Coil::computeAndSetSelfInductance(PrecisionFactor) { cpuOnly("Coil::computeAndSetSelfInductance"); }
+// ComputeMInductanceMethods.cxx:55-80: the coil is moved to the origin with zero angles so that the coaxial rules
+// apply, with the pair arguments the reference plans for (coil, coil) on the CPU. THIN/RECTANGULAR coils take
+// calculateSelfInductance; FLAT coils take the slow coaxial mutual-inductance rule with itself, whose field is the
+// device field rule (the primary's increments in one block).
+double Coil::computeAndSetSelfInductance(PrecisionFactor pf)
+{
+    if (coilType == CoilType::FILAMENT)
+        throw std::logic_error("Coil loop self inductance is not defined, integral is divergent");
+    const vec3::Vector3 position = positionVector;
+    const double yA = yAxisAngle, zA = zAxisAngle;
+    setPositionAndOrientation();
+    double L = 0.0;
+    try {
+        const auto args = CoilPairArguments::getAppropriateCoilPairArguments(*this, *this, pf, CPU_ST, true, false);
+        L = coilType == CoilType::FLAT ? ccoil::bridge::zAxisMutualInductance(*this, *this, args)
+                                       : ccoil::bridge::selfInductance(*this, args);
+    } catch (...) {
+        setPositionAndOrientation(position, yA, zA);
+        throw;
+    }
+    setSelfInductance(L);
+    setPositionAndOrientation(position, yA, zA);
+    return L;
+}
 
 // --------------------------------------------------------------------------------------------
 // GPU field paths (CalculateAllFieldsGPU.cxx:16-309 -> ccoil_fields with a one-coil descriptor).
@@ -406,10 +429,12 @@ void pairGPU(const Coil &prim, const Coil &sec, const CoilPairArguments &args, u
 
 }  // namespace
 
+// ComputeMInductanceMethods.cxx:7-41: coaxial pairs take the z-axis rules, every other pair the general rule.
 double Coil::computeMutualInductance(const Coil &prim, const Coil &sec, const CoilPairArguments &args,
                                      ComputeMethod m)
 {
     requireGpu(m, "Coil::computeMutualInductance with CPU_ST/CPU_MT");
+    if (isZAxisCase(prim, sec)) return ccoil::bridge::zAxisMutualInductance(prim, sec, args);
     double M = 0.0;
     pairGPU(prim, sec, args, CCOIL_PAIR_M, true, &M, nullptr, nullptr);
     return M;
@@ -418,9 +443,7 @@ double Coil::computeMutualInductance(const Coil &prim, const Coil &sec, const Co
 double Coil::computeMutualInductance(const Coil &prim, const Coil &sec, PrecisionFactor pf, ComputeMethod m)
 {
     requireGpu(m, "Coil::computeMutualInductance with CPU_ST/CPU_MT");
-    // The GPU path integrates every geometry with the general rule (the reference's z-axis special case
-    // is CPU-only); arguments are planned as the reference plans them for this pair.
-    auto args = CoilPairArguments::getAppropriateCoilPairArguments(prim, sec, pf, m, false, false);
+    auto args = CoilPairArguments::getAppropriateCoilPairArguments(prim, sec, pf, m, isZAxisCase(prim, sec), false);
     return computeMutualInductance(prim, sec, args, m);
 }
 
@@ -438,6 +461,9 @@ std::pair<vec3::Vector3, vec3::Vector3> Coil::computeForceTorque(const Coil &pri
                                                                  const CoilPairArguments &args, ComputeMethod m)
 {
     requireGpu(m, "Coil::computeForceTorque with CPU_ST/CPU_MT");
+    // ComputeForceMethods.cxx:5-24: a coaxial pair has only the axial force
+    if (isZAxisCase(prim, sec))
+        return {vec3::Vector3(0.0, 0.0, ccoil::bridge::zAxisForce(prim, sec, args)), vec3::Vector3()};
     double F[3], T[3];
     pairGPU(prim, sec, args, CCOIL_PAIR_FT, false, nullptr, F, T);
     return {vec3::Vector3(F[0], F[1], F[2]), vec3::Vector3(T[0], T[1], T[2])};
@@ -447,7 +473,7 @@ std::pair<vec3::Vector3, vec3::Vector3> Coil::computeForceTorque(const Coil &pri
                                                                  PrecisionFactor pf, ComputeMethod m)
 {
     requireGpu(m, "Coil::computeForceTorque with CPU_ST/CPU_MT");
-    auto args = CoilPairArguments::getAppropriateCoilPairArguments(prim, sec, pf, m, false, false);
+    auto args = CoilPairArguments::getAppropriateCoilPairArguments(prim, sec, pf, m, isZAxisCase(prim, sec), false);
     return computeForceTorque(prim, sec, args, m);
 }
 

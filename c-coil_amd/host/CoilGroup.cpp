@@ -137,41 +137,76 @@ vec3::Matrix3Array CoilGroup::computeAllBGradientMatrices(const vec3::Vector3Arr
 namespace {
 
 // Every member except `secondary` (matched by id, CoilGroup/MInductance/ComputeMInductance.cxx:14-16) against
-// the secondary, general rule, pair arguments planned per member as Coil::computeMutualInductance does.
+// the secondary, each pair planned and dispatched as Coil::computeMutualInductance / computeForceTorque do it:
+// coaxial members take the z-axis rules (the THIN/RECTANGULAR ones batched in one ccoil_pairs_zaxis call), all
+// others the general rule (one ccoil_pairs call). Member results are summed in member order.
 void groupPairsGeneral(const std::vector<std::shared_ptr<Coil>> &coils, const Coil &sec, PrecisionFactor pf,
                        uint32_t what, double *M, double *F, double *T)
 {
-    CoilSoA p, s;
+    CoilSoA p, s, zp, zs;
     std::vector<double> offsets;
     std::vector<int32_t> halves;
+    enum Kind { kGeneral, kZFast, kZSlow };
+    std::vector<std::pair<Kind, size_t>> members;   // (rule, index within that rule's batch)
+    std::vector<double> slowM, slowF;
+    auto fast = [](const Coil &c) {
+        return c.getCoilType() == CoilType::THIN || c.getCoilType() == CoilType::RECTANGULAR;
+    };
     for (auto &c : coils) {
         if (c->getId() == sec.getId()) continue;
-        const auto args = CoilPairArguments::getAppropriateCoilPairArguments(*c, sec, pf, GPU, false, false);
-        p.add(*c, args.primaryPrecision);
-        s.add(sec, args.secondaryPrecision);
-        auto ip = Coil::improvedPrecisionCase(*c, sec);
-        const bool useHalf = (what & CCOIL_PAIR_M) && ip.first;
-        offsets.push_back(useHalf ? ip.second : 0.0);
-        halves.push_back(useHalf ? 1 : 0);
+        const bool z = Coil::isZAxisCase(*c, sec);
+        const auto args = CoilPairArguments::getAppropriateCoilPairArguments(*c, sec, pf, GPU, z, false);
+        if (z && fast(*c) && fast(sec)) {
+            members.emplace_back(kZFast, zp.size());
+            zp.add(*c, args.primaryPrecision);
+            zs.add(sec, args.secondaryPrecision);
+        } else if (z) {
+            members.emplace_back(kZSlow, slowM.size());
+            slowM.push_back((what & CCOIL_PAIR_M) ? ccoil::bridge::zAxisMutualInductance(*c, sec, args) : 0.0);
+            slowF.push_back((what & CCOIL_PAIR_FT) ? ccoil::bridge::zAxisForce(*c, sec, args) : 0.0);
+        } else {
+            members.emplace_back(kGeneral, p.size());
+            p.add(*c, args.primaryPrecision);
+            s.add(sec, args.secondaryPrecision);
+            auto ip = Coil::improvedPrecisionCase(*c, sec);
+            const bool useHalf = (what & CCOIL_PAIR_M) && ip.first;
+            offsets.push_back(useHalf ? ip.second : 0.0);
+            halves.push_back(useHalf ? 1 : 0);
+        }
     }
-    const int64_t n = (int64_t)p.size();
     if (what & CCOIL_PAIR_M) *M = 0.0;
     if (what & CCOIL_PAIR_FT)
         for (int k = 0; k < 3; ++k) F[k] = T[k] = 0.0;
-    if (n == 0) return;
-    std::vector<double> m(n), f(3 * n), t(3 * n);
-    const ccoil_coils pv = p.view(), sv = s.view();
+    const int64_t n = (int64_t)p.size(), nz = (int64_t)zp.size();
+    std::vector<double> m(n), f(3 * n), t(3 * n), zm(nz), zf(nz);
     ccoil_ctx ctx = ccoil::bridge::context();
-    ccoil::bridge::check(ccoil_pairs(ctx, &pv, &sv, nullptr, nullptr, n, offsets.data(), halves.data(), what,
-                                     ccoil::bridge::precision(), 0, m.data(), f.data(), t.data(), nullptr, nullptr),
-                         ctx, "ccoil_pairs");
-    for (int64_t i = 0; i < n; ++i) {
-        if (what & CCOIL_PAIR_M) *M += m[i];
-        if (what & CCOIL_PAIR_FT)
-            for (int k = 0; k < 3; ++k) {
-                F[k] += f[3 * i + k];
-                T[k] += t[3 * i + k];
-            }
+    if (n > 0) {
+        const ccoil_coils pv = p.view(), sv = s.view();
+        ccoil::bridge::check(ccoil_pairs(ctx, &pv, &sv, nullptr, nullptr, n, offsets.data(), halves.data(), what,
+                                         ccoil::bridge::precision(), 0, m.data(), f.data(), t.data(), nullptr,
+                                         nullptr),
+                             ctx, "ccoil_pairs");
+    }
+    if (nz > 0) {
+        const ccoil_coils pv = zp.view(), sv = zs.view();
+        ccoil::bridge::check(ccoil_pairs_zaxis(ctx, &pv, &sv, nullptr, nullptr, nz, what & (CCOIL_PAIR_M | CCOIL_PAIR_FT),
+                                               0, zm.data(), zf.data(), nullptr),
+                             ctx, "ccoil_pairs_zaxis");
+    }
+    for (const auto &mk : members) {
+        const size_t i = mk.second;
+        if (mk.first == kGeneral) {
+            if (what & CCOIL_PAIR_M) *M += m[i];
+            if (what & CCOIL_PAIR_FT)
+                for (int k = 0; k < 3; ++k) {
+                    F[k] += f[3 * i + k];
+                    T[k] += t[3 * i + k];
+                }
+        } else {
+            // a coaxial pair has only the axial force and no torque (ComputeForceMethods.cxx:8-21)
+            if (what & CCOIL_PAIR_M) *M += mk.first == kZFast ? zm[i] : slowM[i];
+            if (what & CCOIL_PAIR_FT) F[2] += mk.first == kZFast ? zf[i] : slowF[i];
+        }
     }
 }
 

@@ -1,6 +1,8 @@
 #include "GpuBridge.h"
 
+#include <cmath>
 #include <cstdlib>
+#include <map>
 #include <memory>
 #include <new>
 #include <stdexcept>
@@ -134,6 +136,113 @@ std::vector<std::pair<vec3::Vector3, vec3::Vector3>> dipoleForceTorque(const Coi
         out[k] = {F, T};
     }
     return out;
+}
+
+namespace {
+
+bool fastPair(const Coil &c)
+{
+    return c.getCoilType() == CoilType::THIN || c.getCoilType() == CoilType::RECTANGULAR;
+}
+
+// Gauss-Legendre rule of order n as the device generates it (cached per thread).
+const std::pair<std::vector<double>, std::vector<double>> &rule(int n)
+{
+    thread_local std::map<int, std::pair<std::vector<double>, std::vector<double>>> cache;
+    auto it = cache.find(n);
+    if (it != cache.end()) return it->second;
+    std::vector<double> x((size_t)n), w((size_t)n);
+    ccoil_ctx ctx = context();
+    check(ccoil_gauss_legendre(ctx, n, x.data(), w.data()), ctx, "ccoil_gauss_legendre");
+    return cache.emplace(n, std::make_pair(std::move(x), std::move(w))).first->second;
+}
+
+// The slow coaxial rule: quantity q (CCOIL_A or CCOIL_B) of the primary at the secondary's r-z grid, weighted by
+// 0.25 r w_z w_r, summed in grid order and divided by the number of blocks.
+double zAxisSlow(const Coil &prim, const Coil &sec, const CoilPairArguments &args, uint32_t q)
+{
+    const PrecisionArguments &sp = args.secondaryPrecision;
+    const int lb = sp.lengthBlocks, li = sp.lengthIncrements, tb = sp.thicknessBlocks, ti = sp.thicknessIncrements;
+    const double lbs = sec.getLength() / lb, tbs = sec.getThickness() / tb;
+    const auto &rz = rule(li);
+    const auto &rr = rule(ti);
+    const double zDisp = sec.getPositionVector().z;
+    vec3::Vector3Array pts;
+    std::vector<double> w;
+    pts.reserve((size_t)lb * li * tb * ti);
+    w.reserve((size_t)lb * li * tb * ti);
+    for (int zb = 0; zb < lb; ++zb)
+        for (int rb = 0; rb < tb; ++rb) {
+            const double zbp = -1 * (sec.getLength() * 0.5) + lbs * (zb + 0.5);
+            const double rbp = sec.getInnerRadius() + tbs * (rb + 0.5);
+            for (int zi = 0; zi < li; ++zi)
+                for (int ri = 0; ri < ti; ++ri) {
+                    const double z = zDisp + zbp + (lbs * 0.5) * rz.first[(size_t)zi];
+                    const double r = rbp + (tbs * 0.5) * rr.first[(size_t)ri];
+                    pts.append(r, 0.0, z);
+                    w.push_back(0.25 * r * rz.second[(size_t)zi] * rr.second[(size_t)ri]);
+                }
+        }
+    CoilSoA p;
+    p.add(prim, args.primaryPrecision);
+    const ccoil_coils v = p.view();
+    std::vector<double> out(3 * pts.size());
+    ccoil_ctx ctx = context();
+    check(ccoil_fields(ctx, &v, pts.data(), (int64_t)pts.size(), q, precision(), 0, q == CCOIL_A ? out.data() : nullptr,
+                       q == CCOIL_B ? out.data() : nullptr, nullptr, nullptr, nullptr),
+          ctx, "ccoil_fields");
+    double sum = 0.0;
+    for (size_t i = 0; i < pts.size(); ++i) {
+        const double *o = &out[3 * i];
+        sum += (q == CCOIL_A ? std::sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]) : o[0]) * w[i];
+    }
+    return sum / (lb * tb);
+}
+
+}  // namespace
+
+double zAxisMutualInductance(const Coil &prim, const Coil &sec, const CoilPairArguments &args)
+{
+    if (fastPair(prim) && fastPair(sec)) {
+        CoilSoA p, s;
+        p.add(prim, args.primaryPrecision);
+        s.add(sec, args.secondaryPrecision);
+        const ccoil_coils pv = p.view(), sv = s.view();
+        double M = 0.0;
+        ccoil_ctx ctx = context();
+        check(ccoil_pairs_zaxis(ctx, &pv, &sv, nullptr, nullptr, 1, CCOIL_PAIR_M, 0, &M, nullptr, nullptr), ctx,
+              "ccoil_pairs_zaxis");
+        return M;
+    }
+    return zAxisSlow(prim, sec, args, CCOIL_A) * 2 * M_PI * sec.getNumOfTurns() / prim.getCurrent();
+}
+
+double zAxisForce(const Coil &prim, const Coil &sec, const CoilPairArguments &args)
+{
+    if (fastPair(prim) && fastPair(sec)) {
+        CoilSoA p, s;
+        p.add(prim, args.primaryPrecision);
+        s.add(sec, args.secondaryPrecision);
+        const ccoil_coils pv = p.view(), sv = s.view();
+        double F = 0.0;
+        ccoil_ctx ctx = context();
+        check(ccoil_pairs_zaxis(ctx, &pv, &sv, nullptr, nullptr, 1, CCOIL_PAIR_FT, 0, nullptr, &F, nullptr), ctx,
+              "ccoil_pairs_zaxis");
+        return F;
+    }
+    return (-1.0) * zAxisSlow(prim, sec, args, CCOIL_B) * 2 * M_PI * sec.getNumOfTurns() * sec.getCurrent();
+}
+
+double selfInductance(const Coil &coil, const CoilPairArguments &args)
+{
+    CoilSoA c;
+    c.add(coil, args.primaryPrecision);
+    const ccoil_coils v = c.view();
+    const int32_t radial[2] = {args.secondaryPrecision.thicknessBlocks, args.secondaryPrecision.thicknessIncrements};
+    double L = 0.0;
+    ccoil_ctx ctx = context();
+    check(ccoil_self_inductance(ctx, &v, radial, 0, &L, nullptr), ctx, "ccoil_self_inductance");
+    return L;
 }
 
 }  // namespace bridge

@@ -38,5 +38,15 @@ std::vector<std::pair<vec3::Vector3, vec3::Vector3>> dipoleForceTorque(const Coi
                                                                        const vec3::Vector3Array &points,
                                                                        const vec3::Vector3Array &moments);
 
+// Coaxial pairs (Coil::isZAxisCase). THIN/RECTANGULAR pairs take the analytic-length rule of ccoil_pairs_zaxis
+// (CalculateMInductanceZAxis.cxx:84-221, CalculateForceZAxis.cxx:82-219); any other pair takes the slow rule:
+// |A| (mutual inductance) or B_x (force) of the primary, evaluated by ccoil_fields with the primary arguments, at
+// the secondary's r-z grid (CalculateMInductanceZAxis.cxx:17-82, CalculateForceZAxis.cxx:16-80).
+double zAxisMutualInductance(const Coil &prim, const Coil &sec, const CoilPairArguments &args);
+double zAxisForce(const Coil &prim, const Coil &sec, const CoilPairArguments &args);
+
+// Coil::calculateSelfInductance (CalculateMInductanceZAxis.cxx:224-317) through ccoil_self_inductance.
+double selfInductance(const Coil &coil, const CoilPairArguments &args);
+
 }  // namespace bridge
 }  // namespace ccoil

@@ -152,6 +152,25 @@ ccoil_status ccoil_field_gradient(ccoil_ctx ctx, const ccoil_coils *coils, const
                                   const double *dL_dB, int32_t precision, uint32_t flags, double *grad,
                                   ccoil_report *report);
 
+/* Coaxial pairs (the z-axis case of Coil::isZAxisCase, SupportFunctions.cxx:55-67: both axes on the global z axis)
+ * with the analytic length integral of both coils, for THIN and RECTANGULAR coils (use_fast = 1). For each pair
+ * writes M (what & CCOIL_PAIR_M) and Fz, the axial force on the secondary (what & CCOIL_PAIR_FT; the transverse
+ * force and the torque of a coaxial pair are zero). Only pos[2] of each coil enters. Primary rule: n_ang, n_thk
+ * and the angular and thickness blocks; secondary rule: n_thk and the thickness blocks (blocks NULL = 1). Needs
+ * J, current (M divides by I1 I2) and turns. Evaluated in FP64 with the reference's per-node expressions.
+ * Replaces Coil::calculateMutualInductanceZAxisFast (CalculateMInductanceZAxis.cxx:84-221) and
+ * Coil::calculateAmpereForceZAxisFast (CalculateForceZAxis.cxx:82-219). */
+ccoil_status ccoil_pairs_zaxis(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coils *sec,
+                               const int64_t *prim_index, const int64_t *sec_index, int64_t n_pairs,
+                               uint32_t what, uint32_t flags, double *M, double *Fz, ccoil_report *report);
+
+/* Self-inductance of n = coils->n THIN or RECTANGULAR coils (L, 1 per coil). Coil rule: n_ang, n_thk and the
+ * angular and thickness blocks; radial: 2 per coil (thickness blocks, thickness increments) of the secondary
+ * precision the reference plans for the pair (coil, coil). Needs J, current and turns.
+ * Replaces Coil::calculateSelfInductance (CalculateMInductanceZAxis.cxx:224-317). */
+ccoil_status ccoil_self_inductance(ccoil_ctx ctx, const ccoil_coils *coils, const int32_t *radial, uint32_t flags,
+                                   double *L, ccoil_report *report);
+
 #ifdef __cplusplus
 }
 #endif

@@ -656,3 +656,35 @@ void orc_force_torque_gpu_rule(const orc_coil *prim, const orc_coil *sec, const 
     pthread_once(&g_once, gl_init);
     gpu_rule_pair(prim, sec, prim_inc, sec_inc, 1, correction, NULL, F, Tq);
 }
+
+#define REAL double
+#define SUF(name) name##_d
+#define SQRT sqrt
+#define LOG log
+#define COS cos
+#include "ccoil_oracle_zaxis.inc"
+#undef REAL
+#undef SUF
+#undef SQRT
+#undef LOG
+#undef COS
+
+#define REAL long double
+#define SUF(name) name##_ld
+#define SQRT sqrtl
+#define LOG logl
+#define COS cosl
+#include "ccoil_oracle_zaxis.inc"
+#undef REAL
+#undef SUF
+#undef SQRT
+#undef LOG
+#undef COS
+
+double orc_zaxis_rule(const orc_coil *prim, const orc_coil *sec, const int32_t args[12], int mode, int extended)
+{
+    pthread_once(&g_once, gl_init);
+    if (mode < 0 || mode > 2) return 0.0;
+    if (extended) return (double)zaxis_rule_ld(prim, sec, args, mode);
+    return zaxis_rule_d(prim, sec, args, mode);
+}

@@ -94,6 +94,15 @@ double orc_mutual_inductance_gpu_rule(const orc_coil *prim, const orc_coil *sec,
 void orc_force_torque_gpu_rule(const orc_coil *prim, const orc_coil *sec, const int32_t prim_inc[3],
                                const int32_t sec_inc[3], double correction, double F[3], double Tq[3]);
 
+/* Coaxial rules (used when Coil::isZAxisCase holds, SupportFunctions.cxx:55-67): mode 0 = mutual inductance
+ * (CalculateMInductanceZAxis.cxx:84-221), 1 = axial force on the secondary (CalculateForceZAxis.cxx:82-219),
+ * 2 = self-inductance of prim (CalculateMInductanceZAxis.cxx:224-317; sec must be prim, the radial rule is the
+ * secondary thickness arguments). extended != 0 evaluates the same rule in long double.
+ * Pinned by: the general rule (orc_mutual_inductance_general / orc_force_torque_general) on coaxial pairs,
+ * F = I1 I2 dM/dz by central differences, and Nagaoka's closed form of a thin solenoid's self-inductance
+ * (tests/test_oracle_zaxis.py). */
+double orc_zaxis_rule(const orc_coil *prim, const orc_coil *sec, const int32_t args[12], int mode, int extended);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,6 +68,8 @@ def lib(fast=False):
         L.orc_mutual_inductance_gpu_rule.argtypes = [cp, cp, ip, ip]
         L.orc_mutual_inductance_gpu_rule.restype = ctypes.c_double
         L.orc_force_torque_gpu_rule.argtypes = [cp, cp, ip, ip, ctypes.c_double, dp, dp]
+        L.orc_zaxis_rule.argtypes = [cp, cp, ip, ctypes.c_int, ctypes.c_int]
+        L.orc_zaxis_rule.restype = ctypes.c_double
         _lib[key] = L
     return _lib[key]
 
@@ -193,3 +195,61 @@ def force_torque_gpu_rule(prim, sec, prim_inc, sec_inc, correction):
     lib().orc_force_torque_gpu_rule(ctypes.byref(prim), ctypes.byref(sec), _ip(pi), _ip(si), correction,
                                     _dp(F), _dp(T))
     return F, T
+
+
+def _zaxis(prim, sec, args12, mode, extended):
+    args12 = np.ascontiguousarray(args12, dtype=np.int32)
+    return lib().orc_zaxis_rule(ctypes.byref(prim), ctypes.byref(sec), _ip(args12), mode, int(extended))
+
+
+def mutual_inductance_zaxis_fast(prim, sec, args12, extended=False):
+    """Coil::calculateMutualInductanceZAxisFast (CalculateMInductanceZAxis.cxx:84-221)."""
+    return _zaxis(prim, sec, args12, 0, extended)
+
+
+def force_zaxis_fast(prim, sec, args12, extended=False):
+    """Coil::calculateAmpereForceZAxisFast (CalculateForceZAxis.cxx:82-219): axial force on the secondary."""
+    return _zaxis(prim, sec, args12, 1, extended)
+
+
+def self_inductance(coil, args12, extended=False):
+    """Coil::calculateSelfInductance (CalculateMInductanceZAxis.cxx:224-317)."""
+    return _zaxis(coil, coil, args12, 2, extended)
+
+
+def _zaxis_slow_grid(sec, args12):
+    """Secondary r-z grid and weights of CalculateMInductanceZAxis.cxx:19-72 / CalculateForceZAxis.cxx:16-69."""
+    lb, li = int(args12[6 + 2]), int(args12[6 + 5])
+    tb, ti = int(args12[6 + 1]), int(args12[6 + 4])
+    lbs, tbs = sec.b / lb, sec.a / tb
+    xl, wl = gauss_legendre(li)
+    xt, wt = gauss_legendre(ti)
+    pts, w = [], []
+    for zb in range(lb):
+        for rb in range(tb):
+            zbp = -(sec.b * 0.5) + lbs * (zb + 0.5)
+            rbp = sec.R + tbs * (rb + 0.5)
+            for zi in range(li):
+                for ri in range(ti):
+                    z = sec.pos[2] + zbp + (lbs * 0.5) * xl[zi]
+                    r = rbp + (tbs * 0.5) * xt[ri]
+                    pts.append((r, 0.0, z))
+                    w.append(0.25 * r * wl[zi] * wt[ri])
+    return np.array(pts), np.array(w), lb * tb
+
+
+def mutual_inductance_zaxis_slow(prim, sec, args12, field_args6, axis_ratio=AXIS_RATIO_GPU, extended=False):
+    """Coil::calculateMutualInductanceZAxisSlow (CalculateMInductanceZAxis.cxx:17-82): |A| of the primary at the
+    secondary's r-z grid, evaluated with field_args6 (the GPU field rule: one block, the primary increments)."""
+    pts, w, blocks = _zaxis_slow_grid(sec, args12)
+    a, _, _ = coil_fields(prim, field_args6, pts, True, False, False, axis_ratio, extended=extended)
+    m = float(np.sum(np.linalg.norm(a, axis=1) * w)) / blocks
+    return m * 2 * np.pi * sec.N / prim.I
+
+
+def force_zaxis_slow(prim, sec, args12, field_args6, axis_ratio=AXIS_RATIO_GPU, extended=False):
+    """Coil::calculateAmpereForceZAxisSlow (CalculateForceZAxis.cxx:16-80): B_x of the primary at the grid."""
+    pts, w, blocks = _zaxis_slow_grid(sec, args12)
+    _, b, _ = coil_fields(prim, field_args6, pts, False, True, False, axis_ratio, extended=extended)
+    f = float(np.sum(b[:, 0] * w)) / blocks
+    return -f * 2 * np.pi * sec.N * sec.I
