@@ -23,12 +23,15 @@ Reported beside it in the same line:
   adjoint      the coil-design workload (64 THIN/RECTANGULAR coils near the axis, 4,096 points in a small
                cylinder): forward B (ccoil_fields) and the adjoint of a field loss (ccoil_field_gradient, all 9
                parameters of every coil), FP64 and FP32C, and the adjoint / forward time ratio;
+  coaxial      4,096 coaxial THIN/RECTANGULAR pairs through the z-axis fast rules (ccoil_pairs_zaxis, M + Fz);
+  io           small-call cost through the c_coil module: Vector3Array API vs numpy overload vs DLPack device tensors;
   cpu_baseline the FP64 CPU restatement of the reference's CPU_MT field path (-O3 -mavx2 -ffast-math, point chunks on
                a thread pool), timed on a bounded sample of the same workload on this host.
 With --large (not part of the default run): B+gradient of a 1,024-coil group at 2^22 points, and the full
 1,024 x 1,024 mutual-inductance matrix with its position gradient (CCOIL_PAIR_DM) checked against F / (I_i I_j).
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -96,6 +99,27 @@ def cpu_baseline(params, pts, rows, n_sample):
                       f"oracle/liboracle_fast.so (-O3 -mavx2 -ffast-math), point chunks on {threads} threads"}
 
 
+def cpu_coaxial_baseline(cpp, csp, cpr, csr, n_sample):
+    """The oracle's fast coaxial rules (the reference's CPU_ST calculateMutualInductanceZAxisFast and
+    calculateAmpereForceZAxisFast, -O3 -mavx2 -ffast-math) on the first n_sample pairs of the coaxial leg, one core."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.lib(True)
+    mk = lambda p: O.make_coil(p["R"], p["a"], p["b"], int(p["N"]), p["I"], 50.0, tuple(float(x) for x in p["pos"]),
+                               p["alpha"], p["beta"])
+    a12 = lambda rp, rs: np.array(list(rp["blocks"]) + [rp["n_ang"], rp["n_thk"], rp["n_len"]] + list(rs["blocks"]) +
+                                  [rs["n_ang"], rs["n_thk"], rs["n_len"]], dtype=np.int32)
+    L = O.lib(True)
+    work = [(mk(p), mk(s), a12(rp, rs)) for p, s, rp, rs in zip(cpp[:n_sample], csp[:n_sample], cpr, csr)]
+    t0 = time.perf_counter()
+    for p, s, a in work:
+        L.orc_zaxis_rule(ctypes.byref(p), ctypes.byref(s), a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), 0, 0)
+        L.orc_zaxis_rule(ctypes.byref(p), ctypes.byref(s), a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), 1, 0)
+    dt = time.perf_counter() - t0
+    return {"value": len(work) / dt, "unit": "coaxial pairs (M + Fz)/s", "cores": 1, "kind": "port",
+            "seconds": dt, "sample": f"the first {len(work)} pairs of the coaxial leg, oracle/liboracle_fast.so"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +135,9 @@ def main():
     ap.add_argument("--no-pairs", action="store_true", help="skip the pair leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--no-coaxial", action="store_true", help="skip the coaxial-pair leg")
+    ap.add_argument("--coaxial-pairs", type=int, default=4096)
+    ap.add_argument("--no-io", action="store_true", help="skip the small-call I/O leg")
     ap.add_argument("--large", action="store_true", help="also run the 1,024-coil field and M-matrix legs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device-to-device copy 3 times (calibrates the HBM byte counters)")
@@ -264,6 +291,55 @@ def main():
         result["adjoint"] = adj
         log(f"adjoint: {json.dumps(adj)}")
 
+    if not args.no_coaxial:
+        cpp, csp, cpr, csr = W.coaxial_workload(args.coaxial_pairs, seed=0)
+        CP, CS = abi.CoilSet.from_rows(cpr), abi.CoilSet.from_rows(csr)
+        ctx.pairs_zaxis(CP, CS, len(cpr), abi.PAIR_M | abi.PAIR_FT)
+        t0 = time.perf_counter()
+        co, crep = ctx.pairs_zaxis(CP, CS, len(cpr), abi.PAIR_M | abi.PAIR_FT)
+        cdt = time.perf_counter() - t0
+        result["coaxial"] = {"workload": f"{len(cpr)} coaxial THIN/RECTANGULAR pairs, z-axis fast rules, M + Fz, "
+                                         f"planned at PrecisionFactor 5 (GPU method)",
+                             "pairs_per_s": len(cpr) / cdt, "ms": cdt * 1e3, "kernel_ms": crep.main_kernel_ms,
+                             "node_evaluations": int(crep.node_evaluations),
+                             "node_evals_per_s": crep.node_evaluations / (crep.main_kernel_ms * 1e-3)}
+        log(f"coaxial: {json.dumps(result['coaxial'])}")
+
+    if not args.no_io:
+        # small calls through the c_coil module: the reference's Vector3Array API, the numpy overload and DLPack
+        # device tensors (compute_all_fields_into), one coil, B at 4,096 points
+        import c_coil
+        C = c_coil.coil
+        C.Coil.set_gpu_precision(C.GPU_FP64)
+        icoil = W.make_coil(params[0])
+        ipts = np.ascontiguousarray(pts[:4096])
+        va = c_coil.tensor.Vector3Array()
+        for p in ipts:
+            va.append(float(p[0]), float(p[1]), float(p[2]))
+        dpts = torch.from_numpy(ipts).to(dev)
+        dB = torch.empty((len(ipts), 3), dtype=torch.float64, device=dev)
+
+        def per_call(fn, reps=50):
+            fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) / reps * 1e3
+
+        def v3():
+            r = icoil.compute_all_B_field_vectors(va, C.GPU)
+            return np.stack([np.array(r.x()), np.array(r.y()), np.array(r.z())], axis=1)
+
+        io = {"workload": "one coil, B at 4,096 points, CCOIL_FP64, per call including conversions",
+              "vector3array_ms": per_call(v3),
+              "numpy_ms": per_call(lambda: icoil.compute_all_B_field_vectors(ipts, C.GPU)),
+              "dlpack_device_ms": per_call(lambda: icoil.compute_all_fields_into(dpts, B=dB))}
+        C.Coil.set_gpu_precision(C.GPU_FP32C)
+        result["io"] = io
+        log(f"io: {json.dumps(io)}")
+
     if args.large:
         lparams, lpts = W.group_workload(1024, 1 << 22, seed=0)
         lrows = W.group_rows(lparams)
@@ -306,6 +382,9 @@ def main():
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(params, pts, rows, min(args.cpu_points, n))
         log(f"cpu baseline: {result['cpu_baseline']['value']:.3g} coil-points/s")
+        if not args.no_coaxial:
+            result["cpu_baseline"]["coaxial"] = cpu_coaxial_baseline(cpp, csp, cpr, csr, 1024)
+            log(f"cpu baseline coaxial: {result['cpu_baseline']['coaxial']['value']:.3g} pairs/s")
     else:
         result["cpu_baseline"] = None
     ctx.close()

@@ -20,7 +20,7 @@ PAIR_M, PAIR_FT, PAIR_DM = 1, 2, 4
 
 EXPORTED = ["ccoil_create", "ccoil_destroy", "ccoil_last_error", "ccoil_version", "ccoil_set_stream",
             "ccoil_synchronize", "ccoil_gauss_legendre", "ccoil_fields", "ccoil_pairs", "ccoil_field_gradient",
-            "ccoil_pairs_zaxis", "ccoil_self_inductance"]
+            "ccoil_pairs_zaxis", "ccoil_self_inductance", "ccoil_get_stream"]
 GRAD_TERMS = ("current", "R", "a", "b", "px", "py", "pz", "alpha", "beta")
 
 

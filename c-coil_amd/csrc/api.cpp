@@ -250,6 +250,13 @@ ccoil_status ccoil_set_stream(ccoil_ctx ctx, void *stream)
     return CCOIL_OK;
 }
 
+ccoil_status ccoil_get_stream(ccoil_ctx ctx, void **stream)
+{
+    if (!ctx || !stream) return fail(ctx, CCOIL_ERR_ARGUMENT, "ccoil_get_stream: ctx and stream must not be NULL");
+    *stream = (void *)ctx->stream;
+    return CCOIL_OK;
+}
+
 ccoil_status ccoil_synchronize(ccoil_ctx ctx)
 {
     if (!ctx) return fail(nullptr, CCOIL_ERR_ARGUMENT, "ccoil_synchronize: ctx is NULL");

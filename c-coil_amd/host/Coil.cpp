@@ -368,6 +368,14 @@ Coil::computeAllForceOnDipoleMoments(const vec3::Vector3Array &points, const vec
     return ccoil::bridge::dipoleForceTorque(s, points, moments);
 }
 
+void Coil::computeAllFieldsGPU(const double *points, size_t n, double *A, double *B, double *E, double *G,
+                               bool devicePointers) const
+{
+    CoilSoA s;
+    s.add(*this, defaultPrecisionGPU);
+    ccoil::bridge::fieldsInto(s, points, n, A, B, E, G, devicePointers);
+}
+
 // --------------------------------------------------------------------------------------------
 // Pairs.
 // --------------------------------------------------------------------------------------------

@@ -225,6 +225,13 @@ class Coil {
     computeAllForceOnDipoleMoments(const vec3::Vector3Array &points, const vec3::Vector3Array &moments,
                                    ComputeMethod computeMethod = GPU) const;
 
+    // Zero-copy GPU fields at the default GPU precision (no counterpart in the reference; backs the numpy and
+    // DLPack entry points of the Python module): points are n AoS xyz doubles and every non-null output is written in
+    // place (A, B, E: 3 doubles per point; G: 9, the reference's T*G_local), all in one launch. devicePointers: all
+    // buffers are device pointers on the context's device, ordered on its stream.
+    void computeAllFieldsGPU(const double *points, size_t n, double *A, double *B, double *E, double *G,
+                             bool devicePointers = false) const;
+
     static std::vector<double> computeAllMutualInductanceArrangements(
         const Coil &primary, const Coil &secondary, const vec3::Vector3Array &primaryPositions,
         const vec3::Vector3Array &secondaryPositions, const std::vector<double> &primaryYAngles,

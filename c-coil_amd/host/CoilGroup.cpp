@@ -92,6 +92,14 @@ void groupFields(const std::vector<std::shared_ptr<Coil>> &coils, PrecisionFacto
 
 }  // namespace
 
+void CoilGroup::computeAllFieldsGPU(const double *points, size_t n, double *A, double *B, double *E, double *G,
+                                    bool devicePointers) const
+{
+    CoilSoA s;
+    for (auto &c : memberCoils) s.add(*c, PrecisionArguments::getCoilPrecisionArgumentsGPU(*c, defaultPrecisionFactor));
+    ccoil::bridge::fieldsInto(s, points, n, A, B, E, G, devicePointers);
+}
+
 std::vector<std::pair<vec3::Vector3, vec3::Vector3>>
 CoilGroup::computeAllForceOnDipoleMoments(const vec3::Vector3Array &points, const vec3::Vector3Array &moments,
                                           ComputeMethod m) const

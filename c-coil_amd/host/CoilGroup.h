@@ -52,6 +52,9 @@ class CoilGroup {
                        ComputeMethod computeMethod = CPU_ST) const;
     [[nodiscard]] std::pair<vec3::Vector3, vec3::Vector3> computeForceOnDipoleMoment(vec3::Vector3 pointVector,
                                                                                      vec3::Vector3 dipoleMoment) const;
+    // Zero-copy GPU fields of all members at the group's precision factor (see Coil::computeAllFieldsGPU).
+    void computeAllFieldsGPU(const double *points, size_t n, double *A, double *B, double *E, double *G,
+                             bool devicePointers = false) const;
     // Force and torque on point dipoles in the group's field (GPU, true Jacobian; see Coil).
     [[nodiscard]] std::vector<std::pair<vec3::Vector3, vec3::Vector3>>
     computeAllForceOnDipoleMoments(const vec3::Vector3Array &points, const vec3::Vector3Array &moments,

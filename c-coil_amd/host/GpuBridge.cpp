@@ -22,6 +22,7 @@ struct CtxHolder {
 };
 
 thread_local CtxHolder t_ctx;
+thread_local int t_device = 0;
 thread_local int t_precision = CCOIL_FP32C;
 
 }  // namespace
@@ -35,6 +36,7 @@ ccoil_ctx context()
         ccoil_status st = ccoil_create(dev, &c);
         if (st != CCOIL_OK) check(st, nullptr, "ccoil_create");
         t_ctx.ctx = c;
+        t_device = dev;
     }
     return t_ctx.ctx;
 }
@@ -49,6 +51,20 @@ void check(ccoil_status st, ccoil_ctx ctx, const char *what)
         case CCOIL_ERR_OUT_OF_MEMORY: throw std::bad_alloc();
         default: throw std::runtime_error(msg);
     }
+}
+
+int device()
+{
+    (void)context();
+    return t_device;
+}
+
+void *stream()
+{
+    ccoil_ctx ctx = context();
+    void *s = nullptr;
+    check(ccoil_get_stream(ctx, &s), ctx, "ccoil_get_stream");
+    return s;
 }
 
 int precision() { return t_precision; }
@@ -136,6 +152,18 @@ std::vector<std::pair<vec3::Vector3, vec3::Vector3>> dipoleForceTorque(const Coi
         out[k] = {F, T};
     }
     return out;
+}
+
+void fieldsInto(const CoilSoA &coils, const double *points, size_t n, double *A, double *B, double *E, double *G,
+                bool devicePointers)
+{
+    const uint32_t q = (A ? CCOIL_A : 0u) | (B ? CCOIL_B : 0u) | (E ? CCOIL_E : 0u) | (G ? CCOIL_G : 0u);
+    if (!q) throw std::invalid_argument("no output requested: pass at least one of A, B, E, G");
+    const ccoil_coils v = coils.view();
+    ccoil_ctx ctx = context();
+    check(ccoil_fields(ctx, &v, points, (int64_t)n, q, precision(), devicePointers ? CCOIL_DEVICE_POINTERS : 0u, A, B,
+                       E, G, nullptr),
+          ctx, "ccoil_fields");
 }
 
 namespace {

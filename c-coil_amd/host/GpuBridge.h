@@ -18,6 +18,12 @@ ccoil_ctx context();
 // CCOIL_ERR_OUT_OF_MEMORY -> std::bad_alloc, CCOIL_ERR_DEVICE -> std::runtime_error.
 void check(ccoil_status st, ccoil_ctx ctx, const char *what);
 
+// HIP device index of the calling thread's context (CCOIL_DEVICE, default 0).
+int device();
+
+// The stream of the calling thread's context (hipStream_t as void*).
+void *stream();
+
 int precision();
 void setPrecision(int precision);
 
@@ -37,6 +43,12 @@ struct CoilSoA {
 std::vector<std::pair<vec3::Vector3, vec3::Vector3>> dipoleForceTorque(const CoilSoA &coils,
                                                                        const vec3::Vector3Array &points,
                                                                        const vec3::Vector3Array &moments);
+
+// All requested fields of `coils` at n AoS xyz points in one ccoil_fields call: every non-null output is written in
+// place (A, B, E: 3 doubles per point; G: 9, the reference's T*G_local). devicePointers: device buffers on the
+// context's device.
+void fieldsInto(const CoilSoA &coils, const double *points, size_t n, double *A, double *B, double *E, double *G,
+                bool devicePointers);
 
 // Coaxial pairs (Coil::isZAxisCase). THIN/RECTANGULAR pairs take the analytic-length rule of ccoil_pairs_zaxis
 // (CalculateMInductanceZAxis.cxx:84-221, CalculateForceZAxis.cxx:82-219); any other pair takes the slow rule:

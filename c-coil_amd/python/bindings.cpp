@@ -4,12 +4,195 @@
 // part of this GPU library.
 #include "Coil.h"
 #include "CoilGroup.h"
+#include "GpuBridge.h"
 #include "ccoil_hip.h"
 
+#include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <cstdint>
+#include <stdexcept>
+
 namespace py = pybind11;
+
+// ------------------------------------------------------------------------------------------------------------------
+// Zero-copy buffers: numpy arrays and DLPack producers (torch tensors on the CPU or on the GPU, ...). No counterpart
+// in the reference, whose Python API converts every call through Vector3Array / Matrix3Array.
+// ------------------------------------------------------------------------------------------------------------------
+namespace zc {
+
+// The DLPack C ABI (dlpack.h of the DLPack specification), legacy unversioned capsule "dltensor".
+struct DLDevice {
+    int32_t device_type;
+    int32_t device_id;
+};
+struct DLDataType {
+    uint8_t code;
+    uint8_t bits;
+    uint16_t lanes;
+};
+struct DLTensor {
+    void *data;
+    DLDevice device;
+    int32_t ndim;
+    DLDataType dtype;
+    int64_t *shape;
+    int64_t *strides;
+    uint64_t byte_offset;
+};
+struct DLManagedTensor {
+    DLTensor dl_tensor;
+    void *manager_ctx;
+    void (*deleter)(DLManagedTensor *);
+};
+enum : int32_t { kDLCPU = 1, kDLCUDA = 2, kDLCUDAHost = 3, kDLROCM = 10, kDLROCMHost = 11 };
+enum : uint8_t { kDLFloat = 2 };
+
+struct Buffer {
+    double *ptr = nullptr;
+    int64_t rows = 0;
+    bool device = false;
+    py::object keep;   // the numpy array or the DLPack capsule: keeps the producer's memory alive for the call
+};
+
+// Rows of `width` doubles, row-major and contiguous: (n, 3) for points, A, B, E; (n, 9) or (n, 3, 3) for G.
+bool shapeOk(int ndim, const int64_t *shape, int width)
+{
+    if (ndim == 2) return shape[1] == width;
+    return ndim == 3 && width == 9 && shape[1] == 3 && shape[2] == 3;
+}
+
+Buffer view(py::handle obj, int width, bool output, const char *name)
+{
+    const std::string who(name);
+    Buffer b;
+    if (py::isinstance<py::array>(obj)) {
+        py::array arr = py::reinterpret_borrow<py::array>(obj);
+        if (output) {
+            if (!arr.dtype().is(py::dtype::of<double>()) || !(arr.flags() & py::array::c_style) || !arr.writeable())
+                throw std::invalid_argument(who + " must be a writable C-contiguous float64 array");
+        } else {
+            arr = py::array_t<double, py::array::c_style | py::array::forcecast>::ensure(obj);
+            if (!arr) throw std::invalid_argument(who + " cannot be read as a float64 array");
+        }
+        std::vector<int64_t> shape(arr.shape(), arr.shape() + arr.ndim());
+        if (!shapeOk((int)arr.ndim(), shape.data(), width))
+            throw std::invalid_argument(who + (width == 3 ? " must have shape (n, 3)" : " must have shape (n, 9) or (n, 3, 3)"));
+        b.ptr = static_cast<double *>(arr.mutable_data());
+        b.rows = shape[0];
+        b.keep = arr;
+        return b;
+    }
+    if (!py::hasattr(obj, "__dlpack__") || !py::hasattr(obj, "__dlpack_device__"))
+        throw std::invalid_argument(who + " must be a numpy array or a DLPack array (e.g. a torch tensor)");
+    py::tuple dev = obj.attr("__dlpack_device__")();
+    const int type = dev[0].cast<int>(), id = dev[1].cast<int>();
+    py::object cap;
+    if (type == kDLCUDA || type == kDLROCM) {
+        if (id != ccoil::bridge::device())
+            throw std::invalid_argument(who + " is on device " + std::to_string(id) + ", the context uses device " +
+                                        std::to_string(ccoil::bridge::device()));
+        // the producer orders its pending work before the context's stream
+        cap = obj.attr("__dlpack__")(py::arg("stream") = (intptr_t)ccoil::bridge::stream());
+        b.device = true;
+    } else if (type == kDLCPU || type == kDLCUDAHost || type == kDLROCMHost) {
+        cap = obj.attr("__dlpack__")();
+    } else {
+        throw std::invalid_argument(who + ": unsupported DLPack device type " + std::to_string(type));
+    }
+    auto *mt = static_cast<DLManagedTensor *>(PyCapsule_GetPointer(cap.ptr(), "dltensor"));
+    if (!mt) throw py::error_already_set();
+    const DLTensor &t = mt->dl_tensor;
+    if (t.dtype.code != kDLFloat || t.dtype.bits != 64 || t.dtype.lanes != 1)
+        throw std::invalid_argument(who + " must be float64");
+    if (!shapeOk(t.ndim, t.shape, width))
+        throw std::invalid_argument(who + (width == 3 ? " must have shape (n, 3)" : " must have shape (n, 9) or (n, 3, 3)"));
+    if (t.strides) {
+        int64_t expect = 1;
+        for (int k = t.ndim - 1; k >= 0; --k) {
+            if (t.shape[k] != 1 && t.strides[k] != expect) throw std::invalid_argument(who + " must be contiguous");
+            expect *= t.shape[k];
+        }
+    }
+    b.ptr = reinterpret_cast<double *>(static_cast<char *>(t.data) + t.byte_offset);
+    b.rows = t.shape[0];
+    b.keep = cap;   // the capsule's destructor returns the tensor to its producer
+    return b;
+}
+
+[[noreturn]] void cpuOnly(const char *what)
+{
+    throw std::logic_error(std::string(what) + " with CPU_ST/CPU_MT is computed on the CPU by C-Coil; this library "
+                                               "provides the GPU path (compute_method=GPU)");
+}
+
+// Fields into caller buffers: all of one side (host or device), one launch for every requested quantity.
+template <class Source>
+void into(const Source &src, py::handle points, py::handle A, py::handle B, py::handle E, py::handle G)
+{
+    Buffer p = view(points, 3, false, "points");
+    Buffer out[4];
+    const py::handle outs[4] = {A, B, E, G};
+    const char *names[4] = {"A", "B", "E", "G"};
+    bool any = false;
+    for (int k = 0; k < 4; ++k) {
+        if (outs[k].is_none()) continue;
+        out[k] = view(outs[k], k == 3 ? 9 : 3, true, names[k]);
+        if (out[k].rows != p.rows) throw std::invalid_argument(std::string(names[k]) + " must have as many rows as points");
+        if (out[k].device != p.device) throw std::invalid_argument("points and outputs must all be host or all device arrays");
+        any = true;
+    }
+    if (!any) throw std::invalid_argument("no output given: pass at least one of A, B, E, G");
+    py::gil_scoped_release nogil;
+    src.computeAllFieldsGPU(p.ptr, (size_t)p.rows, out[0].ptr, out[1].ptr, out[2].ptr, out[3].ptr, p.device);
+}
+
+// Host arrays in, a new numpy array out: q selects the quantity (0 A, 1 B, 2 E, 3 G).
+template <class Source>
+py::array compute(const Source &src, py::handle points, int q, ComputeMethod m, const char *what)
+{
+    if (m != GPU) cpuOnly(what);
+    Buffer p = view(points, 3, false, "point_vectors");
+    if (p.device) throw std::invalid_argument("device arrays: use compute_all_fields_into with device outputs");
+    py::array_t<double> out = q == 3 ? py::array_t<double>({(py::ssize_t)p.rows, (py::ssize_t)3, (py::ssize_t)3})
+                                     : py::array_t<double>({(py::ssize_t)p.rows, (py::ssize_t)3});
+    double *o = out.mutable_data();
+    {
+        py::gil_scoped_release nogil;
+        src.computeAllFieldsGPU(p.ptr, (size_t)p.rows, q == 0 ? o : nullptr, q == 1 ? o : nullptr,
+                                q == 2 ? o : nullptr, q == 3 ? o : nullptr, false);
+    }
+    return out;
+}
+
+template <class Cls, class Source>
+void add_methods(Cls &cls, const char *owner)
+{
+    static const std::string names[4] = {std::string(owner) + ".compute_all_A_potential_vectors",
+                                         std::string(owner) + ".compute_all_B_field_vectors",
+                                         std::string(owner) + ".compute_all_E_field_vectors",
+                                         std::string(owner) + ".compute_all_B_gradient_matrices"};
+    const char *py_names[4] = {"compute_all_A_potential_vectors", "compute_all_B_field_vectors",
+                               "compute_all_E_field_vectors", "compute_all_B_gradient_matrices"};
+    for (int q = 0; q < 4; ++q)
+        cls.def(py_names[q],
+                [q](const Source &self, py::array points, ComputeMethod m) {
+                    return compute(self, points, q, m, names[q].c_str());
+                },
+                py::arg("point_vectors").noconvert(), py::arg("compute_method") = CPU_ST);
+    cls.def("compute_all_fields_into",
+            [](const Source &self, py::object points, py::object A, py::object B, py::object E, py::object G) {
+                into(self, points, A, B, E, G);
+            },
+            py::arg("points"), py::arg("A") = py::none(), py::arg("B") = py::none(), py::arg("E") = py::none(),
+            py::arg("G") = py::none(),
+            "Fields at the GPU precision into caller buffers (numpy arrays, or DLPack arrays such as torch tensors on\n"
+            "the CPU or the GPU; all host or all device), every requested quantity in one launch. points: (n, 3);\n"
+            "A, B, E: (n, 3); G: (n, 9) or (n, 3, 3), the gradient in the reference's convention.");
+}
+
+}  // namespace zc
 
 static void init_tensor(py::module_ &m)
 {
@@ -351,6 +534,7 @@ static void init_coil(py::module_ &m)
         .def_static("set_gpu_precision", &Coil::setGpuPrecision, py::arg("precision"))
         .def_static("get_gpu_precision", &Coil::getGpuPrecision)
         .def("__repr__", &Coil::operator std::string);
+    zc::add_methods<decltype(coil), Coil>(coil, "Coil");
 
     c.attr("GPU_FP64") = (int)CCOIL_FP64;
     c.attr("GPU_FP32C") = (int)CCOIL_FP32C;
@@ -364,8 +548,8 @@ static void init_coil(py::module_ &m)
 static void init_coil_group(py::module_ &m)
 {
     py::module_ g = m.def_submodule("coil_group");
-    py::class_<CoilGroup>(g, "CoilGroup")
-        .def(py::init<std::vector<std::shared_ptr<Coil>>, PrecisionFactor, int>(),
+    py::class_<CoilGroup> group(g, "CoilGroup");
+    group.def(py::init<std::vector<std::shared_ptr<Coil>>, PrecisionFactor, int>(),
              py::arg("member_coils") = std::vector<std::shared_ptr<Coil>>(),
              py::arg("precision_factor") = PrecisionFactor(), py::arg("thread_count") = g_defaultThreadCount)
         .def("get_default_precision_factor", &CoilGroup::getDefaultPrecisionFactor)
@@ -417,6 +601,7 @@ static void init_coil_group(py::module_ &m)
              py::arg("secondary_z_angles"), py::arg("precision_factor") = PrecisionFactor(),
              py::arg("compute_method") = CPU_ST)
         .def("__repr__", &CoilGroup::operator std::string);
+    zc::add_methods<decltype(group), CoilGroup>(group, "CoilGroup");
 }
 
 PYBIND11_MODULE(_c_coil, m)

@@ -204,3 +204,28 @@ def matrix_pairs(n):
     i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
     off = i != j
     return i[off].astype(np.int64), j[off].astype(np.int64)
+
+
+def coaxial_workload(n_pairs=4096, pf=5.0, seed=0):
+    """Coaxial THIN/RECTANGULAR pairs (every pair its own coils on the z axis, z ~ U[-1, 1] m, rotations about z only)
+    with the arguments Coil::computeMutualInductance plans for them (GPU method, z-axis case). Returns
+    (params_prim, params_sec, prim_rows, sec_rows)."""
+    c = _c_coil()
+    rng = np.random.default_rng(seed)
+
+    def draw():
+        thin = rng.random() < 0.3
+        return dict(R=rng.uniform(0.02, 0.5), a=0.0 if thin else rng.uniform(0.005, 0.2), b=rng.uniform(0.005, 0.2),
+                    N=int(rng.integers(1, 2001)), I=float(rng.uniform(0.5, 10.0) * rng.choice([-1.0, 1.0])),
+                    pos=np.array([0.0, 0.0, rng.uniform(-1.0, 1.0)]), alpha=0.0, beta=float(rng.uniform(0, 2 * np.pi)))
+    pp, sp, pr, sr = [], [], [], []
+    for _ in range(n_pairs):
+        p, s = draw(), draw()
+        cp, cs = make_coil(p, pf), make_coil(s, pf)
+        a = c.coil.CoilPairArguments.get_appropriate_coil_pair_arguments(cp, cs, c.coil.PrecisionFactor(pf),
+                                                                         c.coil.GPU, True, False)
+        pp.append(p)
+        sp.append(s)
+        pr.append(coil_row(cp, a.primary_precision))
+        sr.append(coil_row(cs, a.secondary_precision))
+    return pp, sp, pr, sr

@@ -117,6 +117,10 @@ const char *ccoil_version(void);
 /* Use an external stream (hipStream_t cast to void*); NULL restores the context's own stream. */
 ccoil_status ccoil_set_stream(ccoil_ctx ctx, void *stream);
 ccoil_status ccoil_synchronize(ccoil_ctx ctx);
+/* The stream the context's kernels are ordered on (its own, or the one given to ccoil_set_stream), as hipStream_t
+ * cast to void*. No counterpart in the reference: zero-copy callers hand it to a producer (DLPack __dlpack__(stream))
+ * so that the producer's pending writes are ordered before the context's kernels. */
+ccoil_status ccoil_get_stream(ccoil_ctx ctx, void **stream);
 
 /* Gauss-Legendre rule of order n (1..100) as generated in FP64 on the device; replaces the host table
  * src/LegendreMatrix/LegendreMatrix.h:9-10. x, w: host arrays of n doubles, ascending nodes. */

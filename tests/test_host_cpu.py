@@ -178,3 +178,30 @@ def test_abi_without_device_reports_no_device():
     assert L.ccoil_pairs(None, None, None, None, None, 0, None, None, 0, 0, 0, None, None, None, None, None) == \
         ccoil_abi.ERR_ARGUMENT
     assert L.ccoil_field_gradient(None, None, None, 0, None, 0, 0, None, None) == ccoil_abi.ERR_ARGUMENT
+
+
+def test_zero_copy_argument_checks(cc):
+    """The numpy / DLPack entry points check shapes, dtypes, writability and sides before any device work."""
+    import torch
+    C = cc.coil
+    for src in (C.Coil(0.1, 0.05, 0.08, 100), cc.coil_group.CoilGroup([C.Coil(0.1, 0.05, 0.08, 100)])):
+        with pytest.raises(ValueError, match=r"shape \(n, 3\)"):
+            src.compute_all_fields_into(np.zeros((4, 2)), B=np.zeros((4, 3)))
+        with pytest.raises(ValueError, match="as many rows"):
+            src.compute_all_fields_into(np.zeros((4, 3)), B=np.zeros((5, 3)))
+        with pytest.raises(ValueError, match="no output"):
+            src.compute_all_fields_into(np.zeros((4, 3)))
+        with pytest.raises(ValueError, match=r"\(n, 9\) or \(n, 3, 3\)"):
+            src.compute_all_fields_into(np.zeros((4, 3)), G=np.zeros((4, 3)))
+        with pytest.raises(ValueError, match="writable C-contiguous float64"):
+            src.compute_all_fields_into(np.zeros((4, 3)), B=np.zeros((4, 3), dtype=np.float32))
+        with pytest.raises(ValueError, match="writable C-contiguous float64"):
+            src.compute_all_fields_into(np.zeros((4, 3)), B=np.zeros((3, 4)).T)
+        with pytest.raises(ValueError, match="numpy array or a DLPack array"):
+            src.compute_all_fields_into([[0.0, 0.0, 0.2]], B=np.zeros((1, 3)))
+        with pytest.raises(ValueError, match="float64"):
+            src.compute_all_fields_into(torch.zeros(4, 3, dtype=torch.float32), B=np.zeros((4, 3)))
+        with pytest.raises(ValueError, match="contiguous"):
+            src.compute_all_fields_into(torch.zeros(3, 4, dtype=torch.float64).T, B=np.zeros((4, 3)))
+        with pytest.raises(RuntimeError, match="CPU"):
+            src.compute_all_B_field_vectors(np.zeros((4, 3)))
