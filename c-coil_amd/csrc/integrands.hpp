@@ -266,18 +266,18 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
     double hR0 = 0, hR1 = 0, ha0 = 0, ha1 = 0, hb0 = 0, hb1 = 0;   // adjoint row sums
 
     if (c.use_fast) {
-        const double z1d = zd + 0.5 * c.b, z2d = zd - 0.5 * c.b;
+        // The sheet's field is mirror-symmetric in z: B_rho, dB_rho/drho, dB_z/dz (and the adjoint's rho
+        // components) are odd, A, B_z and dB_rho/dz even. The rule is evaluated at |z|, so the upper edge z1 is
+        // positive and z2 > 0 exactly when the point lies beyond the sheet's axial span; odd terms flip at the end.
+        const bool neg = zd < 0.0;
+        const double zm = fabs(zd);
+        const double z1d = zm + 0.5 * c.b, z2d = zm - 0.5 * c.b;
         const T z1 = (T)z1d, z2 = (T)z2d;
         const T z1s = (T)(z1d * z1d), z2s = (T)(z2d * z2d);
-        const T tbz = (T)(2.0 * c.b * zd);
-        const bool same = z1d * z2d > 0.0;
-        // A is even in z: with |z| the upper edge is positive.
-        const double zad = fabs(zd);
-        const T za1 = (T)(zad + 0.5 * c.b), za2 = (T)(zad - 0.5 * c.b);
-        const T za1s = (T)((zad + 0.5 * c.b) * (zad + 0.5 * c.b)), za2s = (T)((zad - 0.5 * c.b) * (zad - 0.5 * c.b));
-        const T tbza = (T)(2.0 * c.b * zad);
-        const bool samea = (zad - 0.5 * c.b) > 0.0;
-        const bool zpos = zd >= 0.0;
+        const T tbz = (T)(2.0 * c.b * zm);
+        const bool same = z2d > 0.0;
+        // same ? x : 1 as one exact FMA, fma(sg, x, sg1): a 64-bit select takes two instructions
+        const T sg = same ? (T)1 : (T)0, sg1 = (T)1 - sg;
         const T bf = (T)c.b;
         for (int j = row0; j < nt; j += rstep) {
             const double rho_d = c.R + 0.5 * c.a * (1.0 + xt[j]);
@@ -304,13 +304,14 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                     const T tP2 = tbz * (K12 * K12);
                     const T g1 = z1 * K1, g2 = z2 * K2;
                     const T gs = g1 + g2;
-                    const T Y = same ? M * gs : M;
+                    const T gsel = fma_<T>(sg, gs, sg1);                // same ? gs : 1
+                    const T Y = M * gsel;
                     const T Ks = K1 + K2;
                     const T Rb = prcp<P>(Ks * Y);
                     const T S = Y * Rb, Rr = Ks * Rb;
                     const T dK = -tP2 * S;                              // K1 - K2
-                    const T invM = same ? gs * Rr : Rr;
-                    const T dgM = same ? tP2 * (M * Rr) : (g1 - g2) * Rr;   // (z1 K1 - z2 K2) / M
+                    const T invM = gsel * Rr;
+                    const T dgM = (same ? tP2 * M : g1 - g2) * Rr;      // (z1 K1 - z2 K2) / M
                     const T rmc = fma_<T>(rf, s, dm);                       // rho - r cos
                     rBh = fma_<T>(wc, -dK, rBh);       // B_rho, also the gradient's rho-phi term
                     if (WB) rBz = fma_<T>(w, rmc * dgM, rBz);
@@ -321,7 +322,9 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                         rRR = fma_<T>(wc, rcm * dL, rRR);
                         rZZ = fma_<T>(w, rmc * dL, rZZ);
                         const T dg = dgM * M;                               // z1 K1 - z2 K2
-                        const T dh = same ? fma_<T>(dg, K1s, -tP2 * g2) : (g1 * K1s - g2 * K2s);
+                        // z1 K1^3 - z2 K2^3 = dg K1^2 + z2 K2 (K1^2 - K2^2), K1^2 - K2^2 = -tP2: with z >= 0
+                        // both terms are positive when z2 < 0, and dg carries no cancellation when z2 > 0
+                        const T dh = fma_<T>(dg, K1s, -tP2 * g2);
                         const T O = fma_<T>(cp, base, -qs);                 // cos (rho^2 + r^2) - 2 rho r
                         const T N = fma_<T>(s, fma_<T>(s, fma_<T>(s, n3, n2), n1), base2);
                         const T t = fma_<T>(O, dg, N * dh);
@@ -334,23 +337,14 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                     }
                 }
                 if (WA) {
-                    // asinh(za1 / sqrt(M)) - asinh(za2 / sqrt(M)) = log(P1 / P2), P_k = za_k + sqrt(M + za_k^2),
+                    // asinh(z1 / sqrt(M)) - asinh(z2 / sqrt(M)) = log(P1 / P2), P_k = z_k + sqrt(M + z_k^2),
                     // = log1p(x) with x = (P1 - P2) / P2 = (b (sJ1 + sJ2) + 2 b |z|) / ((sJ1 + sJ2) P2) >= 0;
-                    // for za2 < 0, P2 = M / (sJ2 - za2). {za1^2, za2^2} = {z1^2, z2^2}, so the B terms' K serve.
-                    T sJa1, sJa2;
-                    if (WB || WG) {
-                        const T J1 = M + z1s, J2 = M + z2s;
-                        const T K1 = prsq<P>(J1), K2 = prsq<P>(J2);   // the same values as above (CSE)
-                        sJa1 = zpos ? J1 * K1 : J2 * K2;
-                        sJa2 = zpos ? J2 * K2 : J1 * K1;
-                    } else {
-                        const T J1 = M + za1s, J2 = M + za2s;
-                        sJa1 = J1 * prsq<P>(J1);
-                        sJa2 = J2 * prsq<P>(J2);
-                    }
+                    // for z2 < 0, P2 = M / (sJ2 - z2). A is even in z, so |z| serves directly.
+                    const T J1 = M + z1s, J2 = M + z2s;
+                    const T sJa1 = J1 * prsq<P>(J1), sJa2 = J2 * prsq<P>(J2);   // the B terms' K1, K2 (CSE)
                     const T sum = sJa1 + sJa2;
-                    const T numr = fma_<T>(bf, sum, tbza);
-                    const T x = samea ? numr * prcp<P>(sum * (za2 + sJa2)) : numr * (sJa2 - za2) * prcp<P>(sum * M);
+                    const T numr = fma_<T>(bf, sum, tbz);
+                    const T x = same ? numr * prcp<P>(sum * (z2 + sJa2)) : numr * (sJa2 - z2) * prcp<P>(sum * M);
                     rA = fma_<T>(wc, log1p_pos<P>(x), rA);
                 }
             }
@@ -364,8 +358,8 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             if constexpr (AD) {
                 const double Wr = c.cf * wt[j], hx = 0.5 * (1.0 + xt[j]);
                 const double e0 = 0.5 * (double)eR, e1 = 0.5 * (double)eZ;
-                const double g0 = rd * (double)rRR + zd * (double)rRZ + c.b * e0;   // -d f_rho / d rho
-                const double g1 = rd * (double)rRZ + zd * (double)rZZ + c.b * e1;   // -d f_z / d rho
+                const double g0 = rd * (double)rRR + zm * (double)rRZ + c.b * e0;   // -d f_rho / d rho
+                const double g1 = rd * (double)rRZ + zm * (double)rZZ + c.b * e1;   // -d f_z / d rho
                 hR0 += Wr * g0;
                 hR1 += Wr * g1;
                 ha0 += Wr * hx * g0;
@@ -374,21 +368,22 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                 hb1 += W * e1;
             }
         }
+        const double sgn = neg ? -1.0 : 1.0;   // odd terms at z < 0
         if constexpr (AD) {
-            ad->dR[0] = -hR0;
+            ad->dR[0] = -sgn * hR0;
             ad->dR[1] = -hR1;
-            ad->da[0] = -ha0;
+            ad->da[0] = -sgn * ha0;
             ad->da[1] = -ha1;
-            ad->db[0] = hb0 - aBh / c.b;
+            ad->db[0] = sgn * (hb0 - aBh / c.b);
             ad->db[1] = hb1 - aBz / c.b;
         }
         L.A = aA;
-        L.Bh = aBh;
+        L.Bh = sgn * aBh;
         L.Bz = aBz;
-        L.RP = aBh / rd;
-        L.RR = aRR;
+        L.RP = L.Bh / rd;
+        L.RR = sgn * aRR;
         L.RZ = aRZ;
-        L.ZZ = aZZ;
+        L.ZZ = sgn * aZZ;
     } else {
         for (int j = row0; j < nt; j += rstep) {
             const double rho_d = c.R + 0.5 * c.a * (1.0 + xt[j]);
