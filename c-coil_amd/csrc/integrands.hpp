@@ -313,14 +313,17 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                     const T invM = gsel * Rr;
                     const T dgM = (same ? tP2 * M : g1 - g2) * Rr;      // (z1 K1 - z2 K2) / M
                     const T rmc = fma_<T>(rf, s, dm);                       // rho - r cos
+                    const T wr = w * rmc;                                   // shared by B_z and dB_z/dz
                     rBh = fma_<T>(wc, -dK, rBh);       // B_rho, also the gradient's rho-phi term
-                    if (WB) rBz = fma_<T>(w, rmc * dgM, rBz);
+                    if (WB) rBz = fma_<T>(wr, dgM, rBz);
                     if (WG) {
-                        const T K1s = K1 * K1, K2s = K2 * K2;
-                        const T dL = dK * (K1s + K12 + K2s);                 // K1^3 - K2^3
+                        const T K1s = K1 * K1;
+                        // K1^3 - K2^3 = (K1 - K2)(K1^2 + K1 K2 + K2^2), the second factor as (K1 + K2)^2 - K1 K2
+                        // (the subtraction removes at most a quarter: no cancellation)
+                        const T dL = dK * fma_<T>(Ks, Ks, -K12);
                         const T rcm = fma_<T>(rho, s, -dm);                 // r - rho cos
                         rRR = fma_<T>(wc, rcm * dL, rRR);
-                        rZZ = fma_<T>(w, rmc * dL, rZZ);
+                        rZZ = fma_<T>(wr, dL, rZZ);
                         const T dg = dgM * M;                               // z1 K1 - z2 K2
                         // z1 K1^3 - z2 K2^3 = dg K1^2 + z2 K2 (K1^2 - K2^2), K1^2 - K2^2 = -tP2: with z >= 0
                         // both terms are positive when z2 < 0, and dg carries no cancellation when z2 > 0
@@ -330,7 +333,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                         const T t = fma_<T>(O, dg, N * dh);
                         rRZ = fma_<T>(w, t * (invM * invM), rRZ);
                         if constexpr (AD) {
-                            const T K13 = K1s * K1, K23 = K2s * K2;
+                            const T K13 = K1s * K1, K23 = K2 * K2 * K2;
                             eR = fma_<T>(wc, fma_<T>(z1, K13, z2 * K23), eR);
                             eZ = fma_<T>(w, rmc * (K13 + K23), eZ);
                         }
@@ -406,13 +409,14 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                     const T K2 = K * K;
                     const T K3 = K2 * K;
                     const T rmc = fma_<T>(rf, s, dm);     // rho - r cos
+                    const T wr = w * rmc;                 // shared by B_z and dB_z/dz
                     rBh = fma_<T>(wc, K3, rBh);        // B_rho, also the gradient's rho-phi term
-                    if (WB) rBz = fma_<T>(w, rmc * K3, rBz);
+                    if (WB) rBz = fma_<T>(wr, K3, rBz);
                     if (WG) {
                         const T K5 = K3 * K2;
                         const T rcm = fma_<T>(rho, s, -dm); // r - rho cos
                         rRR = fma_<T>(wc, rcm * K5, rRR);
-                        rZZ = fma_<T>(w, rmc * K5, rZZ);
+                        rZZ = fma_<T>(wr, K5, rZZ);
                         const T poly = k0 - s * fma_<T>(k2, s, k1);
                         rRZ = fma_<T>(w, poly * K5, rRZ);
                     }
