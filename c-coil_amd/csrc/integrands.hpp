@@ -148,11 +148,13 @@ __device__ inline void eval_plain(const CoilDev &c, const Tables &tab, double zd
 // Cancellation-free integrands (the same discretisation as eval_plain, algebraically rearranged).
 // Notation: rho = source radius of the row, d = rho - r, base = d^2, q = 2 rho r,
 // s = 1 - cos(phi) (table), M = base + q s (in-plane distance squared: a sum of non-negative terms),
-// J_k = M + z_k^2 with z_1 = z + b/2, z_2 = z - b/2, K_k = J_k^-1/2.
+// J_k = M + z_k^2 with z_1 = z + b/2, z_2 = z - b/2, K_k = J_k^-1/2. The fast rule is evaluated at z >= 0 (the
+// sheet is mirror-symmetric; odd terms flip sign afterwards), so z1 > 0 and z2 < 0 exactly inside the axial span.
 //   K1 - K2       = -2 b z (K1 K2)^2 / (K1 + K2)                  (exact identity, no cancellation)
-//   z1 K1 - z2 K2 = 2 b z M (K1 K2)^2 / (z1 K1 + z2 K2)            when z1 z2 > 0
-//                 = |z1| K1 + |z2| K2 (direct)                      otherwise
-//   z1 K1^3 - z2 K2^3 = (z1 K1 - z2 K2) K1^2 - 2 b z z2 K2 (K1 K2)^2  when z1 z2 > 0, direct otherwise
+//   z1 K1 - z2 K2 = 2 b z M (K1 K2)^2 / (z1 K1 + z2 K2)            when z2 > 0
+//                 = z1 K1 + |z2| K2 (direct)                        otherwise
+//   z1 K1^3 - z2 K2^3 = (z1 K1 - z2 K2) K1^2 - 2 b z z2 K2 (K1 K2)^2  (both terms non-negative when z2 < 0)
+//   K1^3 - K2^3   = (K1 - K2) ((K1 + K2)^2 - K1 K2)
 //   N (gradient RZ term) as a polynomial in s with coefficients in base and q.
 // One reciprocal serves 1/(K1 + K2), 1/M and 1/(z1 K1 + z2 K2): with Y = M (z1 K1 + z2 K2) (or M) and
 // Rb = 1 / ((K1 + K2) Y), 1/(K1 + K2) = Y Rb and 1/Y = (K1 + K2) Rb. The angular loops are unrolled by 4 so
@@ -302,8 +304,8 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                     const T K1 = prsq<P>(J1), K2 = prsq<P>(J2);
                     const T K12 = K1 * K2;
                     const T tP2 = tbz * (K12 * K12);
-                    const T g1 = z1 * K1, g2 = z2 * K2;
-                    const T gs = g1 + g2;
+                    const T g2 = z2 * K2;
+                    const T gs = fma_<T>(z1, K1, g2);                   // z1 K1 + z2 K2
                     const T gsel = fma_<T>(sg, gs, sg1);                // same ? gs : 1
                     const T Y = M * gsel;
                     const T Ks = K1 + K2;
@@ -311,7 +313,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                     const T S = Y * Rb, Rr = Ks * Rb;
                     const T dK = -tP2 * S;                              // K1 - K2
                     const T invM = gsel * Rr;
-                    const T dgM = (same ? tP2 * M : g1 - g2) * Rr;      // (z1 K1 - z2 K2) / M
+                    const T dgM = (same ? tP2 * M : fma_<T>(z1, K1, -g2)) * Rr;   // (z1 K1 - z2 K2) / M
                     const T rmc = fma_<T>(rf, s, dm);                       // rho - r cos
                     const T wr = w * rmc;                                   // shared by B_z and dB_z/dz
                     rBh = fma_<T>(wc, -dK, rBh);       // B_rho, also the gradient's rho-phi term
@@ -324,14 +326,15 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                         const T rcm = fma_<T>(rho, s, -dm);                 // r - rho cos
                         rRR = fma_<T>(wc, rcm * dL, rRR);
                         rZZ = fma_<T>(wr, dL, rZZ);
-                        const T dg = dgM * M;                               // z1 K1 - z2 K2
-                        // z1 K1^3 - z2 K2^3 = dg K1^2 + z2 K2 (K1^2 - K2^2), K1^2 - K2^2 = -tP2: with z >= 0
-                        // both terms are positive when z2 < 0, and dg carries no cancellation when z2 > 0
-                        const T dh = fma_<T>(dg, K1s, -tP2 * g2);
+                        // hM = (z1 K1^3 - z2 K2^3) / M = dgM K1^2 + z2 K2 (K1^2 - K2^2) / M, K1^2 - K2^2 = -tP2:
+                        // with z >= 0 both terms are positive when z2 < 0, and dgM carries no cancellation when
+                        // z2 > 0
+                        const T hM = fma_<T>(dgM, K1s, -(tP2 * g2) * invM);
                         const T O = fma_<T>(cp, base, -qs);                 // cos (rho^2 + r^2) - 2 rho r
                         const T N = fma_<T>(s, fma_<T>(s, fma_<T>(s, n3, n2), n1), base2);
-                        const T t = fma_<T>(O, dg, N * dh);
-                        rRZ = fma_<T>(w, t * (invM * invM), rRZ);
+                        // the rho-z term (O (z1 K1 - z2 K2) + N (z1 K1^3 - z2 K2^3)) / M^2
+                        const T u = fma_<T>(O, dgM, N * hM);
+                        rRZ = fma_<T>(w, u * invM, rRZ);
                         if constexpr (AD) {
                             const T K13 = K1s * K1, K23 = K2 * K2 * K2;
                             eR = fma_<T>(wc, fma_<T>(z1, K13, z2 * K23), eR);
