@@ -50,6 +50,8 @@ struct Tables {
     const double *rsin;   // sin(pi (1 + x))
     const double *wcos;   // w cos(pi/2 (1 + x)): the angular weight times the cosine
     const float *wcosf;
+    const double *opc;    // 1 + cos(pi/2 (1 + x)) = 2 cos^2(pi/4 (1 + x))
+    const float *opcf;
 };
 
 // Quantity bits (mirror of the C ABI enum).

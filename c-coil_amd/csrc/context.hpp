@@ -36,6 +36,8 @@ struct ccoil_ctx_s {
     float *gl_xf = nullptr, *gl_wf = nullptr, *gl_cosf = nullptr, *gl_omcf = nullptr;
     double *gl_rcos = nullptr, *gl_rsin = nullptr, *gl_wcos = nullptr;
     float *gl_wcosf = nullptr;
+    double *gl_opc = nullptr;
+    float *gl_opcf = nullptr;
     ccoil::DevBuf coils, pts, outA, outB, outE, outG, queue, counter, fpartial;
     ccoil::DevBuf pprim, psec, ppairs, ppartial, pcounter, pM, pF, pT, pdM;
     ccoil::DevBuf acoils, apts, agB, apartial, agrad;   // field adjoint

@@ -208,17 +208,25 @@ void launch_gl_tables(hipStream_t s, double *x, double *w, double *cosv, double 
                        rcos, rsin);
 }
 
-__global__ void wcos_table_kernel(const double *w, const double *cosv, double *wc, float *wcf)
+// Derived angular tables: w cos(pi/2 (1 + x)) and 1 + cos(pi/2 (1 + x)) = 2 cos^2(pi/4 (1 + x)) (no cancellation
+// near phi = pi).
+__global__ void angular_table_kernel(const double *x, const double *w, const double *cosv, double *wc, float *wcf,
+                                     double *opc, float *opcf)
 {
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= kTableSize) return;
     wc[e] = w[e] * cosv[e];
     wcf[e] = (float)wc[e];
+    const double c = cospi(0.25 * (1.0 + x[e]));
+    opc[e] = 2.0 * c * c;
+    opcf[e] = (float)opc[e];
 }
 
-void launch_wcos_table(hipStream_t s, const double *w, const double *cosv, double *wc, float *wcf)
+void launch_angular_tables(hipStream_t s, const double *x, const double *w, const double *cosv, double *wc,
+                           float *wcf, double *opc, float *opcf)
 {
-    hipLaunchKernelGGL(wcos_table_kernel, dim3((kTableSize + 255) / 256), dim3(256), 0, s, w, cosv, wc, wcf);
+    hipLaunchKernelGGL(angular_table_kernel, dim3((kTableSize + 255) / 256), dim3(256), 0, s, x, w, cosv, wc, wcf,
+                       opc, opcf);
 }
 
 __global__ void fill_zero_kernel(unsigned long long *p, int64_t n)

@@ -236,6 +236,10 @@ template <typename T> __device__ inline const T *tab_wcos(const Tables &t);
 template <> __device__ inline const double *tab_wcos<double>(const Tables &t) { return t.wcos; }
 template <> __device__ inline const float *tab_wcos<float>(const Tables &t) { return t.wcosf; }
 
+template <typename T> __device__ inline const T *tab_opc(const Tables &t);
+template <> __device__ inline const double *tab_opc<double>(const Tables &t) { return t.opc; }
+template <> __device__ inline const float *tab_opc<float>(const Tables &t) { return t.opcf; }
+
 template <typename T> __device__ inline const T *tab_omc(const Tables &t);
 template <> __device__ inline const double *tab_omc<double>(const Tables &t) { return t.omc; }
 template <> __device__ inline const float *tab_omc<float>(const Tables &t) { return t.omcf; }
@@ -259,10 +263,10 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
     using T = real_t<P>;
     const double *xt = tab.x + table_offset(c.n_thk);
     const double *wt = tab.w + table_offset(c.n_thk);
-    const T *ct = tab_cos<T>(tab) + table_offset(c.n_ang);
     const T *st = tab_omc<T>(tab) + table_offset(c.n_ang);
     const T *wa = tab_w<T>(tab) + table_offset(c.n_ang);
     const T *wct = tab_wcos<T>(tab) + table_offset(c.n_ang);   // w cos
+    const T *opt = tab_opc<T>(tab) + table_offset(c.n_ang);    // 1 + cos
     const int na = c.n_ang, nt = c.n_thk;
     double aA = 0, aBh = 0, aBz = 0, aRR = 0, aRZ = 0, aZZ = 0;
     double hR0 = 0, hR1 = 0, ha0 = 0, ha1 = 0, hb0 = 0, hb1 = 0;   // adjoint row sums
@@ -296,9 +300,8 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             T eR = 0, eZ = 0;   // adjoint: end rings, sum w cos (z1 K1^3 + z2 K2^3), sum w (rho - r cos)(K1^3 + K2^3)
 #pragma unroll 4
             for (int i = 0; i < na; ++i) {
-                const T cp = ct[i], s = st[i], w = wa[i], wc = wct[i];
-                const T qs = q * s;
-                const T M = base + qs;
+                const T s = st[i], w = wa[i], wc = wct[i];
+                const T M = fma_<T>(q, s, base);
                 if (WB || WG) {
                     const T J1 = M + z1s, J2 = M + z2s;
                     const T K1 = prsq<P>(J1), K2 = prsq<P>(J2);
@@ -330,7 +333,8 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                         // with z >= 0 both terms are positive when z2 < 0, and dgM carries no cancellation when
                         // z2 > 0
                         const T hM = fma_<T>(dgM, K1s, -(tP2 * g2) * invM);
-                        const T O = fma_<T>(cp, base, -qs);                 // cos (rho^2 + r^2) - 2 rho r
+                        // cos (rho^2 + r^2) - 2 rho r = cos base - q s = (1 + cos) base - M
+                        const T O = fma_<T>(opt[i], base, -M);
                         const T N = fma_<T>(s, fma_<T>(s, fma_<T>(s, n3, n2), n1), base2);
                         // the rho-z term (O (z1 K1 - z2 K2) + N (z1 K1^3 - z2 K2^3)) / M^2
                         const T u = fma_<T>(O, dgM, N * hM);
