@@ -201,14 +201,25 @@ template <> __device__ inline double prcp<f64s>(double x)
     return fma(y, e, y);                                   // y (1 + e)
 }
 
-// log(1 + x) for x >= 0: 2 atanh(u), u = x / (2 + x), by its odd series while u <= 0.2 (x <= 0.5), else log.
-// The series is exact to the type's precision there (u^21 / 21 < 1e-16 in double, u^11 / 11 < 1e-8 in float) and
-// keeps full relative accuracy for small x, where log(1 + x) would lose it.
+template <typename T> __device__ inline T frexp_(T x, int *e);
+template <> __device__ inline double frexp_<double>(double x, int *e) { return frexp(x, e); }
+template <> __device__ inline float frexp_<float>(float x, int *e) { return frexpf(x, e); }
+
+// log(1 + x) for x >= 0 in one branch-free instruction stream (the lanes of a wave take different x):
+// 1 + x = m 2^e with m in [sqrt(1/2), sqrt(2)), log(1 + x) = e ln 2 + 2 atanh(u), u = (m - 1) / (m + 1), |u| <= 0.1716,
+// by the odd series of atanh (u^21 / 21 < 1e-17 relative in double, u^11 / 11 < 3e-9 in float). For e = 0 the
+// numerator is x itself (exact), so small x keep full relative accuracy; for e != 0, m - 1 is exact (Sterbenz) and
+// log(1 + x) >= 0.34 bounds the effect of rounding 1 + x.
 template <typename P> __device__ inline real_t<P> log1p_pos(real_t<P> x)
 {
     using T = real_t<P>;
-    if (x > (T)0.5) return log_<T>((T)1 + x);
-    const T u = x * prcp<P>((T)2 + x), u2 = u * u;
+    int e;
+    T m = frexp_<T>((T)1 + x, &e);                       // m in [1/2, 1)
+    const bool lo = m < (T)0.70710678118654752440;
+    m = lo ? m + m : m;
+    e = lo ? e - 1 : e;
+    const T num = e == 0 ? x : m - (T)1;
+    const T u = num * prcp<P>(m + (T)1), u2 = u * u;
     T s;
     if constexpr (sizeof(T) == 8) {
         s = (T)(1.0 / 21);
@@ -229,7 +240,7 @@ template <typename P> __device__ inline real_t<P> log1p_pos(real_t<P> x)
         s = fma_<T>(s, u2, (T)(1.0 / 3));
     }
     s = fma_<T>(s, u2, (T)1);
-    return (T)2 * u * s;
+    return fma_<T>((T)e, (T)0.69314718055994530942, (T)2 * u * s);
 }
 
 template <typename T> __device__ inline const T *tab_wcos(const Tables &t);

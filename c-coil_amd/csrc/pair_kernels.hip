@@ -67,7 +67,7 @@ __device__ inline void grad_m_term(const CoilDev &pc, const Local &L, double r, 
 }
 
 template <int PREC, bool WM, bool WF, bool WD>
-__global__ __launch_bounds__(64) void pair_kernel(PairParams p)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pair_kernel(PairParams p)
 {
     const int lane = threadIdx.x;
     const int64_t item = blockIdx.x;
