@@ -227,3 +227,52 @@ def test_dropin_group_mixed_members(cc):
     assert abs(M - m_ref) <= 1e-9 * abs(m_ref), (M, m_ref)
     assert np.linalg.norm(np.array([F.x, F.y, F.z]) - f_ref) <= 1e-9 * np.linalg.norm(f_ref)
     assert np.linalg.norm(np.array([T.x, T.y, T.z]) - t_ref) <= 1e-9 * np.linalg.norm(t_ref)
+
+
+def test_dropin_reference_zaxis_comparison_workload(cc):
+    """The reference's own z-axis comparison (Compare::mutualInductanceZAxis, CompMInductanceZAxis.cxx:16-53): the
+    19 coaxial geometries of data/values_MInductance_zAxis.txt at PrecisionFactor 1..8 with the CPU_ST arguments the
+    reference plans, here computed on the GPU, against the oracle's restatement of the same rule."""
+    import os
+    C, V = cc.coil, cc.tensor.Vector3
+    rows = np.loadtxt(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                   "values_MInductance_zAxis.txt"))
+    for R1, a1, b1, N1, R2, a2, b2, N2, d in rows:
+        prim = C.Coil(R1, a1, b1, int(N1))
+        sec = C.Coil(R2, a2, b2, int(N2))
+        sec.set_position_and_orientation(V(0.0, 0.0, d), 0.0, 0.0)
+        op, os_ = O.make_coil(R1, a1, b1, int(N1)), O.make_coil(R2, a2, b2, int(N2), pos=(0.0, 0.0, d))
+        fast = op.use_fast and os_.use_fast
+        for pf in range(1, 9):
+            args = C.CoilPairArguments.get_appropriate_coil_pair_arguments(prim, sec, C.PrecisionFactor(pf), C.CPU_ST,
+                                                                          True, False)
+            M = C.Coil.compute_mutual_inductance(prim, sec, args, C.GPU)
+            a = O.pair_precision_args(op, os_, float(pf), False, True, False)
+            if fast:
+                assert _close(M, O.mutual_inductance_zaxis_fast(op, os_, a),
+                              O.mutual_inductance_zaxis_fast(op, os_, a, True)), (R1, R2, d, pf)
+            else:
+                fa = np.array([1, 1, 1, a[3], a[4], a[5]], dtype=np.int32)
+                m = O.mutual_inductance_zaxis_slow(op, os_, a, fa, extended=True)
+                assert abs(M - m) <= 1e-11 * abs(m), (R1, R2, d, pf)
+
+
+def test_dropin_reference_self_inductance_workload(cc):
+    """The reference's self-inductance comparison (Compare::selfInductance, CompMInductanceZAxis.cxx:56-140): a
+    RECTANGULAR, a THIN and a FLAT coil and Grover's (alpha, beta) thick coils at several precision factors."""
+    C = cc.coil
+    r1 = 0.1
+    dims = [(0.03, 0.03, 0.12, 3600), (0.03, 0.0, 0.12, 120), (0.03, 0.03, 0.0, 30)]
+    dims += [(r1, r1 * (al - 1), 2 * r1 * be, 1) for al, be in ((1.5, 0.25), (3.0, 1.0), (4.0, 3.0), (7.0, 6.0), (9.0, 4.0))]
+    for R, a, b, N in dims:
+        coil = C.Coil(R, a, b, N)
+        oc = O.make_coil(R, a, b, N)
+        for pf in (2.0, 5.0, 8.0):
+            L = coil.compute_and_set_self_inductance(C.PrecisionFactor(pf))
+            args = O.pair_precision_args(oc, oc, pf, False, True, False)
+            if oc.type == O.FLAT:
+                fa = np.array([1, 1, 1, args[3], args[4], args[5]], dtype=np.int32)
+                ref = O.mutual_inductance_zaxis_slow(oc, oc, args, fa, extended=True)
+                assert abs(L - ref) <= 1e-10 * abs(ref), (R, a, b, pf)
+            else:
+                assert _close(L, O.self_inductance(oc, args), O.self_inductance(oc, args, True)), (R, a, b, pf)

@@ -144,3 +144,12 @@ def test_mirror_symmetry_and_attraction():
     assert abs(O.force_zaxis_fast(p, s_m, za, extended=True) + f) <= 1e-12 * abs(f)
     # attraction of co-directed currents: the force on a secondary above the primary points down
     assert f < 0 < m
+
+
+def test_self_inductance_brooks_coil():
+    """The Brooks coil (inner radius = thickness = length = c) has L = 2.5491e-6 c N^2 H (Grover's tables, five
+    significant figures): an independent pin of the thick-coil rule, which the Nagaoka test does not reach."""
+    c = O.make_coil(0.1, 0.1, 0.1, 1, 1.0)
+    for pf in (7.0, 9.0):
+        a = O.pair_precision_args(c, c, pf, False, True, False)
+        assert abs(O.self_inductance(c, a) / (2.5491e-6 * 0.1) - 1) < 3e-5
