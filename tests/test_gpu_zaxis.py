@@ -276,3 +276,37 @@ def test_dropin_reference_self_inductance_workload(cc):
                 assert abs(L - ref) <= 1e-10 * abs(ref), (R, a, b, pf)
             else:
                 assert _close(L, O.self_inductance(oc, args), O.self_inductance(oc, args, True)), (R, a, b, pf)
+
+
+def test_dropin_reference_zaxis_force_workloads(cc):
+    """The reference's coaxial force comparisons (CompForce.cxx: forceTorqueFilamentsZAxis :8-20,
+    forceTorqueThinCoilsZAxis :102-150, forceTorqueZAxis :325-337 on 25 of its 500 positions) with the arguments the
+    reference plans for CPU_ST / CPU_MT, computed on the GPU, against the oracle."""
+    C, V = cc.coil, cc.tensor.Vector3
+    cases = [((0.5, 0.0, 0.0, 1, 100.0), (0.3, 0.0, 0.0, 1, 200.0), [0.4], range(1, 9)),
+             ((0.5, 0.0, 0.3, 200, 1.0), (0.5, 0.0, 0.2, 100, 1.0), [0.4], range(1, 11)),
+             ((0.1, 0.0, 0.15, 225, 1.0), (0.09, 0.0, 0.15, 300, 1.0), [0.05], range(1, 13)),
+             ((0.25, 0.0, 0.06, 40, 1.0), (0.2, 0.0, 0.04, 120, 1.0), [0.3], range(1, 11)),
+             ((0.0762, 0.0832, 0.0, 516, 1.42), (0.0762, 0.0832, 0.0, 516, 1.42), [0.0468], range(1, 9)),
+             ((0.03, 0.03, 0.12, 3600, 1.0), (0.02, 0.025, 0.04, 1000, 1.0),
+              [0.08 + i * 0.001 for i in range(0, 500, 20)], [8])]
+    for pdims, sdims, zs, pfs in cases:
+        prim = C.Coil(*pdims)
+        op = O.make_coil(*pdims)
+        for z in zs:
+            sec = C.Coil(*sdims)
+            sec.set_position_and_orientation(V(0.0, 0.0, z), 0.0, 0.0)
+            os_ = O.make_coil(*sdims, pos=(0.0, 0.0, z))
+            for pf in pfs:
+                args = C.CoilPairArguments.get_appropriate_coil_pair_arguments(prim, sec, C.PrecisionFactor(pf),
+                                                                              C.CPU_ST, True, False)
+                F, T = C.Coil.compute_ampere_force(prim, sec, args, C.GPU)
+                assert F.x == 0.0 and F.y == 0.0 and (T.x, T.y, T.z) == (0.0, 0.0, 0.0)
+                a = O.pair_precision_args(op, os_, float(pf), False, True, False)
+                if op.use_fast and os_.use_fast:
+                    assert _close(F.z, O.force_zaxis_fast(op, os_, a), O.force_zaxis_fast(op, os_, a, True)), \
+                        (pdims, z, pf)
+                else:
+                    fa = np.array([1, 1, 1, a[3], a[4], a[5]], dtype=np.int32)
+                    f = O.force_zaxis_slow(op, os_, a, fa, extended=True)
+                    assert abs(F.z - f) <= 1e-11 * abs(f), (pdims, z, pf, F.z, f)

@@ -153,3 +153,17 @@ def test_self_inductance_brooks_coil():
     for pf in (7.0, 9.0):
         a = O.pair_precision_args(c, c, pf, False, True, False)
         assert abs(O.self_inductance(c, a) / (2.5491e-6 * 0.1) - 1) < 3e-5
+
+
+@pytest.mark.parametrize("R1,R2,d", [(0.5, 0.3, 0.4), (0.1, 0.07, 0.05), (0.2, 0.25, -0.1)])
+def test_slow_force_filaments_maxwell_derivative(R1, R2, d):
+    """Axial force between coaxial loops = I1 I2 dM/dd of Maxwell's closed form (central differences of the
+    elliptic-integral formula), the reference's own filament comparison (CompForce.cxx:8-20) at other sizes."""
+    I1, I2 = 100.0, 200.0
+    p = O.make_coil(R1, 0.0, 0.0, 1, I1)
+    s = O.make_coil(R2, 0.0, 0.0, 1, I2, pos=(0, 0, d))
+    a = O.pair_precision_args(p, s, 5.0, True, True, False)
+    fa = np.array([1, 1, 1, a[3], a[4], a[5]], dtype=np.int32)
+    h = 1e-6 * abs(d)
+    ref = (_maxwell(R1, R2, d + h, 1, 1) - _maxwell(R1, R2, d - h, 1, 1)) / (2 * h) * I1 * I2
+    assert abs(O.force_zaxis_slow(p, s, a, fa, extended=True) / ref - 1) < 1e-8
