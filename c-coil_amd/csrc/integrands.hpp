@@ -418,28 +418,61 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             const T k1 = (T)(2.0 * dd * dd + 2.0 * rho_d * rd - zz);
             const T k2 = (T)(rho_d * rd);
             T rBh = 0, rBz = 0, rRR = 0, rRZ = 0, rZZ = 0, rA = 0;
+            if constexpr (sizeof(T) == 8) {
+                // Double: the angular sums as moments in s = 1 - cos of positive terms, b_k = sum w s^k K^3 and
+                // g_k = sum w s^k K^5, combined once per row: (1 - s)(rho s - d), d + r s and the rho-z polynomial
+                // are at most quadratic in s. The combinations cancel no more than the per-node sums they replace
+                // (the cos-weighted first harmonic), and five products per node are saved.
+                T b0 = 0, b1 = 0, g0 = 0, g1 = 0, g2 = 0;
 #pragma unroll 4
-            for (int i = 0; i < na; ++i) {
-                const T s = st[i], w = wa[i], wc = wct[i];
-                const T J = fma_<T>(q, s, baseJ);
-                const T K = prsq<P>(J);
-                if (WB || WG) {
-                    const T K2 = K * K;
-                    const T K3 = K2 * K;
-                    const T rmc = fma_<T>(rf, s, dm);     // rho - r cos
-                    const T wr = w * rmc;                 // shared by B_z and dB_z/dz
-                    rBh = fma_<T>(wc, K3, rBh);        // B_rho, also the gradient's rho-phi term
-                    if (WB) rBz = fma_<T>(wr, K3, rBz);
-                    if (WG) {
-                        const T K5 = K3 * K2;
-                        const T rcm = fma_<T>(rho, s, -dm); // r - rho cos
-                        rRR = fma_<T>(wc, rcm * K5, rRR);
-                        rZZ = fma_<T>(wr, K5, rZZ);
-                        const T poly = k0 - s * fma_<T>(k2, s, k1);
-                        rRZ = fma_<T>(w, poly * K5, rRZ);
+                for (int i = 0; i < na; ++i) {
+                    const T s = st[i], w = wa[i];
+                    const T J = fma_<T>(q, s, baseJ);
+                    const T K = prsq<P>(J);
+                    if (WB || WG) {
+                        const T K2 = K * K;
+                        const T wK3 = w * (K2 * K);
+                        b0 += wK3;
+                        b1 = fma_<T>(wK3, s, b1);
+                        if (WG) {
+                            const T wK5 = wK3 * K2;
+                            const T t1 = wK5 * s;
+                            g0 += wK5;
+                            g1 += t1;
+                            g2 = fma_<T>(t1, s, g2);
+                        }
                     }
+                    if (WA) rA = fma_<T>(wct[i], K, rA);
                 }
-                if (WA) rA = fma_<T>(wc, K, rA);
+                rBh = b0 - b1;                                  // sum w cos K^3
+                rBz = fma_<T>(dm, b0, rf * b1);                 // sum w (rho - r cos) K^3
+                rRR = fma_<T>(-dm, g0, fma_<T>(rho + dm, g1, -rho * g2));   // sum w cos (r - rho cos) K^5
+                rZZ = fma_<T>(dm, g0, rf * g1);                 // sum w (rho - r cos) K^5
+                rRZ = fma_<T>(k0, g0, -fma_<T>(k1, g1, k2 * g2));  // sum w (k0 - k1 s - k2 s^2) K^5
+            } else {
+#pragma unroll 4
+                for (int i = 0; i < na; ++i) {
+                    const T s = st[i], w = wa[i], wc = wct[i];
+                    const T J = fma_<T>(q, s, baseJ);
+                    const T K = prsq<P>(J);
+                    if (WB || WG) {
+                        const T K2 = K * K;
+                        const T K3 = K2 * K;
+                        const T rmc = fma_<T>(rf, s, dm);     // rho - r cos
+                        const T wr = w * rmc;                 // shared by B_z and dB_z/dz
+                        rBh = fma_<T>(wc, K3, rBh);        // B_rho, also the gradient's rho-phi term
+                        if (WB) rBz = fma_<T>(wr, K3, rBz);
+                        if (WG) {
+                            const T K5 = K3 * K2;
+                            const T rcm = fma_<T>(rho, s, -dm); // r - rho cos
+                            rRR = fma_<T>(wc, rcm * K5, rRR);
+                            rZZ = fma_<T>(wr, K5, rZZ);
+                            const T poly = k0 - s * fma_<T>(k2, s, k1);
+                            rRZ = fma_<T>(w, poly * K5, rRZ);
+                        }
+                    }
+                    if (WA) rA = fma_<T>(wc, K, rA);
+                }
             }
             const double W = c.cf * wt[j] * rho_d;
             aBh += W * (double)rBh;
