@@ -313,9 +313,21 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             for (int i = 0; i < na; ++i) {
                 const T s = st[i], w = wa[i], wc = wct[i];
                 const T M = fma_<T>(q, s, base);
+                const T J1 = M + z1s, J2 = M + z2s;
+                const T K1 = prsq<P>(J1), K2 = prsq<P>(J2);
+                // A: asinh(z1 / sqrt(M)) - asinh(z2 / sqrt(M)) = log(P1 / P2), P_k = z_k + sqrt(M + z_k^2),
+                // = log1p(x) with x = (P1 - P2) / P2 = (b (sJ1 + sJ2) + 2 b |z|) / ((sJ1 + sJ2) P2) >= 0;
+                // for z2 < 0, P2 = M / (sJ2 - z2). A is even in z, so |z| serves directly. x = numr * nsel / Da.
+                T numr = 0, nsel = 0, Da = 1;
+                if (WA) {
+                    const T sJa1 = J1 * K1, sJa2 = J2 * K2;
+                    const T sum = sJa1 + sJa2;
+                    numr = fma_<T>(bf, sum, tbz);
+                    nsel = fma_<T>(sg1, sJa2 - z2, sg);                    // same ? 1 : sJ2 - z2
+                    Da = sum * (same ? z2 + sJa2 : M);
+                }
+                T Ra = 0;   // 1 / Da
                 if (WB || WG) {
-                    const T J1 = M + z1s, J2 = M + z2s;
-                    const T K1 = prsq<P>(J1), K2 = prsq<P>(J2);
                     const T K12 = K1 * K2;
                     const T tP2 = tbz * (K12 * K12);
                     const T g2 = z2 * K2;
@@ -323,7 +335,11 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                     const T gsel = fma_<T>(sg, gs, sg1);                // same ? gs : 1
                     const T Y = M * gsel;
                     const T Ks = K1 + K2;
-                    const T Rb = prcp<P>(Ks * Y);
+                    const T KsY = Ks * Y;
+                    // one reciprocal for 1 / (K1 + K2), 1 / Y and, with A, 1 / Da
+                    const T R = prcp<P>(WA ? KsY * Da : KsY);
+                    const T Rb = WA ? Da * R : R;
+                    if (WA) Ra = KsY * R;
                     const T S = Y * Rb, Rr = Ks * Rb;
                     const T dK = -tP2 * S;                              // K1 - K2
                     const T invM = gsel * Rr;
@@ -356,18 +372,10 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                             eZ = fma_<T>(w, rmc * (K13 + K23), eZ);
                         }
                     }
+                } else if (WA) {
+                    Ra = prcp<P>(Da);
                 }
-                if (WA) {
-                    // asinh(z1 / sqrt(M)) - asinh(z2 / sqrt(M)) = log(P1 / P2), P_k = z_k + sqrt(M + z_k^2),
-                    // = log1p(x) with x = (P1 - P2) / P2 = (b (sJ1 + sJ2) + 2 b |z|) / ((sJ1 + sJ2) P2) >= 0;
-                    // for z2 < 0, P2 = M / (sJ2 - z2). A is even in z, so |z| serves directly.
-                    const T J1 = M + z1s, J2 = M + z2s;
-                    const T sJa1 = J1 * prsq<P>(J1), sJa2 = J2 * prsq<P>(J2);   // the B terms' K1, K2 (CSE)
-                    const T sum = sJa1 + sJa2;
-                    const T numr = fma_<T>(bf, sum, tbz);
-                    const T x = same ? numr * prcp<P>(sum * (z2 + sJa2)) : numr * (sJa2 - z2) * prcp<P>(sum * M);
-                    rA = fma_<T>(wc, log1p_pos<P>(x), rA);
-                }
+                if (WA) rA = fma_<T>(wc, log1p_pos<P>(numr * nsel * Ra), rA);
             }
             const double W = c.cf * wt[j] * rho_d;
             aBh += W * (double)rBh;
