@@ -235,7 +235,7 @@ ccoil_status ccoil_destroy(ccoil_ctx ctx)
     for (auto p : fp)
         if (p) (void)hipFree(p);
     DevBuf *bufs[] = {&ctx->coils, &ctx->fpartial, &ctx->pts, &ctx->outA, &ctx->outB, &ctx->outE, &ctx->outG,
-                      &ctx->queue, &ctx->counter, &ctx->pprim, &ctx->psec, &ctx->ppairs, &ctx->ppartial,
+                      &ctx->ncounts, &ctx->noffsets, &ctx->nentries, &ctx->nvals, &ctx->ntemp, &ctx->pprim, &ctx->psec, &ctx->ppairs, &ctx->ppartial,
                       &ctx->pcounter, &ctx->pM, &ctx->pF, &ctx->pT, &ctx->pdM, &ctx->acoils, &ctx->apts, &ctx->agB,
                       &ctx->apartial, &ctx->agrad, &ctx->zitems, &ctx->zpartial, &ctx->zout};
     for (auto b : bufs) free_buf(*b);
@@ -391,23 +391,9 @@ ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, const double 
     CC_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (near_fp64) {
         p.near_fp64 = 1;
-        int64_t cap = n * 4 + 4096;
-        for (int attempt = 0; attempt < 2; ++attempt) {
-            CC_HIP(ctx, ensure(ctx->queue, sizeof(int2) * (size_t)cap));
-            CC_HIP(ctx, ensure(ctx->counter, sizeof(unsigned long long)));
-            p.near_queue = (int2 *)ctx->queue.p;
-            p.near_count = (unsigned long long *)ctx->counter.p;
-            p.near_capacity = cap;
-            launch_fill_zero(ctx->stream, p.near_count, 1);
-            launch_near_classify(ctx->stream, p);
-            CC_HIP(ctx, hipGetLastError());
-            unsigned long long cnt = 0;
-            CC_HIP(ctx, hipMemcpyAsync(&cnt, p.near_count, sizeof cnt, hipMemcpyDeviceToHost, ctx->stream));
-            CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            near = (int64_t)cnt;
-            if (near <= cap) break;
-            cap = near + 4096;
-        }
+        CC_HIP(ctx, ensure(ctx->ncounts, sizeof(int64_t) * (size_t)n));
+        p.near_counts = (int64_t *)ctx->ncounts.p;
+        launch_fill_zero(ctx->stream, p.near_counts, n);
     }
     CC_HIP(ctx, hipEventRecord(ctx->evm0, ctx->stream));
     launch_field(ctx->stream, p, precision, split);
@@ -415,9 +401,35 @@ ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, const double 
     CC_HIP(ctx, hipEventRecord(ctx->evm1, ctx->stream));
     launch_field_reduce(ctx->stream, p);
     CC_HIP(ctx, hipGetLastError());
-    if (near_fp64 && near > 0) {
-        launch_near_eval(ctx->stream, p, near);
-        CC_HIP(ctx, hipGetLastError());
+    if (near_fp64) {
+        // offsets of each point's near entries, then the entries in (point, coil) order, their FP64 values, and
+        // the in-order sums (field_kernel counted the near pairs it skipped)
+        CC_HIP(ctx, ensure(ctx->noffsets, sizeof(int64_t) * (size_t)n));
+        p.near_offsets = (const int64_t *)ctx->noffsets.p;
+        size_t tb = 0;
+        CC_HIP(ctx, near_scan(ctx->stream, p.near_counts, (int64_t *)ctx->noffsets.p, n, nullptr, tb));
+        CC_HIP(ctx, ensure(ctx->ntemp, tb > 0 ? tb : 1));
+        CC_HIP(ctx, near_scan(ctx->stream, p.near_counts, (int64_t *)ctx->noffsets.p, n, ctx->ntemp.p, tb));
+        int64_t last[2] = {0, 0};
+        CC_HIP(ctx, hipMemcpyAsync(&last[0], p.near_offsets + (n - 1), sizeof(int64_t), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        CC_HIP(ctx, hipMemcpyAsync(&last[1], p.near_counts + (n - 1), sizeof(int64_t), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        near = last[0] + last[1];
+        if (near > 0) {
+            p.near_width = (p.A ? 3 : 0) + (p.E ? 3 : 0) + (p.B ? 3 : 0) + (p.G ? 9 : 0);
+            CC_HIP(ctx, ensure(ctx->nentries, sizeof(int2) * (size_t)near));
+            CC_HIP(ctx, ensure(ctx->nvals, sizeof(double) * (size_t)p.near_width * (size_t)near));
+            p.near_entries = (int2 *)ctx->nentries.p;
+            p.near_vals = (double *)ctx->nvals.p;
+            launch_near_fill(ctx->stream, p);
+            CC_HIP(ctx, hipGetLastError());
+            launch_near_eval(ctx->stream, p, near);
+            CC_HIP(ctx, hipGetLastError());
+            launch_near_add(ctx->stream, p);
+            CC_HIP(ctx, hipGetLastError());
+        }
     }
     CC_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     if (!dev) {

@@ -38,7 +38,8 @@ struct ccoil_ctx_s {
     float *gl_wcosf = nullptr;
     double *gl_opc = nullptr;
     float *gl_opcf = nullptr;
-    ccoil::DevBuf coils, pts, outA, outB, outE, outG, queue, counter, fpartial;
+    ccoil::DevBuf coils, pts, outA, outB, outE, outG, fpartial;
+    ccoil::DevBuf ncounts, noffsets, nentries, nvals, ntemp;   // FP32C near-winding pass
     ccoil::DevBuf pprim, psec, ppairs, ppartial, pcounter, pM, pF, pT, pdM;
     ccoil::DevBuf acoils, apts, agB, apartial, agrad;   // field adjoint
     ccoil::DevBuf zitems, zpartial, zout;               // coaxial rules

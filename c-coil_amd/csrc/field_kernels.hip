@@ -102,6 +102,7 @@ __global__ __launch_bounds__(kFieldBlock) void field_kernel(FieldParams p)
     for (int k = 0; k < 3; ++k) { acc.A[k] = 0.0; acc.E[k] = 0.0; acc.B[k] = 0.0; }
     for (int k = 0; k < 9; ++k) acc.G[k] = 0.0;
 
+    int n_near = 0;   // FP32C: near-winding coils of this point in this chunk
     const int c0 = blockIdx.y * p.coils_per_chunk, c1 = min(c0 + p.coils_per_chunk, p.n_coils);
     for (int ci = c0; ci < c1; ++ci) {
         const CoilDev &c = p.coils[ci];
@@ -119,11 +120,14 @@ __global__ __launch_bounds__(kFieldBlock) void field_kernel(FieldParams p)
         } else {
             const bool skip = p.near_fp64 && is_near(c, z, r);
             if (!skip) eval_stable<float, WA, WB, WG>(c, p.tab, z, r, lane, SPLIT, L);
+            else ++n_near;
         }
         if (SPLIT > 1) reduce_local<SPLIT>(L);
         accumulate_global(c, L, r, cph, sph, p.q, acc);
     }
     if (!active || lane != 0) return;
+    if (PREC == kFP32C && n_near > 0)   // integer sums: exact in any order
+        atomicAdd((unsigned long long *)&p.near_counts[pi], (unsigned long long)n_near);
     if (p.n_chunks > 1) {
         double *o = p.partial + ((int64_t)blockIdx.y * p.n_pts + pi) * p.partial_width;
         if (p.A) { for (int k = 0; k < 3; ++k) o[k] = acc.A[k]; o += 3; }
@@ -154,48 +158,6 @@ __global__ __launch_bounds__(256) void field_reduce_kernel(FieldParams p)
     if (p.E) { if (k < 3) { p.E[3 * pi + k] = s; return; } k -= 3; }
     if (p.B) { if (k < 3) { p.B[3 * pi + k] = s; return; } k -= 3; }
     if (p.G) p.G[9 * pi + k] = s;
-}
-
-// ---------------------------------------------------------------------------------------------
-// Near-winding pass (FP32C): classify (coil, point) pairs, then re-evaluate them in FP64.
-// ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void near_classify_kernel(FieldParams p)
-{
-    const int64_t pi = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (pi >= p.n_pts) return;
-    const double px = p.pts[3 * pi], py = p.pts[3 * pi + 1], pz = p.pts[3 * pi + 2];
-    for (int ci = 0; ci < p.n_coils; ++ci) {
-        const CoilDev &c = p.coils[ci];
-        double z, r, cph, sph;
-        to_local(c, px, py, pz, z, r, cph, sph);
-        if (is_near(c, z, r)) {
-            unsigned long long slot = atomicAdd(p.near_count, 1ull);
-            if (slot < (unsigned long long)p.near_capacity) p.near_queue[slot] = make_int2((int)pi, ci);
-        }
-    }
-}
-
-template <bool WA, bool WB, bool WG>
-__global__ __launch_bounds__(256) void near_eval_kernel(FieldParams p, int64_t count)
-{
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= count) return;
-    const int2 ent = p.near_queue[e];
-    const int64_t pi = ent.x;
-    const CoilDev &c = p.coils[ent.y];
-    const double px = p.pts[3 * pi], py = p.pts[3 * pi + 1], pz = p.pts[3 * pi + 2];
-    double z, r, cph, sph;
-    to_local(c, px, py, pz, z, r, cph, sph);
-    Local L;
-    eval_stable<double, WA, WB, WG>(c, p.tab, z, r, 0, 1, L);
-    Cart acc;
-    for (int k = 0; k < 3; ++k) { acc.A[k] = 0.0; acc.E[k] = 0.0; acc.B[k] = 0.0; }
-    for (int k = 0; k < 9; ++k) acc.G[k] = 0.0;
-    accumulate_global(c, L, r, cph, sph, p.q, acc);
-    if (p.A) for (int k = 0; k < 3; ++k) unsafeAtomicAdd(&p.A[3 * pi + k], acc.A[k]);
-    if (p.E) for (int k = 0; k < 3; ++k) unsafeAtomicAdd(&p.E[3 * pi + k], acc.E[k]);
-    if (p.B) for (int k = 0; k < 3; ++k) unsafeAtomicAdd(&p.B[3 * pi + k], acc.B[k]);
-    if (p.G) for (int k = 0; k < 9; ++k) unsafeAtomicAdd(&p.G[9 * pi + k], acc.G[k]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -316,30 +278,6 @@ void launch_field_reduce(hipStream_t s, const FieldParams &p)
     if (p.n_pts <= 0 || p.n_chunks <= 1) return;
     const int64_t n = p.n_pts * p.partial_width;
     hipLaunchKernelGGL(field_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p);
-}
-
-void launch_near_classify(hipStream_t s, const FieldParams &p)
-{
-    if (p.n_pts <= 0) return;
-    const unsigned blocks = (unsigned)((p.n_pts + 255) / 256);
-    hipLaunchKernelGGL(near_classify_kernel, dim3(blocks), dim3(256), 0, s, p);
-}
-
-void launch_near_eval(hipStream_t s, const FieldParams &p, int64_t count)
-{
-    if (count <= 0) return;
-    const unsigned blocks = (unsigned)((count + 255) / 256);
-    const bool wa = p.q & (Q_A | Q_E), wb = p.q & Q_B, wg = p.q & (Q_G | Q_GJ);
-    const int code = (wa ? 4 : 0) | (wb ? 2 : 0) | (wg ? 1 : 0);
-    switch (code) {
-        case 1: hipLaunchKernelGGL((near_eval_kernel<false, false, true>), dim3(blocks), dim3(256), 0, s, p, count); break;
-        case 2: hipLaunchKernelGGL((near_eval_kernel<false, true, false>), dim3(blocks), dim3(256), 0, s, p, count); break;
-        case 3: hipLaunchKernelGGL((near_eval_kernel<false, true, true>), dim3(blocks), dim3(256), 0, s, p, count); break;
-        case 4: hipLaunchKernelGGL((near_eval_kernel<true, false, false>), dim3(blocks), dim3(256), 0, s, p, count); break;
-        case 5: hipLaunchKernelGGL((near_eval_kernel<true, false, true>), dim3(blocks), dim3(256), 0, s, p, count); break;
-        case 6: hipLaunchKernelGGL((near_eval_kernel<true, true, false>), dim3(blocks), dim3(256), 0, s, p, count); break;
-        default: hipLaunchKernelGGL((near_eval_kernel<true, true, true>), dim3(blocks), dim3(256), 0, s, p, count); break;
-    }
 }
 
 }  // namespace ccoil

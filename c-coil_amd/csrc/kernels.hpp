@@ -18,9 +18,15 @@ struct FieldParams {
     uint32_t q;            // Q_* mask
     Tables tab;
     int32_t near_fp64;     // FP32C: skip near pairs here, re-evaluate in FP64
-    int2 *near_queue;      // (point, coil)
-    unsigned long long *near_count;
-    int64_t near_capacity;
+    // FP32C near-winding pass: field_kernel counts each point's near coils (near_counts); an exclusive scan gives
+    // near_offsets; near_fill writes the (point, coil) entries in point-then-coil order; near_eval writes each
+    // entry's FP64 contribution (near_width doubles: the requested A, E, B, G in that order); near_add adds a
+    // point's entries in order. No floating-point atomics: the results are deterministic.
+    int64_t *near_counts;
+    const int64_t *near_offsets;
+    int2 *near_entries;
+    double *near_vals;
+    int32_t near_width;
     // Coil chunks (grid y): each block row sums its chunk of coils into partial ([chunk][point][width]: the
     // requested outputs A, E, B, G in that order, 3 or 9 doubles each) and field_reduce_kernel adds the chunks in
     // order.
@@ -58,7 +64,11 @@ void launch_field(hipStream_t s, const FieldParams &p, int precision, int split)
 void launch_field_reduce(hipStream_t s, const FieldParams &p);   // after launch_field when n_chunks > 1
 // Resident 256-thread blocks per CU of the field kernel instance (0 if the runtime cannot tell).
 int field_blocks_per_cu(uint32_t q, int precision, int split);
-void launch_near_classify(hipStream_t s, const FieldParams &p);
+// Exclusive prefix sum of n counts into offsets; with temp == NULL only sets temp_bytes.
+hipError_t near_scan(hipStream_t s, const int64_t *counts, int64_t *offsets, int64_t n, void *temp,
+                     size_t &temp_bytes);
+void launch_near_fill(hipStream_t s, const FieldParams &p);
 void launch_near_eval(hipStream_t s, const FieldParams &p, int64_t count);
+void launch_near_add(hipStream_t s, const FieldParams &p);
 
 }  // namespace ccoil

@@ -1,0 +1,112 @@
+// FP32C near-winding pass (gfx950): (coil, point) pairs inside a coil's near shell are re-evaluated in FP64.
+//
+// field_kernel<kFP32C> skips those pairs and counts them per point; an exclusive scan of the counts places each
+// point's entries; near_fill_kernel writes them in (point, coil) order; near_eval_kernel evaluates one entry per
+// thread into its own slot; near_add_kernel adds a point's slots to the outputs in coil order. Every sum has a fixed
+// order, so FP32C results are bitwise reproducible.
+#include "kernels.hpp"
+#include "integrands.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+namespace ccoil {
+
+__global__ __launch_bounds__(256) void near_fill_kernel(FieldParams p)
+{
+    const int64_t pi = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (pi >= p.n_pts) return;
+    const int64_t count = p.near_counts[pi];
+    if (count == 0) return;
+    const double px = p.pts[3 * pi], py = p.pts[3 * pi + 1], pz = p.pts[3 * pi + 2];
+    int64_t slot = p.near_offsets[pi];
+    const int64_t end = slot + count;
+    for (int ci = 0; ci < p.n_coils && slot < end; ++ci) {
+        const CoilDev &c = p.coils[ci];
+        double z, r, cph, sph;
+        to_local(c, px, py, pz, z, r, cph, sph);
+        if (is_near(c, z, r)) p.near_entries[slot++] = make_int2((int)pi, ci);   // the field kernel's own test
+    }
+    // the same test on the same inputs finds the same coils; should it ever find fewer, the rest stay inert
+    for (; slot < end; ++slot) p.near_entries[slot] = make_int2((int)pi, -1);
+}
+
+template <bool WA, bool WB, bool WG>
+__global__ __launch_bounds__(256) void near_eval_kernel(FieldParams p, int64_t count)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= count) return;
+    const int2 ent = p.near_entries[e];
+    double *o = p.near_vals + e * p.near_width;
+    if (ent.y < 0 || ent.y >= p.n_coils) {
+        for (int k = 0; k < p.near_width; ++k) o[k] = 0.0;
+        return;
+    }
+    const int64_t pi = ent.x;
+    const CoilDev &c = p.coils[ent.y];
+    const double px = p.pts[3 * pi], py = p.pts[3 * pi + 1], pz = p.pts[3 * pi + 2];
+    double z, r, cph, sph;
+    to_local(c, px, py, pz, z, r, cph, sph);
+    Local L;
+    eval_stable<double, WA, WB, WG>(c, p.tab, z, r, 0, 1, L);
+    Cart acc;
+    for (int k = 0; k < 3; ++k) { acc.A[k] = 0.0; acc.E[k] = 0.0; acc.B[k] = 0.0; }
+    for (int k = 0; k < 9; ++k) acc.G[k] = 0.0;
+    accumulate_global(c, L, r, cph, sph, p.q, acc);
+    if (p.A) { for (int k = 0; k < 3; ++k) o[k] = acc.A[k]; o += 3; }
+    if (p.E) { for (int k = 0; k < 3; ++k) o[k] = acc.E[k]; o += 3; }
+    if (p.B) { for (int k = 0; k < 3; ++k) o[k] = acc.B[k]; o += 3; }
+    if (p.G) for (int k = 0; k < 9; ++k) o[k] = acc.G[k];
+}
+
+__global__ __launch_bounds__(256) void near_add_kernel(FieldParams p)
+{
+    const int64_t pi = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (pi >= p.n_pts) return;
+    const int64_t count = p.near_counts[pi];
+    if (count == 0) return;
+    const int64_t e0 = p.near_offsets[pi];
+    for (int64_t e = e0; e < e0 + count; ++e) {
+        const double *o = p.near_vals + e * p.near_width;
+        if (p.A) { for (int k = 0; k < 3; ++k) p.A[3 * pi + k] += o[k]; o += 3; }
+        if (p.E) { for (int k = 0; k < 3; ++k) p.E[3 * pi + k] += o[k]; o += 3; }
+        if (p.B) { for (int k = 0; k < 3; ++k) p.B[3 * pi + k] += o[k]; o += 3; }
+        if (p.G) for (int k = 0; k < 9; ++k) p.G[9 * pi + k] += o[k];
+    }
+}
+
+hipError_t near_scan(hipStream_t s, const int64_t *counts, int64_t *offsets, int64_t n, void *temp,
+                     size_t &temp_bytes)
+{
+    return hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, counts, offsets, (size_t)n, s);
+}
+
+void launch_near_fill(hipStream_t s, const FieldParams &p)
+{
+    if (p.n_pts <= 0) return;
+    hipLaunchKernelGGL(near_fill_kernel, dim3((unsigned)((p.n_pts + 255) / 256)), dim3(256), 0, s, p);
+}
+
+void launch_near_eval(hipStream_t s, const FieldParams &p, int64_t count)
+{
+    if (count <= 0) return;
+    const unsigned blocks = (unsigned)((count + 255) / 256);
+    const bool wa = p.q & (Q_A | Q_E), wb = p.q & Q_B, wg = p.q & (Q_G | Q_GJ);
+    const int code = (wa ? 4 : 0) | (wb ? 2 : 0) | (wg ? 1 : 0);
+    switch (code) {
+        case 1: hipLaunchKernelGGL((near_eval_kernel<false, false, true>), dim3(blocks), dim3(256), 0, s, p, count); break;
+        case 2: hipLaunchKernelGGL((near_eval_kernel<false, true, false>), dim3(blocks), dim3(256), 0, s, p, count); break;
+        case 3: hipLaunchKernelGGL((near_eval_kernel<false, true, true>), dim3(blocks), dim3(256), 0, s, p, count); break;
+        case 4: hipLaunchKernelGGL((near_eval_kernel<true, false, false>), dim3(blocks), dim3(256), 0, s, p, count); break;
+        case 5: hipLaunchKernelGGL((near_eval_kernel<true, false, true>), dim3(blocks), dim3(256), 0, s, p, count); break;
+        case 6: hipLaunchKernelGGL((near_eval_kernel<true, true, false>), dim3(blocks), dim3(256), 0, s, p, count); break;
+        default: hipLaunchKernelGGL((near_eval_kernel<true, true, true>), dim3(blocks), dim3(256), 0, s, p, count); break;
+    }
+}
+
+void launch_near_add(hipStream_t s, const FieldParams &p)
+{
+    if (p.n_pts <= 0) return;
+    hipLaunchKernelGGL(near_add_kernel, dim3((unsigned)((p.n_pts + 255) / 256)), dim3(256), 0, s, p);
+}
+
+}  // namespace ccoil

@@ -210,3 +210,29 @@ def test_coil_chunking_matches_single_pass(ctx, prec):
     for key in ("A", "B", "G"):
         e = H.rel_err_vec(small[key], big[key][:1024])
         assert np.nanmax(e) < 1e-12, (key, np.nanmax(e))
+
+
+def test_fp32c_near_pass_is_deterministic(ctx):
+    """FP32C re-evaluates near-winding (coil, point) pairs in FP64 and adds them per point in coil order (no
+    floating-point atomics): repeated calls give bitwise equal results, also where a point is near several coils;
+    with CCOIL_NO_NEAR_FP64 nothing is re-evaluated."""
+    params, pts = W.group_workload(48, 8192, seed=11)
+    pts = pts * 0.4                                   # dense: many points near several coils
+    rows, _ = H.gpu_args_rows(params)
+    cs = abi.CoilSet.from_rows(rows)
+    q = abi.A | abi.B | abi.G
+    first, rep = ctx.fields(cs, pts, q, abi.FP32C)
+    near = H.near_mask(params, pts)
+    assert rep.near_pairs > len(pts) // 10
+    for _ in range(3):
+        again, rep2 = ctx.fields(cs, pts, q, abi.FP32C)
+        assert rep2.near_pairs == rep.near_pairs
+        for k in ("A", "B", "G"):
+            assert np.array_equal(again[k], first[k]), k
+    ref, _ = ctx.fields(cs, pts, q, abi.FP64)
+    far_only, rep3 = ctx.fields(cs, pts, q, abi.FP32C, abi.NO_NEAR_FP64)
+    assert rep3.near_pairs == 0
+    # the FP64 near pass brings the near points to FP32C's far-point accuracy
+    e_near = H.rel_err_vec(first["B"], ref["B"])[near]
+    e_skip = H.rel_err_vec(far_only["B"], ref["B"])[near]
+    assert np.nanmedian(e_near) < np.nanmedian(e_skip)
