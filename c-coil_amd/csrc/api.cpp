@@ -235,8 +235,8 @@ ccoil_status ccoil_destroy(ccoil_ctx ctx)
     for (auto p : fp)
         if (p) (void)hipFree(p);
     DevBuf *bufs[] = {&ctx->coils, &ctx->fpartial, &ctx->pts, &ctx->outA, &ctx->outB, &ctx->outE, &ctx->outG,
-                      &ctx->ncounts, &ctx->noffsets, &ctx->nentries, &ctx->nvals, &ctx->ntemp, &ctx->pprim, &ctx->psec, &ctx->ppairs, &ctx->ppartial,
-                      &ctx->pcounter, &ctx->pM, &ctx->pF, &ctx->pT, &ctx->pdM, &ctx->acoils, &ctx->apts, &ctx->agB,
+                      &ctx->ncounts, &ctx->noffsets, &ctx->nkeys, &ctx->nkeys2, &ctx->nrecs, &ctx->nrecs2,
+                      &ctx->nvals, &ctx->ntemp, &ctx->pprim, &ctx->psec, &ctx->ppairs, &ctx->ppartial, &ctx->pcounter, &ctx->pM, &ctx->pF, &ctx->pT, &ctx->pdM, &ctx->acoils, &ctx->apts, &ctx->agB,
                       &ctx->apartial, &ctx->agrad, &ctx->zitems, &ctx->zpartial, &ctx->zout};
     for (auto b : bufs) free_buf(*b);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -402,8 +402,8 @@ ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, const double 
     launch_field_reduce(ctx->stream, p);
     CC_HIP(ctx, hipGetLastError());
     if (near_fp64) {
-        // offsets of each point's near entries, then the entries in (point, coil) order, their FP64 values, and
-        // the in-order sums (field_kernel counted the near pairs it skipped)
+        // offsets of each point's near slots, then the (coil, point | slot) records in (point, coil) order, sorted
+        // by coil, their FP64 values, and the in-order sums (field_kernel counted the near pairs it skipped)
         CC_HIP(ctx, ensure(ctx->noffsets, sizeof(int64_t) * (size_t)n));
         p.near_offsets = (const int64_t *)ctx->noffsets.p;
         size_t tb = 0;
@@ -417,14 +417,27 @@ ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, const double 
                                    ctx->stream));
         CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
         near = last[0] + last[1];
+        if (near > (int64_t)INT32_MAX)
+            return fail(ctx, CCOIL_ERR_OUT_OF_MEMORY, "FP32C: " + std::to_string(near) +
+                        " near-winding (coil, point) pairs exceed the near pass's limit of 2^31 - 1; split the points");
         if (near > 0) {
             p.near_width = (p.A ? 3 : 0) + (p.E ? 3 : 0) + (p.B ? 3 : 0) + (p.G ? 9 : 0);
-            CC_HIP(ctx, ensure(ctx->nentries, sizeof(int2) * (size_t)near));
+            CC_HIP(ctx, ensure(ctx->nkeys, sizeof(uint32_t) * (size_t)near));
+            CC_HIP(ctx, ensure(ctx->nkeys2, sizeof(uint32_t) * (size_t)near));
+            CC_HIP(ctx, ensure(ctx->nrecs, sizeof(uint64_t) * (size_t)near));
+            CC_HIP(ctx, ensure(ctx->nrecs2, sizeof(uint64_t) * (size_t)near));
             CC_HIP(ctx, ensure(ctx->nvals, sizeof(double) * (size_t)p.near_width * (size_t)near));
-            p.near_entries = (int2 *)ctx->nentries.p;
+            p.near_keys = (uint32_t *)ctx->nkeys.p;
+            p.near_keys_sorted = (uint32_t *)ctx->nkeys2.p;
+            p.near_recs = (uint64_t *)ctx->nrecs.p;
+            p.near_recs_sorted = (uint64_t *)ctx->nrecs2.p;
             p.near_vals = (double *)ctx->nvals.p;
             launch_near_fill(ctx->stream, p);
             CC_HIP(ctx, hipGetLastError());
+            size_t sb = 0;
+            CC_HIP(ctx, near_sort(ctx->stream, p, near, nullptr, sb));
+            CC_HIP(ctx, ensure(ctx->ntemp, sb > 0 ? sb : 1));
+            CC_HIP(ctx, near_sort(ctx->stream, p, near, ctx->ntemp.p, sb));
             launch_near_eval(ctx->stream, p, near);
             CC_HIP(ctx, hipGetLastError());
             launch_near_add(ctx->stream, p);

@@ -39,7 +39,7 @@ struct ccoil_ctx_s {
     double *gl_opc = nullptr;
     float *gl_opcf = nullptr;
     ccoil::DevBuf coils, pts, outA, outB, outE, outG, fpartial;
-    ccoil::DevBuf ncounts, noffsets, nentries, nvals, ntemp;   // FP32C near-winding pass
+    ccoil::DevBuf ncounts, noffsets, nkeys, nkeys2, nrecs, nrecs2, nvals, ntemp;   // FP32C near-winding pass
     ccoil::DevBuf pprim, psec, ppairs, ppartial, pcounter, pM, pF, pT, pdM;
     ccoil::DevBuf acoils, apts, agB, apartial, agrad;   // field adjoint
     ccoil::DevBuf zitems, zpartial, zout;               // coaxial rules

@@ -19,12 +19,14 @@ struct FieldParams {
     Tables tab;
     int32_t near_fp64;     // FP32C: skip near pairs here, re-evaluate in FP64
     // FP32C near-winding pass: field_kernel counts each point's near coils (near_counts); an exclusive scan gives
-    // near_offsets; near_fill writes the (point, coil) entries in point-then-coil order; near_eval writes each
-    // entry's FP64 contribution (near_width doubles: the requested A, E, B, G in that order); near_add adds a
-    // point's entries in order. No floating-point atomics: the results are deterministic.
+    // near_offsets; near_fill writes per slot, in point-then-coil order, the coil (near_keys) and point << 32 |
+    // slot (near_recs); a radix sort by coil gives near_keys_sorted / near_recs_sorted; near_eval writes each
+    // slot's FP64 contribution (near_width doubles: the requested A, E, B, G in that order); near_add adds a
+    // point's slots in order. No floating-point atomics: the results are deterministic.
     int64_t *near_counts;
     const int64_t *near_offsets;
-    int2 *near_entries;
+    uint32_t *near_keys, *near_keys_sorted;
+    uint64_t *near_recs, *near_recs_sorted;
     double *near_vals;
     int32_t near_width;
     // Coil chunks (grid y): each block row sums its chunk of coils into partial ([chunk][point][width]: the
@@ -67,6 +69,8 @@ int field_blocks_per_cu(uint32_t q, int precision, int split);
 // Exclusive prefix sum of n counts into offsets; with temp == NULL only sets temp_bytes.
 hipError_t near_scan(hipStream_t s, const int64_t *counts, int64_t *offsets, int64_t n, void *temp,
                      size_t &temp_bytes);
+// Sorts count near records by coil (stable: point order within a coil); with temp == NULL only sets temp_bytes.
+hipError_t near_sort(hipStream_t s, const FieldParams &p, int64_t count, void *temp, size_t &temp_bytes);
 void launch_near_fill(hipStream_t s, const FieldParams &p);
 void launch_near_eval(hipStream_t s, const FieldParams &p, int64_t count);
 void launch_near_add(hipStream_t s, const FieldParams &p);

@@ -1,9 +1,11 @@
 // FP32C near-winding pass (gfx950): (coil, point) pairs inside a coil's near shell are re-evaluated in FP64.
 //
-// field_kernel<kFP32C> skips those pairs and counts them per point; an exclusive scan of the counts places each
-// point's entries; near_fill_kernel writes them in (point, coil) order; near_eval_kernel evaluates one entry per
-// thread into its own slot; near_add_kernel adds a point's slots to the outputs in coil order. Every sum has a fixed
-// order, so FP32C results are bitwise reproducible.
+// field_kernel<kFP32C> skips those pairs and counts them per point; an exclusive scan of the counts gives each
+// point's slots; near_fill_kernel writes one (coil key, point | slot) record per slot in (point, coil) order; a
+// radix sort by coil groups the records so that the lanes of a wave evaluate the same coil (uniform loop bounds,
+// one branch of the fast / slow rule); near_eval_kernel evaluates one record per thread into its slot;
+// near_add_kernel adds a point's slots to the outputs in coil order. A slot's value depends only on its (coil,
+// point), and every sum has a fixed order, so FP32C results are bitwise reproducible.
 #include "kernels.hpp"
 #include "integrands.hpp"
 
@@ -20,34 +22,45 @@ __global__ __launch_bounds__(256) void near_fill_kernel(FieldParams p)
     const double px = p.pts[3 * pi], py = p.pts[3 * pi + 1], pz = p.pts[3 * pi + 2];
     int64_t slot = p.near_offsets[pi];
     const int64_t end = slot + count;
+    const uint64_t hi = (uint64_t)pi << 32;
     for (int ci = 0; ci < p.n_coils && slot < end; ++ci) {
         const CoilDev &c = p.coils[ci];
         double z, r, cph, sph;
         to_local(c, px, py, pz, z, r, cph, sph);
-        if (is_near(c, z, r)) p.near_entries[slot++] = make_int2((int)pi, ci);   // the field kernel's own test
+        if (is_near(c, z, r)) {   // the field kernel's own test
+            p.near_keys[slot] = (uint32_t)ci;
+            p.near_recs[slot] = hi | (uint64_t)slot;
+            ++slot;
+        }
     }
     // the same test on the same inputs finds the same coils; should it ever find fewer, the rest stay inert
-    for (; slot < end; ++slot) p.near_entries[slot] = make_int2((int)pi, -1);
+    // (key n_coils, sorted last, evaluated as zero)
+    for (; slot < end; ++slot) {
+        p.near_keys[slot] = (uint32_t)p.n_coils;
+        p.near_recs[slot] = hi | (uint64_t)slot;
+    }
 }
 
+// Records sorted by coil: lane e takes the e-th record; its value goes to the record's slot.
 template <bool WA, bool WB, bool WG>
 __global__ __launch_bounds__(256) void near_eval_kernel(FieldParams p, int64_t count)
 {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= count) return;
-    const int2 ent = p.near_entries[e];
-    double *o = p.near_vals + e * p.near_width;
-    if (ent.y < 0 || ent.y >= p.n_coils) {
+    const uint32_t ci = p.near_keys_sorted[e];
+    const uint64_t rec = p.near_recs_sorted[e];
+    const int64_t pi = (int64_t)(rec >> 32), slot = (int64_t)(rec & 0xffffffffu);
+    double *o = p.near_vals + slot * p.near_width;
+    if (ci >= (uint32_t)p.n_coils) {
         for (int k = 0; k < p.near_width; ++k) o[k] = 0.0;
         return;
     }
-    const int64_t pi = ent.x;
-    const CoilDev &c = p.coils[ent.y];
+    const CoilDev &c = p.coils[ci];
     const double px = p.pts[3 * pi], py = p.pts[3 * pi + 1], pz = p.pts[3 * pi + 2];
     double z, r, cph, sph;
     to_local(c, px, py, pz, z, r, cph, sph);
     Local L;
-    eval_stable<double, WA, WB, WG>(c, p.tab, z, r, 0, 1, L);
+    eval_stable<f64s, WA, WB, WG>(c, p.tab, z, r, 0, 1, L);
     Cart acc;
     for (int k = 0; k < 3; ++k) { acc.A[k] = 0.0; acc.E[k] = 0.0; acc.B[k] = 0.0; }
     for (int k = 0; k < 9; ++k) acc.G[k] = 0.0;
@@ -78,6 +91,14 @@ hipError_t near_scan(hipStream_t s, const int64_t *counts, int64_t *offsets, int
                      size_t &temp_bytes)
 {
     return hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, counts, offsets, (size_t)n, s);
+}
+
+hipError_t near_sort(hipStream_t s, const FieldParams &p, int64_t count, void *temp, size_t &temp_bytes)
+{
+    int bits = 1;   // keys are 0 .. n_coils
+    while (bits < 32 && ((uint64_t)p.n_coils >> bits) != 0) ++bits;
+    return hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, p.near_keys, p.near_keys_sorted, p.near_recs,
+                                              p.near_recs_sorted, (int)count, 0, bits, s);
 }
 
 void launch_near_fill(hipStream_t s, const FieldParams &p)
