@@ -14,7 +14,8 @@ Reported beside it in the same line:
                (same integrands, IEEE division and sqrt) and CCOIL_FP64_REFERENCE (the reference's integrands in
                double: the plain-FP64 comparator): speed and roofline fraction;
   accuracy     FP32C and FP64_IEEE against CCOIL_FP64 at every point (max / p99 / p99.9, points inside the
-               near-winding shell of any coil and the rest);
+               near-winding shell of any coil and the rest), relative to the field and relative to the sum of the
+               coils' magnitudes at the point (the superposition's conditioning);
   pairs        mutual inductance + force + torque of 65,536 random coil pairs with per-dimension increments
                ~ U{4, 8, ..., 80} (one ccoil_pairs call) in CCOIL_FP64 and CCOIL_FP32C: pairs/s, node-pairs/s and the
                FP32C error;
@@ -234,11 +235,28 @@ def main():
             b64, g64 = head["B"].cpu().numpy(), head["G"].cpu().numpy()
             near = W.near_mask(rows, pts)
             acc = {"against": "CCOIL_FP64 on the same points", "near_points": int(near.sum())}
+            # the superposition's own conditioning: sum over coils of |B_k| and |G_k| (Frobenius) per point, from one
+            # CCOIL_FP64 launch per coil
+            mb = torch.zeros(n, dtype=torch.float64, device=dev)
+            mg = torch.zeros(n, dtype=torch.float64, device=dev)
+            one = new_out()
+            for r in rows:
+                ctx.fields(abi.CoilSet.from_rows([r]), d_pts, q, abi.FP64, abi.DEVICE_POINTERS, out=one)
+                mb += torch.linalg.vector_norm(one["B"], dim=1)
+                mg += torch.linalg.vector_norm(one["G"], dim=1)
+            mb, mg = mb.cpu().numpy(), mg.cpu().numpy()
+            acc["conditioned"] = ("error over the sum of the coils' magnitudes at the point, sum_k |B_k| "
+                                  "(sum_k |G_k| for the gradient)")
             for name in ("fp32c", "fp64_ieee"):
-                eb, okb = rel_err(outs[name]["B"].cpu().numpy(), b64)
-                eg, okg = rel_err(outs[name]["G"].cpu().numpy(), g64)
-                acc[name] = {"B": {"far": stats(eb[~near[okb]]), "near": stats(eb[near[okb]])},
-                             "G": {"far": stats(eg[~near[okg]]), "near": stats(eg[near[okg]])}}
+                ob, og = outs[name]["B"].cpu().numpy(), outs[name]["G"].cpu().numpy()
+                eb, okb = rel_err(ob, b64)
+                eg, okg = rel_err(og, g64)
+                cb = np.linalg.norm(ob - b64, axis=1) / mb
+                cg = np.linalg.norm(og - g64, axis=1) / mg
+                acc[name] = {"B": {"far": stats(eb[~near[okb]]), "near": stats(eb[near[okb]]),
+                                   "far_conditioned": stats(cb[~near]), "near_conditioned": stats(cb[near])},
+                             "G": {"far": stats(eg[~near[okg]]), "near": stats(eg[near[okg]]),
+                                   "far_conditioned": stats(cg[~near]), "near_conditioned": stats(cg[near])}}
             result["accuracy"] = acc
             log(f"accuracy: {json.dumps(acc)}")
         if "fp64_reference" in side:
