@@ -130,7 +130,7 @@ def main():
     ap.add_argument("--points", type=int, default=1 << 20)
     ap.add_argument("--pairs", type=int, default=65536)
     ap.add_argument("--side-steps", type=int, default=3, help="timed steps of each secondary field mode")
-    ap.add_argument("--cpu-points", type=int, default=65536)
+    ap.add_argument("--cpu-points", type=int, default=1 << 19)
     ap.add_argument("--precision", choices=("fp64", "fp32c"), default="fp64", help="mode of the timed steps")
     ap.add_argument("--no-side", action="store_true", help="skip the secondary field modes and the accuracy report")
     ap.add_argument("--no-pairs", action="store_true", help="skip the pair leg")
