@@ -354,13 +354,18 @@ ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, const double 
     }
 
     // Launch shape: split the thickness rows over 4 lanes when there are too few points to give every
-    // SIMD of the 256 CUs several waves, and split the coils into chunks (grid y, summed in order by
+    // SIMD of the 256 CUs several waves, and over 16 lanes when even one chunk per coil would leave the launch
+    // below one wave per SIMD (few points and few coils: then each lane's serial chain of nodes, not the issue
+    // rate, sets the kernel's duration; one coil at 4,096 points measured 46 us at 4 lanes and 25 us at 16, while
+    // at 65,536 points 16 lanes were slower). Split the coils into chunks (grid y, summed in order by
     // field_reduce_kernel) while the launch still has fewer than about 8 waves per SIMD. Large launches take
     // 1-4 chunks, whichever makes the last round of resident waves fullest: every wave does the same work, so
     // with R rounds of resident waves the launch runs ceil(R) rounds.
-    const int split = n < (int64_t)256 * 4 * 64 * 4 ? 4 : 1;
+    const int64_t target = (int64_t)256 * 4 * 8;
+    int split = n < (int64_t)256 * 4 * 64 * 4 ? 4 : 1;
+    if (split == 4 && (n * 4 + 63) / 64 * (int64_t)hc.size() < (int64_t)256 * 4) split = 16;
     {
-        const int64_t waves = (n * split + 63) / 64, target = (int64_t)256 * 4 * 8;
+        const int64_t waves = (n * split + 63) / 64;
         int64_t chunks = waves >= target ? 1 : std::min<int64_t>((int64_t)hc.size(), (target + waves - 1) / waves);
         if (waves >= target && n * 18 < ((int64_t)1 << 31)) {   // the reduction grid stays below 2^31 threads
             const int bpc = field_blocks_per_cu(quantities, precision, split);

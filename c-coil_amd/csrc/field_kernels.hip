@@ -216,7 +216,8 @@ static void launch_field_q(hipStream_t s, const FieldParams &p, int split)
     const int64_t threads = p.n_pts * split;
     const dim3 grid((unsigned)((threads + kFieldBlock - 1) / kFieldBlock), (unsigned)p.n_chunks);
     if (split == 1) hipLaunchKernelGGL((field_kernel<PREC, WA, WB, WG, 1>), grid, dim3(kFieldBlock), 0, s, p);
-    else hipLaunchKernelGGL((field_kernel<PREC, WA, WB, WG, 4>), grid, dim3(kFieldBlock), 0, s, p);
+    else if (split == 4) hipLaunchKernelGGL((field_kernel<PREC, WA, WB, WG, 4>), grid, dim3(kFieldBlock), 0, s, p);
+    else hipLaunchKernelGGL((field_kernel<PREC, WA, WB, WG, 16>), grid, dim3(kFieldBlock), 0, s, p);
 }
 
 template <int PREC>
@@ -239,9 +240,10 @@ template <int PREC, bool WA, bool WB, bool WG>
 static int blocks_per_cu_q(int split)
 {
     int b = 0;
-    hipError_t e = split == 1
-                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, field_kernel<PREC, WA, WB, WG, 1>, kFieldBlock, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, field_kernel<PREC, WA, WB, WG, 4>, kFieldBlock, 0);
+    hipError_t e =
+        split == 1   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, field_kernel<PREC, WA, WB, WG, 1>, kFieldBlock, 0)
+        : split == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, field_kernel<PREC, WA, WB, WG, 4>, kFieldBlock, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, field_kernel<PREC, WA, WB, WG, 16>, kFieldBlock, 0);
     return e == hipSuccess ? b : 0;
 }
 

@@ -59,7 +59,7 @@ void launch_gl_tables(hipStream_t s, double *x, double *w, double *cosv, double 
 // wc = w * cos for every entry of the tables (after launch_gl_tables on the same stream).
 void launch_angular_tables(hipStream_t s, const double *x, const double *w, const double *cosv, double *wc,
                            float *wcf, double *opc, float *opcf);
-// split: 1 or 4 lanes per point.
+// split: 1, 4 or 16 lanes per point.
 // Zeroes n_words 8-byte words with a kernel (no hipMemsetAsync, so callers may capture it in a graph).
 void launch_fill_zero(hipStream_t s, void *p, int64_t n_words);
 void launch_field(hipStream_t s, const FieldParams &p, int precision, int split);

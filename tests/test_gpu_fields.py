@@ -200,16 +200,19 @@ def test_argument_errors(ctx):
 
 @pytest.mark.parametrize("prec", [abi.FP64, abi.FP32C])
 def test_coil_chunking_matches_single_pass(ctx, prec):
-    """Small calls split the coils over grid rows (summed in order by field_reduce_kernel); a large call runs one
-    pass per point. Both give the same field to rounding."""
+    """Smaller calls split a point's thickness rows over 4 or 16 lanes and the coils over grid rows (summed in
+    order by field_reduce_kernel); a large call runs one lane and one pass per point. All give the same field to
+    rounding."""
     params, pts = W.group_workload(16, 600000, seed=13)
     rows, _ = H.gpu_args_rows(params)
     big, rep_big = _run(ctx, rows, pts, abi.A | abi.B | abi.G, prec)
-    small, rep_small = _run(ctx, rows, pts[:1024], abi.A | abi.B | abi.G, prec)
-    assert rep_big.split == 1 and rep_small.split == 4
+    mid, rep_mid = _run(ctx, rows, pts[:40000], abi.A | abi.B | abi.G, prec)
+    small, rep_small = _run(ctx, rows, pts[:256], abi.A | abi.B | abi.G, prec)
+    assert rep_big.split == 1 and rep_mid.split == 4 and rep_small.split == 16
     for key in ("A", "B", "G"):
-        e = H.rel_err_vec(small[key], big[key][:1024])
-        assert np.nanmax(e) < 1e-12, (key, np.nanmax(e))
+        for part in (small, mid):
+            e = H.rel_err_vec(part[key][:256], big[key][:256])
+            assert np.nanmax(e) < 1e-12, (key, np.nanmax(e))
 
 
 def test_fp32c_near_pass_is_deterministic(ctx):
