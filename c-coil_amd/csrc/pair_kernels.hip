@@ -1,14 +1,14 @@
 // Pair kernels: M, F and T of many (primary, secondary) pairs of mixed type and uneven cost in one launch.
 //
-// A work item is (pair, chunk of secondary nodes) and one wave (one 64-lane workgroup) owns one item:
-// item = blockIdx.x. Items are laid out longest-first on the host, and workgroups are dispatched in index
-// order, so the launch is an LPT schedule without a shared cursor. Each lane evaluates the primary's A or B
-// at one secondary node at a time (the primary coil is wave-uniform, so its loops and tables are scalar),
-// the weighted sums (M, F, T and the gradient of M) are reduced across the wave in FP64 and one partial is
-// written per item. A second
-// kernel adds the partials of each pair in item order, so results are deterministic. The reference instead
-// runs one thread per (configuration, secondary node) and FP32 atomicAdds per configuration
-// (MInductance.cu:157, Force.cu:172-178).
+// A work item is (pair, chunk of secondary nodes, group of the primary's thickness rows) and one wave (one
+// 64-lane workgroup) owns one item: item = blockIdx.x. Items are laid out longest-first on the host, and
+// workgroups are dispatched in index order, so the launch is an LPT schedule without a shared cursor. Each lane
+// evaluates the primary's A or B (over the item's rows) at one secondary node at a time (the primary coil is
+// wave-uniform, so its loops and tables are scalar), the weighted sums (M, F, T and the gradient of M, all linear
+// in the row sums) are reduced across the wave in FP64 and one partial is written per item. A second kernel
+// adds the partials of each pair in a fixed order, so results are deterministic. The reference instead runs one
+// thread per (configuration, secondary node) and FP32 atomicAdds per configuration (MInductance.cu:157,
+// Force.cu:172-178).
 #include "integrands.hpp"
 #include "pairs.hpp"
 
@@ -25,20 +25,21 @@ __device__ inline double wave_sum(double v)
 // nodes in FP64), 2 FP32 (reference integrands in float), 3 FP64 reference integrands, 4 FP64 with IEEE
 // division and square root.
 template <int PREC, bool WM, bool WF>
-__device__ inline void eval_prim(const CoilDev &pc, const Tables &tab, double z, double r, Local &L)
+__device__ inline void eval_prim(const CoilDev &pc, const Tables &tab, double z, double r, int row0, int rstep,
+                                 Local &L)
 {
     if constexpr (PREC == 0) {
-        eval_stable<f64s, WM, WF, false>(pc, tab, z, r, 0, 1, L);
+        eval_stable<f64s, WM, WF, false>(pc, tab, z, r, row0, rstep, L);
     } else if constexpr (PREC == 4) {
-        eval_stable<double, WM, WF, false>(pc, tab, z, r, 0, 1, L);
+        eval_stable<double, WM, WF, false>(pc, tab, z, r, row0, rstep, L);
     } else if constexpr (PREC == 2) {
-        eval_plain<float, WM, WF, false>(pc, tab, z, r, 0, 1, L);
+        eval_plain<float, WM, WF, false>(pc, tab, z, r, row0, rstep, L);
     } else if constexpr (PREC == 3) {
-        eval_plain<double, WM, WF, false>(pc, tab, z, r, 0, 1, L);
+        eval_plain<double, WM, WF, false>(pc, tab, z, r, row0, rstep, L);
     } else if (is_near(pc, z, r)) {
-        eval_stable<double, WM, WF, false>(pc, tab, z, r, 0, 1, L);
+        eval_stable<double, WM, WF, false>(pc, tab, z, r, row0, rstep, L);
     } else {
-        eval_stable<float, WM, WF, false>(pc, tab, z, r, 0, 1, L);
+        eval_stable<float, WM, WF, false>(pc, tab, z, r, row0, rstep, L);
     }
 }
 
@@ -81,7 +82,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
         const PairDev &pd = p.pairs[lo];
         const CoilDev &pc = p.prim[pd.prim];
         const CoilDev &sc = p.sec[pd.sec];
-        const int64_t n0 = (item - pd.first_item) * pd.nodes_per_item;
+        const int64_t it_pair = item - pd.first_item;
+        const int rg = (int)(it_pair % pd.row_groups);
+        const int64_t n0 = (it_pair / pd.row_groups) * pd.nodes_per_item;
         const int64_t n1 = min(n0 + (int64_t)pd.nodes_per_item, pd.n_nodes);
         const int NA = sc.blk_ang * sc.n_ang, NT = sc.blk_thk * sc.n_thk;
         const bool table_ring = sc.blk_ang == 1 && pd.half_ring == 0 && pd.ring_offset == 0.0;
@@ -116,7 +119,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
             double z, r, pcph, psph;
             to_local(pc, sc.pos[0] + arm[0], sc.pos[1] + arm[1], sc.pos[2] + arm[2], z, r, pcph, psph);
             Local L = {0, 0, 0, 0, 0, 0, 0};
-            eval_prim<PREC, WM || WD, WF || WD>(pc, p.tab, z, r, L);
+            eval_prim<PREC, WM || WD, WF || WD>(pc, p.tab, z, r, rg, pd.row_groups, L);
             if (WM) {
                 const double vl[3] = {-L.A * psph, L.A * pcph, 0.0};
                 double ag[3];
@@ -167,16 +170,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
     }
 }
 
-__global__ __launch_bounds__(256) void pair_reduce_kernel(PairParams p)
+__global__ __launch_bounds__(64) void pair_reduce_kernel(PairParams p)
 {
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= p.n_pairs) return;
+    const int k = blockIdx.x, lane = threadIdx.x;
     const PairDev &pd = p.pairs[k];
     const int64_t end = (k + 1 < p.n_pairs) ? p.pairs[k + 1].first_item : p.n_items;
     double s[kPairTerms];
     for (int q = 0; q < kPairTerms; ++q) s[q] = 0.0;
-    for (int64_t it = pd.first_item; it < end; ++it)
+    for (int64_t it = pd.first_item + lane; it < end; it += 64)
         for (int q = 0; q < kPairTerms; ++q) s[q] += p.partial[kPairTerms * it + q];
+    for (int q = 0; q < kPairTerms; ++q) s[q] = wave_sum(s[q]);
+    if (lane != 0) return;
     if (p.what & 1) p.M[pd.out] = s[0] * pd.m_scale;
     if (p.what & 2) {
         for (int q = 0; q < 3; ++q) p.F[3 * pd.out + q] = s[1 + q] * pd.f_scale;
@@ -216,7 +220,7 @@ void launch_pairs(hipStream_t s, const PairParams &p)
 void launch_pair_reduce(hipStream_t s, const PairParams &p)
 {
     if (p.n_pairs <= 0) return;
-    hipLaunchKernelGGL(pair_reduce_kernel, dim3((p.n_pairs + 255) / 256), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(pair_reduce_kernel, dim3((unsigned)p.n_pairs), dim3(64), 0, s, p);
 }
 
 }  // namespace ccoil

@@ -15,7 +15,7 @@ struct PairDev {
     int64_t n_nodes;       // secondary nodes of this pair
     double ring_offset;    // improved-precision angular offset (SupportFunctions.cxx:152-157)
     int32_t half_ring;     // 1: integrate phi over [offset, offset + pi] (SupportFunctions.cxx:86-87)
-    int32_t pad;
+    int32_t row_groups;    // the primary's thickness rows are split over this many items (rows g, g + G, ...)
     double m_scale;        // 2 pi N2 / (I1 * blocks)   (CalculateMInductanceGeneral.cxx:115-116)
     double f_scale;        // 2 pi N2 I2 / blocks, or 2 pi N2 / I1 (reference arrangement scale)
 };
@@ -34,8 +34,11 @@ struct PairParams {
 };
 constexpr int kPairTerms = 10;
 
-// One 64-lane workgroup per work item (grid = n_items, at most 2^31 - 1).
+// One 64-lane workgroup per work item (grid = n_items, at most 2^31 - 1). A pair's items are its chunks of
+// secondary nodes times its primary row groups: item i of a pair takes chunk i / row_groups and row group
+// i % row_groups.
 void launch_pairs(hipStream_t s, const PairParams &p);
+// One 64-lane workgroup per pair: lane l adds items l, l + 64, ... and a fixed butterfly combines the lanes.
 void launch_pair_reduce(hipStream_t s, const PairParams &p);
 
 }  // namespace ccoil

@@ -10,13 +10,32 @@ using namespace ccoil;
 
 namespace {
 
-// Nodes per work item: enough secondary nodes that every lane of a wave performs about 8192 primary-node
-// evaluations, at least one node per lane, at most 64.
-int nodes_per_item(int64_t prim_nodes)
+// Primary-node evaluations per lane and work item. Large batches give every lane about 8192 (each item's
+// fixed costs, the wave reduction and the partial, stay small); a batch with too little work to give the 1,024
+// SIMDs about 4 waves each at that size lowers it, down to 256, so that a single pair or a few pairs still spread
+// over the chip instead of running one long chain per lane. Powers of two only.
+int64_t lane_target(int64_t node_pairs)
 {
-    int64_t per_lane = (8192 + prim_nodes - 1) / prim_nodes;
-    per_lane = std::max<int64_t>(1, std::min<int64_t>(64, per_lane));
-    return (int)(64 * per_lane);
+    int64_t t = 8192;
+    while (t > 256 && node_pairs < (int64_t)64 * 4096 * t) t >>= 1;
+    return t;
+}
+
+// Work decomposition of one pair for a lane target: secondary nodes per item (64 lanes times nodes per lane)
+// and the number of groups the primary's thickness rows are split into. Small primaries take several secondary
+// nodes per lane; a primary with more nodes than the target splits its rows.
+void decompose(int64_t target, int n_ang, int n_thk, int &nodes_per_item, int &row_groups)
+{
+    const int64_t prim_nodes = (int64_t)n_ang * n_thk;
+    if (prim_nodes <= target) {
+        const int64_t per_lane = std::max<int64_t>(1, std::min<int64_t>(64, (target + prim_nodes - 1) / prim_nodes));
+        nodes_per_item = (int)(64 * per_lane);
+        row_groups = 1;
+    } else {
+        nodes_per_item = 64;
+        const int64_t rows_per_group = std::max<int64_t>(1, target / n_ang);
+        row_groups = (int)std::min<int64_t>(n_thk, (n_thk + rows_per_group - 1) / rows_per_group);
+    }
 }
 
 }  // namespace
@@ -58,14 +77,21 @@ extern "C" ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, cons
         if (pi < 0 || pi >= (int64_t)hp.size() || si < 0 || si >= (int64_t)hs.size())
             return fail(ctx, CCOIL_ERR_ARGUMENT, "pair " + std::to_string(k) + ": coil index out of range");
         const CoilDev &p = hp[(size_t)pi], &s = hs[(size_t)si];
+        node_pairs += (int64_t)s.blk_ang * s.n_ang * s.blk_thk * s.n_thk * s.blk_len * s.n_len * p.n_ang * p.n_thk;
+    }
+    const int64_t target = lane_target(node_pairs);
+    int max_row_groups = 1;
+    for (int64_t k = 0; k < n_pairs; ++k) {
+        const int64_t pi = prim_index ? prim_index[k] : k, si = sec_index ? sec_index[k] : k;
+        const CoilDev &p = hp[(size_t)pi], &s = hs[(size_t)si];
         PairDev d;
         std::memset(&d, 0, sizeof d);
         d.prim = (int32_t)pi;
         d.sec = (int32_t)si;
         d.out = (int32_t)k;
         d.n_nodes = (int64_t)s.blk_ang * s.n_ang * s.blk_thk * s.n_thk * s.blk_len * s.n_len;
-        const int64_t pn = (int64_t)p.n_ang * p.n_thk;
-        d.nodes_per_item = nodes_per_item(pn);
+        decompose(target, p.n_ang, p.n_thk, d.nodes_per_item, d.row_groups);
+        max_row_groups = std::max(max_row_groups, d.row_groups);
         d.ring_offset = ring_offset ? ring_offset[k] : 0.0;
         d.half_ring = half_ring ? (half_ring[k] ? 1 : 0) : 0;
         const double blocks = (double)s.blk_ang * s.blk_thk * s.blk_len;
@@ -73,14 +99,15 @@ extern "C" ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, cons
             return fail(ctx, CCOIL_ERR_ARGUMENT, "pair " + std::to_string(k) + ": primary current is zero");
         d.m_scale = 2.0 * kPi * s.turns / (p.current * blocks);
         d.f_scale = ref_scale ? 2.0 * kPi * s.turns / (p.current * blocks) : 2.0 * kPi * s.turns * s.current / blocks;
-        node_pairs += d.n_nodes * pn;
         pd[(size_t)k] = d;
     }
+    // cost of one work item: its secondary nodes times the primary nodes of its row group
     std::vector<int64_t> cost((size_t)n_pairs);
     for (int64_t k = 0; k < n_pairs; ++k) {
-        const CoilDev &p = hp[(size_t)pd[(size_t)k].prim];
-        cost[(size_t)k] = (int64_t)std::min<int64_t>(pd[(size_t)k].nodes_per_item, pd[(size_t)k].n_nodes) *
-                          p.n_ang * p.n_thk * (p.use_fast ? 2 : 1);
+        const PairDev &d = pd[(size_t)k];
+        const CoilDev &p = hp[(size_t)d.prim];
+        const int64_t rows = (p.n_thk + d.row_groups - 1) / d.row_groups;
+        cost[(size_t)k] = std::min<int64_t>(d.nodes_per_item, d.n_nodes) * p.n_ang * rows * (p.use_fast ? 2 : 1);
     }
     std::vector<int32_t> order((size_t)n_pairs);
     std::iota(order.begin(), order.end(), 0);
@@ -91,7 +118,7 @@ extern "C" ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, cons
     for (int64_t k = 0; k < n_pairs; ++k) {
         PairDev d = pd[(size_t)order[(size_t)k]];
         d.first_item = items;
-        items += (d.n_nodes + d.nodes_per_item - 1) / d.nodes_per_item;
+        items += (d.n_nodes + d.nodes_per_item - 1) / d.nodes_per_item * d.row_groups;
         sorted[(size_t)k] = d;
     }
 
@@ -158,7 +185,7 @@ extern "C" ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, cons
         report->kernel_ms = ms;
         report->main_kernel_ms = mms;
         report->block_size = 64;
-        report->split = 1;
+        report->split = max_row_groups;
     }
     return CCOIL_OK;
 }

@@ -105,7 +105,8 @@ typedef struct {
     int64_t near_pairs;        /* FP32C: (coil, point) pairs re-evaluated in FP64 */
     double kernel_ms;          /* device time of the compute kernels of this call (hipEvent) */
     int32_t block_size;        /* threads per workgroup of the main kernel */
-    int32_t split;             /* lanes cooperating on one point (field) */
+    int32_t split;             /* fields: lanes cooperating on one point; pairs: most work items one pair's
+                                  primary thickness rows were split over */
     double main_kernel_ms;     /* device time of the main kernel alone (field_kernel or pair_kernel; hipEvent) */
 } ccoil_report;
 
