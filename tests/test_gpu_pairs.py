@@ -156,3 +156,36 @@ def test_mutual_inductance_matrix_and_gradient(ctx):
           f"reciprocity max {np.max(np.abs(M - M.T)) / np.max(np.abs(M)):.3g}")
     assert np.median(e) < 1e-6 and np.percentile(e, 99) < 1e-4
     assert np.max(np.abs(M - M.T)) <= 1e-3 * np.max(np.abs(M))
+
+
+def test_row_split_matches_oracle_and_large_batch(ctx):
+    """A small batch splits large primaries' thickness rows over several work items (report.split > 1); a large
+    batch does not. Both match the oracle's GPU pair rule, and each other to rounding."""
+    rng = np.random.default_rng(41)
+    prim = W.random_coil_params(rng, 3)
+    sec = W.random_coil_params(rng, 3)
+    for p in prim:
+        p.update(type="RECTANGULAR", a=max(p["R"] * 0.3, 0.01), b=max(p["R"] * 0.4, 0.01))
+    for s in sec:
+        s["pos"] = s["pos"] * 2.0
+    pi = [(40, 20, 1)] * 3
+    si = [(24, 4 if s["type"] in ("FLAT", "RECTANGULAR") else 1, 4 if s["type"] in ("THIN", "RECTANGULAR") else 1)
+          for s in sec]
+    prow, srow = _rows(prim, pi), _rows(sec, si)
+    small, rep = ctx.pairs(abi.CoilSet.from_rows(prow), abi.CoilSet.from_rows(srow), 3, abi.PAIR_M | abi.PAIR_FT,
+                           abi.FP64)
+    assert rep.split > 1
+    for k in range(3):
+        op, os_ = H.oracle_coil(prim[k]), H.oracle_coil(sec[k])
+        m = O.mutual_inductance_gpu_rule(op, os_, pi[k], si[k])
+        F, T = O.force_torque_gpu_rule(op, os_, pi[k], si[k], 2 * np.pi * sec[k]["N"] * sec[k]["I"])
+        assert abs(small["M"][k] - m) <= 1e-10 * abs(m), (k, small["M"][k], m)
+        assert np.linalg.norm(small["F"][k] - F) <= 1e-10 * np.linalg.norm(F), k
+        assert np.linalg.norm(small["T"][k] - T) <= 1e-10 * max(np.linalg.norm(T), 1e-300), k
+    bp, bs = W.pair_workload(8192, seed=42)
+    big, rep_big = ctx.pairs(abi.CoilSet.from_rows(bp + prow), abi.CoilSet.from_rows(bs + srow), 8195,
+                             abi.PAIR_M | abi.PAIR_FT, abi.FP64)
+    assert rep_big.split == 1
+    assert np.allclose(big["M"][-3:], small["M"], rtol=1e-12, atol=0)
+    assert np.allclose(big["F"][-3:], small["F"], rtol=1e-12, atol=1e-12 * np.abs(small["F"]).max())
+    assert np.allclose(big["T"][-3:], small["T"], rtol=1e-12, atol=1e-12 * np.abs(small["T"]).max())
