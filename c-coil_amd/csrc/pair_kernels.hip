@@ -95,7 +95,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
         const double v[3] = {-sc.sb, sc.cb, 0.0};
         const int oa = table_offset(sc.n_ang), ot = table_offset(sc.n_thk), ol = table_offset(sc.n_len);
         double m = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0;
-        for (int64_t k = n0 + lane; k < n1; k += 64) {
+        // Secondary node k: weight, arm from the secondary's centre, unit tangent, and the node in the primary's
+        // local frame.
+        struct Node {
+            double w, arm[3], tan[3], z, r, pcph, psph;
+        };
+        auto node = [&](int64_t k, Node &n) {
             const int ia = (int)(k % NA);
             const int64_t rest = k / NA;
             const int it = (int)(rest % NT);
@@ -111,41 +116,92 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
             }
             const double rho = sc.R + tbs * ((it / sc.n_thk) + 0.5) + 0.5 * tbs * p.tab.x[ot + j];
             const double ell = -0.5 * sc.b + lbs * ((il / sc.n_len) + 0.5) + 0.5 * lbs * p.tab.x[ol + l];
-            const double w = 0.125 * rho * p.tab.w[ol + l] * p.tab.w[ot + j] * p.tab.w[oa + i];
-            const double arm[3] = {ell * ax[0] + rho * (cph * u[0] + sph * v[0]),
-                                   ell * ax[1] + rho * (cph * u[1] + sph * v[1]),
-                                   ell * ax[2] + rho * (cph * u[2] + sph * v[2])};
-            const double tan[3] = {-sph * u[0] + cph * v[0], -sph * u[1] + cph * v[1], -sph * u[2] + cph * v[2]};
-            double z, r, pcph, psph;
-            to_local(pc, sc.pos[0] + arm[0], sc.pos[1] + arm[1], sc.pos[2] + arm[2], z, r, pcph, psph);
-            Local L = {0, 0, 0, 0, 0, 0, 0};
-            eval_prim<PREC, WM || WD, WF || WD>(pc, p.tab, z, r, rg, pd.row_groups, L);
+            n.w = 0.125 * rho * p.tab.w[ol + l] * p.tab.w[ot + j] * p.tab.w[oa + i];
+            for (int q = 0; q < 3; ++q) {
+                n.arm[q] = ell * ax[q] + rho * (cph * u[q] + sph * v[q]);
+                n.tan[q] = -sph * u[q] + cph * v[q];
+            }
+            to_local(pc, sc.pos[0] + n.arm[0], sc.pos[1] + n.arm[1], sc.pos[2] + n.arm[2], n.z, n.r, n.pcph, n.psph);
+        };
+        // Adds the node's weighted terms: M from A, F and T from B, the gradient of M from both.
+        auto accumulate = [&](const Node &n, const Local &L) {
             if (WM) {
-                const double vl[3] = {-L.A * psph, L.A * pcph, 0.0};
+                const double vl[3] = {-L.A * n.psph, L.A * n.pcph, 0.0};
                 double ag[3];
                 mat_vec(pc.T, vl, ag);
-                m += w * (ag[0] * tan[0] + ag[1] * tan[1] + ag[2] * tan[2]);
+                m += n.w * (ag[0] * n.tan[0] + ag[1] * n.tan[1] + ag[2] * n.tan[2]);
             }
             if (WF) {
-                const double vl[3] = {L.Bh * pcph, L.Bh * psph, L.Bz};
+                const double vl[3] = {L.Bh * n.pcph, L.Bh * n.psph, L.Bz};
                 double bg[3];
                 mat_vec(pc.T, vl, bg);
-                const double df0 = w * (tan[1] * bg[2] - tan[2] * bg[1]);
-                const double df1 = w * (tan[2] * bg[0] - tan[0] * bg[2]);
-                const double df2 = w * (tan[0] * bg[1] - tan[1] * bg[0]);
+                const double df0 = n.w * (n.tan[1] * bg[2] - n.tan[2] * bg[1]);
+                const double df1 = n.w * (n.tan[2] * bg[0] - n.tan[0] * bg[2]);
+                const double df2 = n.w * (n.tan[0] * bg[1] - n.tan[1] * bg[0]);
                 f0 += df0;
                 f1 += df1;
                 f2 += df2;
-                t0 += arm[1] * df2 - arm[2] * df1;
-                t1 += arm[2] * df0 - arm[0] * df2;
-                t2 += arm[0] * df1 - arm[1] * df0;
+                t0 += n.arm[1] * df2 - n.arm[2] * df1;
+                t1 += n.arm[2] * df0 - n.arm[0] * df2;
+                t2 += n.arm[0] * df1 - n.arm[1] * df0;
             }
             if (WD) {
                 double gd[3];
-                grad_m_term(pc, L, r, pcph, psph, tan, gd);
-                d0 += w * gd[0];
-                d1 += w * gd[1];
-                d2 += w * gd[2];
+                grad_m_term(pc, L, n.r, n.pcph, n.psph, n.tan, gd);
+                d0 += n.w * gd[0];
+                d1 += n.w * gd[1];
+                d2 += n.w * gd[2];
+            }
+        };
+        if constexpr (PREC == 1) {
+            // FP32C: far nodes in FP32 as they come; nodes in the primary's near-winding shell are queued (ring of
+            // 128 in LDS, wave-uniform head and count) and evaluated in FP64 64 at a time, so the FP64 evaluation
+            // runs with full waves instead of beside the FP32 one in every wave that holds a near node.
+            __shared__ int64_t queue[128];
+            int qh = 0, qn = 0;
+            for (int64_t base = n0; base < n1; base += 64) {
+                const int64_t k = base + lane;
+                Node n;
+                bool near = false;
+                if (k < n1) {
+                    node(k, n);
+                    near = is_near(pc, n.z, n.r);
+                    if (!near) {
+                        Local L = {0, 0, 0, 0, 0, 0, 0};
+                        eval_stable<float, WM || WD, WF || WD, false>(pc, p.tab, n.z, n.r, rg, pd.row_groups, L);
+                        accumulate(n, L);
+                    }
+                }
+                const unsigned long long mask = __ballot(near);
+                if (mask) {
+                    if (near) queue[(qh + qn + __popcll(mask & ((1ull << lane) - 1ull))) & 127] = k;
+                    qn += __popcll(mask);
+                    __syncthreads();
+                    if (qn >= 64) {
+                        node(queue[(qh + lane) & 127], n);
+                        Local L = {0, 0, 0, 0, 0, 0, 0};
+                        eval_stable<double, WM || WD, WF || WD, false>(pc, p.tab, n.z, n.r, rg, pd.row_groups, L);
+                        accumulate(n, L);
+                        qh = (qh + 64) & 127;
+                        qn -= 64;
+                    }
+                    __syncthreads();
+                }
+            }
+            if (qn > 0 && lane < qn) {
+                Node n;
+                node(queue[(qh + lane) & 127], n);
+                Local L = {0, 0, 0, 0, 0, 0, 0};
+                eval_stable<double, WM || WD, WF || WD, false>(pc, p.tab, n.z, n.r, rg, pd.row_groups, L);
+                accumulate(n, L);
+            }
+        } else {
+            for (int64_t k = n0 + lane; k < n1; k += 64) {
+                Node n;
+                node(k, n);
+                Local L = {0, 0, 0, 0, 0, 0, 0};
+                eval_prim<PREC, WM || WD, WF || WD>(pc, p.tab, n.z, n.r, rg, pd.row_groups, L);
+                accumulate(n, L);
             }
         }
         double *out = p.partial + kPairTerms * item;
