@@ -22,7 +22,7 @@ __device__ inline double wave_sum(double v)
 }
 
 // PREC: 0 FP64 (stable integrands, refined hardware reciprocals), 1 FP32C (stable in FP32, near-winding
-// nodes in FP64), 2 FP32 (reference integrands in float), 3 FP64 reference integrands, 4 FP64 with IEEE
+// nodes as in FP64), 2 FP32 (reference integrands in float), 3 FP64 reference integrands, 4 FP64 with IEEE
 // division and square root.
 template <int PREC, bool WM, bool WF>
 __device__ inline void eval_prim(const CoilDev &pc, const Tables &tab, double z, double r, int row0, int rstep,
@@ -37,7 +37,7 @@ __device__ inline void eval_prim(const CoilDev &pc, const Tables &tab, double z,
     } else if constexpr (PREC == 3) {
         eval_plain<double, WM, WF, false>(pc, tab, z, r, row0, rstep, L);
     } else if (is_near(pc, z, r)) {
-        eval_stable<double, WM, WF, false>(pc, tab, z, r, row0, rstep, L);
+        eval_stable<f64s, WM, WF, false>(pc, tab, z, r, row0, rstep, L);
     } else {
         eval_stable<float, WM, WF, false>(pc, tab, z, r, row0, rstep, L);
     }
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
                     if (qn >= 64) {
                         node(queue[(qh + lane) & 127], n);
                         Local L = {0, 0, 0, 0, 0, 0, 0};
-                        eval_stable<double, WM || WD, WF || WD, false>(pc, p.tab, n.z, n.r, rg, pd.row_groups, L);
+                        eval_stable<f64s, WM || WD, WF || WD, false>(pc, p.tab, n.z, n.r, rg, pd.row_groups, L);
                         accumulate(n, L);
                         qh = (qh + 64) & 127;
                         qn -= 64;
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
                 Node n;
                 node(queue[(qh + lane) & 127], n);
                 Local L = {0, 0, 0, 0, 0, 0, 0};
-                eval_stable<double, WM || WD, WF || WD, false>(pc, p.tab, n.z, n.r, rg, pd.row_groups, L);
+                eval_stable<f64s, WM || WD, WF || WD, false>(pc, p.tab, n.z, n.r, rg, pd.row_groups, L);
                 accumulate(n, L);
             }
         } else {

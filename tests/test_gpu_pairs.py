@@ -189,3 +189,21 @@ def test_row_split_matches_oracle_and_large_batch(ctx):
     assert np.allclose(big["M"][-3:], small["M"], rtol=1e-12, atol=0)
     assert np.allclose(big["F"][-3:], small["F"], rtol=1e-12, atol=1e-12 * np.abs(small["F"]).max())
     assert np.allclose(big["T"][-3:], small["T"], rtol=1e-12, atol=1e-12 * np.abs(small["T"]).max())
+
+
+def test_fp32c_near_nodes_run_in_fp64(ctx):
+    """FP32C evaluates secondary nodes inside the primary's near-winding shell in FP64 (queued and run in full
+    waves): a secondary wound inside the primary's winding has only such nodes, so FP32C equals FP64 up to the
+    order of the sums."""
+    prim = [dict(type="RECTANGULAR", R=0.1, a=0.02, b=0.1, N=200, I=2.0, pos=np.zeros(3), alpha=0.0, beta=0.0)]
+    sec = [dict(type="THIN", R=0.105, a=0.0, b=0.05, N=50, I=1.5, pos=np.array([0.001, -0.002, 0.003]), alpha=0.02,
+                beta=0.4)]
+    prow, srow = _rows(prim, [(48, 16, 1)]), _rows(sec, [(64, 1, 24)])
+    P, S = abi.CoilSet.from_rows(prow), abi.CoilSet.from_rows(srow)
+    o64, _ = ctx.pairs(P, S, 1, abi.PAIR_M | abi.PAIR_FT, abi.FP64)
+    o32, _ = ctx.pairs(P, S, 1, abi.PAIR_M | abi.PAIR_FT, abi.FP32C)
+    assert abs(o32["M"][0] - o64["M"][0]) <= 1e-12 * abs(o64["M"][0])
+    assert np.linalg.norm(o32["F"][0] - o64["F"][0]) <= 1e-12 * np.linalg.norm(o64["F"][0])
+    # the torque of a nearly coaxial pair is small: its error is measured against the force times the radius too
+    scale_t = np.linalg.norm(o64["T"][0]) + 0.1 * np.linalg.norm(o64["F"][0])
+    assert np.linalg.norm(o32["T"][0] - o64["T"][0]) <= 1e-12 * scale_t
