@@ -239,3 +239,22 @@ def test_fp32c_near_pass_is_deterministic(ctx):
     e_near = H.rel_err_vec(first["B"], ref["B"])[near]
     e_skip = H.rel_err_vec(far_only["B"], ref["B"])[near]
     assert np.nanmedian(e_near) < np.nanmedian(e_skip)
+
+
+def test_fp32c_small_call_near_and_far_points(ctx):
+    """A small FP32C call (16 lanes per point): near-winding points are re-evaluated with the CCOIL_FP64 arithmetic
+    and match CCOIL_FP64; far points keep the FP32 error."""
+    rng = np.random.default_rng(61)
+    for typ, R, a, b in (("RECTANGULAR", 0.1, 0.05, 0.08), ("FLAT", 0.12, 0.06, 0.0), ("THIN", 0.09, 0.0, 0.1)):
+        params = [dict(type=typ, R=R, a=a, b=b, N=300, I=2.5, pos=np.array([0.05, 0.02, -0.03]), alpha=0.6, beta=1.2)]
+        rows, _ = H.gpu_args_rows(params)
+        pts = np.array(params[0]["pos"]) + rng.uniform(-0.3, 0.3, size=(200, 3))
+        near = H.near_mask(params, pts)
+        assert 0 < near.sum() < len(pts)
+        o64, _ = _run(ctx, rows, pts, abi.B | abi.G, abi.FP64)
+        o32, rep = _run(ctx, rows, pts, abi.B | abi.G, abi.FP32C)
+        assert rep.split == 16 and rep.near_pairs == int(near.sum())
+        eb = H.rel_err_vec(o32["B"], o64["B"])
+        eg = H.rel_err_vec(o32["G"], o64["G"])
+        assert np.nanmax(eb[near]) < 1e-13 and np.nanmax(eg[near]) < 1e-12, typ
+        assert np.nanmax(eb[~near]) < 1e-5 and np.nanmax(eg[~near]) < 1e-4, typ
