@@ -69,6 +69,47 @@ def test_planning_product_matches_oracle(cc):
     assert bad == 0
 
 
+def test_planning_invariants():
+    """Properties of the planning that hold whatever the details of the balancing: GPU arguments never exceed the
+    cap and use one block per dimension; dimensions a coil type does not have get one increment; more precision
+    never means fewer GPU nodes; and the measures sqrt(2 pi (R + a/2)), sqrt(2 a), sqrt(2 b) are homogeneous of one
+    degree, so scaling a coil by 4 (exact in floating point) leaves every argument unchanged. CPU arguments are
+    not monotone: the validating constructor (PrecisionArguments.cxx:36-43) replaces an increment count above 100
+    (a layer of 200 split over 2 blocks gives 101) by the default 20; a FLAT coil at PrecisionFactor 9 shows it."""
+    rng = np.random.default_rng(7)
+    bad = []
+    for R, a, b, N in _grid(rng, 300):
+        oc = O.make_coil(R, a, b, N)
+        big = O.make_coil(4 * R, 4 * a, 4 * b, N)
+        prev_gpu = 0
+        for pf in (1.0, 2.0, 3.5, 5.0, 7.0, 9.0, 12.0, 15.0):
+            cpu = list(O.coil_precision_args(oc, pf, False))
+            gpu = list(O.coil_precision_args(oc, pf, True))
+            sec = list(O.secondary_precision_args_gpu(oc, pf))
+            if gpu[:3] != [1, 1, 1] or max(gpu[3:]) > 80 or sec[:3] != [1, 1, 1] or max(sec[3:]) > 80:
+                bad.append(("cap", R, a, b, pf, gpu, sec))
+            if a == 0.0 and (cpu[1] * cpu[4] != 1 or gpu[4] != 1 or sec[4] != 1):
+                bad.append(("no thickness", R, a, b, pf, cpu, gpu, sec))
+            if b == 0.0 and sec[5] != 1:
+                bad.append(("no length", R, a, b, pf, sec))
+            if min(cpu[:3]) < 1 or min(cpu[3:]) < 1 or max(cpu[3:]) > 100:
+                bad.append(("range", R, a, b, pf, cpu))
+            n_gpu = int(np.prod(gpu[3:]))
+            if n_gpu < prev_gpu:
+                bad.append(("monotone", R, a, b, pf, gpu))
+            prev_gpu = n_gpu
+            if (list(O.coil_precision_args(big, pf, False)) != cpu or list(O.coil_precision_args(big, pf, True)) != gpu
+                    or list(O.secondary_precision_args_gpu(big, pf)) != sec):
+                bad.append(("scale", R, a, b, pf))
+    assert not bad, bad[:5]
+    # by hand at PrecisionFactor 9: 25,600 increments, measures 0.70078 (angular) and 0.45028 (thickness), linear
+    # measure sqrt(0.70078 * 0.45028 / 25600) = 3.5109e-3, layers round(199.6) = 200 and round(128.25) = 128, so
+    # 2 blocks of 200 / 2 + 1 = 101 -> 20 and 2 blocks of 128 / 2 + 1 = 65 increments
+    flat = O.make_coil(0.027467899781113806, 0.10137763355974024, 0.0, 100)
+    assert list(O.coil_precision_args(flat, 8.0, False)) == [2, 1, 1, 71, 91, 1]
+    assert list(O.coil_precision_args(flat, 9.0, False)) == [2, 2, 1, 20, 65, 1]
+
+
 def test_pair_planning_product_matches_oracle(cc):
     C = cc.coil
     rng = np.random.default_rng(1)
