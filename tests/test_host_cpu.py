@@ -108,6 +108,13 @@ def test_planning_invariants():
     flat = O.make_coil(0.027467899781113806, 0.10137763355974024, 0.0, 100)
     assert list(O.coil_precision_args(flat, 8.0, False)) == [2, 1, 1, 71, 91, 1]
     assert list(O.coil_precision_args(flat, 9.0, False)) == [2, 2, 1, 20, 65, 1]
+    # GPU spill, by hand: RECTANGULAR R = 0.5, a = b = 0.01 at PF 5: 1,600 increments, measures 1.7813 and 0.14142,
+    # angular layer round(141.96) = 142 > 80 -> 80, the rest ceil(1600 / 80) = 20 to the thickness, and the length
+    # layer round(80 * 0.14142 / 1.7813) = round(6.35) = 6
+    assert list(O.coil_precision_args(O.make_coil(0.5, 0.01, 0.01, 100), 5.0, True)) == [1, 1, 1, 80, 20, 6]
+    # secondary GPU rule, by hand: THIN R = b = 0.1 at PF 5 balances angle and length: 1,600 increments, measures
+    # 0.79267 and 0.44721, linear measure 0.014885, layers round(53.25) = 53 and round(30.04) = 30
+    assert list(O.secondary_precision_args_gpu(O.make_coil(0.1, 0.0, 0.1, 100), 5.0)) == [1, 1, 1, 53, 1, 30]
 
 
 def test_pair_planning_product_matches_oracle(cc):
