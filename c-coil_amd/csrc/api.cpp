@@ -165,6 +165,8 @@ Tables ctx_tables(ccoil_ctx ctx)
     t.wcosf = ctx->gl_wcosf;
     t.opc = ctx->gl_opc;
     t.opcf = ctx->gl_opcf;
+    t.ws = ctx->gl_ws;
+    t.ws2 = ctx->gl_ws2;
     t.rsin = ctx->gl_rsin;
     return t;
 }
@@ -204,8 +206,8 @@ ccoil_status ccoil_create(int device, ccoil_ctx *out)
     if ((e = hipEventCreate(&ctx->evm0)) != hipSuccess) return bail(e, "hipEventCreate");
     if ((e = hipEventCreate(&ctx->evm1)) != hipSuccess) return bail(e, "hipEventCreate");
     const size_t nd = sizeof(double) * kTableSize, nf = sizeof(float) * kTableSize;
-    double **dp[8] = {&ctx->gl_x, &ctx->gl_w, &ctx->gl_cos, &ctx->gl_omc, &ctx->gl_rcos, &ctx->gl_rsin, &ctx->gl_wcos,
-                      &ctx->gl_opc};
+    double **dp[10] = {&ctx->gl_x, &ctx->gl_w, &ctx->gl_cos, &ctx->gl_omc, &ctx->gl_rcos, &ctx->gl_rsin, &ctx->gl_wcos,
+                       &ctx->gl_opc, &ctx->gl_ws, &ctx->gl_ws2};
     float **fp[6] = {&ctx->gl_xf, &ctx->gl_wf, &ctx->gl_cosf, &ctx->gl_omcf, &ctx->gl_wcosf, &ctx->gl_opcf};
     for (auto p : dp)
         if ((e = hipMalloc((void **)p, nd)) != hipSuccess) return bail(e, "hipMalloc(tables)");
@@ -214,8 +216,8 @@ ccoil_status ccoil_create(int device, ccoil_ctx *out)
     launch_gl_tables(ctx->stream, ctx->gl_x, ctx->gl_w, ctx->gl_cos, ctx->gl_omc, ctx->gl_xf, ctx->gl_wf,
                      ctx->gl_cosf, ctx->gl_omcf, ctx->gl_rcos, ctx->gl_rsin);
     if ((e = hipGetLastError()) != hipSuccess) return bail(e, "gl_table_kernel launch");
-    launch_angular_tables(ctx->stream, ctx->gl_x, ctx->gl_w, ctx->gl_cos, ctx->gl_wcos, ctx->gl_wcosf, ctx->gl_opc,
-                          ctx->gl_opcf);
+    launch_angular_tables(ctx->stream, ctx->gl_x, ctx->gl_w, ctx->gl_cos, ctx->gl_omc, ctx->gl_wcos, ctx->gl_wcosf,
+                          ctx->gl_opc, ctx->gl_opcf, ctx->gl_ws, ctx->gl_ws2);
     if ((e = hipGetLastError()) != hipSuccess) return bail(e, "angular_table_kernel launch");
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return bail(e, "gl_table_kernel");
     *out = ctx;
@@ -227,8 +229,8 @@ ccoil_status ccoil_destroy(ccoil_ctx ctx)
     if (!ctx) return CCOIL_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->own_stream) (void)hipStreamSynchronize(ctx->own_stream);
-    double *dp[8] = {ctx->gl_x, ctx->gl_w, ctx->gl_cos, ctx->gl_omc, ctx->gl_rcos, ctx->gl_rsin, ctx->gl_wcos,
-                     ctx->gl_opc};
+    double *dp[10] = {ctx->gl_x, ctx->gl_w, ctx->gl_cos, ctx->gl_omc, ctx->gl_rcos, ctx->gl_rsin, ctx->gl_wcos,
+                      ctx->gl_opc, ctx->gl_ws, ctx->gl_ws2};
     float *fp[6] = {ctx->gl_xf, ctx->gl_wf, ctx->gl_cosf, ctx->gl_omcf, ctx->gl_wcosf, ctx->gl_opcf};
     for (auto p : dp)
         if (p) (void)hipFree(p);

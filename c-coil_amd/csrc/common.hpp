@@ -52,6 +52,8 @@ struct Tables {
     const float *wcosf;
     const double *opc;    // 1 + cos(pi/2 (1 + x)) = 2 cos^2(pi/4 (1 + x))
     const float *opcf;
+    const double *ws;     // w (1 - cos): the weight times the first and second power of s = 1 - cos
+    const double *ws2;    // w (1 - cos)^2
 };
 
 // Quantity bits (mirror of the C ABI enum).

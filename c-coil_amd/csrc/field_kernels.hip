@@ -178,8 +178,8 @@ void launch_gl_tables(hipStream_t s, double *x, double *w, double *cosv, double 
 
 // Derived angular tables: w cos(pi/2 (1 + x)) and 1 + cos(pi/2 (1 + x)) = 2 cos^2(pi/4 (1 + x)) (no cancellation
 // near phi = pi).
-__global__ void angular_table_kernel(const double *x, const double *w, const double *cosv, double *wc, float *wcf,
-                                     double *opc, float *opcf)
+__global__ void angular_table_kernel(const double *x, const double *w, const double *cosv, const double *omc,
+                                     double *wc, float *wcf, double *opc, float *opcf, double *ws, double *ws2)
 {
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= kTableSize) return;
@@ -188,13 +188,15 @@ __global__ void angular_table_kernel(const double *x, const double *w, const dou
     const double c = cospi(0.25 * (1.0 + x[e]));
     opc[e] = 2.0 * c * c;
     opcf[e] = (float)opc[e];
+    ws[e] = w[e] * omc[e];
+    ws2[e] = ws[e] * omc[e];
 }
 
-void launch_angular_tables(hipStream_t s, const double *x, const double *w, const double *cosv, double *wc,
-                           float *wcf, double *opc, float *opcf)
+void launch_angular_tables(hipStream_t s, const double *x, const double *w, const double *cosv, const double *omc,
+                           double *wc, float *wcf, double *opc, float *opcf, double *ws, double *ws2)
 {
-    hipLaunchKernelGGL(angular_table_kernel, dim3((kTableSize + 255) / 256), dim3(256), 0, s, x, w, cosv, wc, wcf,
-                       opc, opcf);
+    hipLaunchKernelGGL(angular_table_kernel, dim3((kTableSize + 255) / 256), dim3(256), 0, s, x, w, cosv, omc, wc,
+                       wcf, opc, opcf, ws, ws2);
 }
 
 __global__ void fill_zero_kernel(unsigned long long *p, int64_t n)

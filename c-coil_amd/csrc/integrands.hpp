@@ -431,7 +431,9 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                 // Double: the angular sums as moments in s = 1 - cos of positive terms, b_k = sum w s^k K^3 and
                 // g_k = sum w s^k K^5, combined once per row: (1 - s)(rho s - d), d + r s and the rho-z polynomial
                 // are at most quadratic in s. The combinations cancel no more than the per-node sums they replace
-                // (the cos-weighted first harmonic), and five products per node are saved.
+                // (the cos-weighted first harmonic). The weights w s and w s^2 are tabulated, so each moment is one
+                // FMA per node.
+                const double *wst = tab.ws + table_offset(c.n_ang), *ws2t = tab.ws2 + table_offset(c.n_ang);
                 T b0 = 0, b1 = 0, g0 = 0, g1 = 0, g2 = 0;
 #pragma unroll kStableUnroll<T, WA>
                 for (int i = 0; i < na; ++i) {
@@ -440,15 +442,14 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                     const T K = prsq<P>(J);
                     if (WB || WG) {
                         const T K2 = K * K;
-                        const T wK3 = w * (K2 * K);
-                        b0 += wK3;
-                        b1 = fma_<T>(wK3, s, b1);
+                        const T K3 = K2 * K;
+                        b0 = fma_<T>(w, K3, b0);
+                        b1 = fma_<T>(wst[i], K3, b1);
                         if (WG) {
-                            const T wK5 = wK3 * K2;
-                            const T t1 = wK5 * s;
-                            g0 += wK5;
-                            g1 += t1;
-                            g2 = fma_<T>(t1, s, g2);
+                            const T K5 = K3 * K2;
+                            g0 = fma_<T>(w, K5, g0);
+                            g1 = fma_<T>(wst[i], K5, g1);
+                            g2 = fma_<T>(ws2t[i], K5, g2);
                         }
                     }
                     if (WA) rA = fma_<T>(wct[i], K, rA);
