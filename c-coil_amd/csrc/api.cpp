@@ -167,6 +167,7 @@ Tables ctx_tables(ccoil_ctx ctx)
     t.opcf = ctx->gl_opcf;
     t.ws = ctx->gl_ws;
     t.ws2 = ctx->gl_ws2;
+    t.wcs = ctx->gl_wcs;
     t.rsin = ctx->gl_rsin;
     return t;
 }
@@ -206,8 +207,8 @@ ccoil_status ccoil_create(int device, ccoil_ctx *out)
     if ((e = hipEventCreate(&ctx->evm0)) != hipSuccess) return bail(e, "hipEventCreate");
     if ((e = hipEventCreate(&ctx->evm1)) != hipSuccess) return bail(e, "hipEventCreate");
     const size_t nd = sizeof(double) * kTableSize, nf = sizeof(float) * kTableSize;
-    double **dp[10] = {&ctx->gl_x, &ctx->gl_w, &ctx->gl_cos, &ctx->gl_omc, &ctx->gl_rcos, &ctx->gl_rsin, &ctx->gl_wcos,
-                       &ctx->gl_opc, &ctx->gl_ws, &ctx->gl_ws2};
+    double **dp[11] = {&ctx->gl_x, &ctx->gl_w, &ctx->gl_cos, &ctx->gl_omc, &ctx->gl_rcos, &ctx->gl_rsin, &ctx->gl_wcos,
+                       &ctx->gl_opc, &ctx->gl_ws, &ctx->gl_ws2, &ctx->gl_wcs};
     float **fp[6] = {&ctx->gl_xf, &ctx->gl_wf, &ctx->gl_cosf, &ctx->gl_omcf, &ctx->gl_wcosf, &ctx->gl_opcf};
     for (auto p : dp)
         if ((e = hipMalloc((void **)p, nd)) != hipSuccess) return bail(e, "hipMalloc(tables)");
@@ -217,7 +218,7 @@ ccoil_status ccoil_create(int device, ccoil_ctx *out)
                      ctx->gl_cosf, ctx->gl_omcf, ctx->gl_rcos, ctx->gl_rsin);
     if ((e = hipGetLastError()) != hipSuccess) return bail(e, "gl_table_kernel launch");
     launch_angular_tables(ctx->stream, ctx->gl_x, ctx->gl_w, ctx->gl_cos, ctx->gl_omc, ctx->gl_wcos, ctx->gl_wcosf,
-                          ctx->gl_opc, ctx->gl_opcf, ctx->gl_ws, ctx->gl_ws2);
+                          ctx->gl_opc, ctx->gl_opcf, ctx->gl_ws, ctx->gl_ws2, ctx->gl_wcs);
     if ((e = hipGetLastError()) != hipSuccess) return bail(e, "angular_table_kernel launch");
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return bail(e, "gl_table_kernel");
     *out = ctx;
@@ -229,8 +230,8 @@ ccoil_status ccoil_destroy(ccoil_ctx ctx)
     if (!ctx) return CCOIL_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->own_stream) (void)hipStreamSynchronize(ctx->own_stream);
-    double *dp[10] = {ctx->gl_x, ctx->gl_w, ctx->gl_cos, ctx->gl_omc, ctx->gl_rcos, ctx->gl_rsin, ctx->gl_wcos,
-                      ctx->gl_opc, ctx->gl_ws, ctx->gl_ws2};
+    double *dp[11] = {ctx->gl_x, ctx->gl_w, ctx->gl_cos, ctx->gl_omc, ctx->gl_rcos, ctx->gl_rsin, ctx->gl_wcos,
+                      ctx->gl_opc, ctx->gl_ws, ctx->gl_ws2, ctx->gl_wcs};
     float *fp[6] = {ctx->gl_xf, ctx->gl_wf, ctx->gl_cosf, ctx->gl_omcf, ctx->gl_wcosf, ctx->gl_opcf};
     for (auto p : dp)
         if (p) (void)hipFree(p);

@@ -179,7 +179,8 @@ void launch_gl_tables(hipStream_t s, double *x, double *w, double *cosv, double 
 // Derived angular tables: w cos(pi/2 (1 + x)) and 1 + cos(pi/2 (1 + x)) = 2 cos^2(pi/4 (1 + x)) (no cancellation
 // near phi = pi).
 __global__ void angular_table_kernel(const double *x, const double *w, const double *cosv, const double *omc,
-                                     double *wc, float *wcf, double *opc, float *opcf, double *ws, double *ws2)
+                                     double *wc, float *wcf, double *opc, float *opcf, double *ws, double *ws2,
+                                     double *wcs)
 {
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= kTableSize) return;
@@ -190,13 +191,14 @@ __global__ void angular_table_kernel(const double *x, const double *w, const dou
     opcf[e] = (float)opc[e];
     ws[e] = w[e] * omc[e];
     ws2[e] = ws[e] * omc[e];
+    wcs[e] = wc[e] * omc[e];
 }
 
 void launch_angular_tables(hipStream_t s, const double *x, const double *w, const double *cosv, const double *omc,
-                           double *wc, float *wcf, double *opc, float *opcf, double *ws, double *ws2)
+                           double *wc, float *wcf, double *opc, float *opcf, double *ws, double *ws2, double *wcs)
 {
     hipLaunchKernelGGL(angular_table_kernel, dim3((kTableSize + 255) / 256), dim3(256), 0, s, x, w, cosv, omc, wc,
-                       wcf, opc, opcf, ws, ws2);
+                       wcf, opc, opcf, ws, ws2, wcs);
 }
 
 __global__ void fill_zero_kernel(unsigned long long *p, int64_t n)

@@ -279,6 +279,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
     const T *wa = tab_w<T>(tab) + table_offset(c.n_ang);
     const T *wct = tab_wcos<T>(tab) + table_offset(c.n_ang);   // w cos
     const T *opt = tab_opc<T>(tab) + table_offset(c.n_ang);    // 1 + cos
+    const double *wcst = tab.wcs + table_offset(c.n_ang);        // w cos (1 - cos), double instances only
     const int na = c.n_ang, nt = c.n_thk;
     double aA = 0, aBh = 0, aBz = 0, aRR = 0, aRZ = 0, aZZ = 0;
     double hR0 = 0, hR1 = 0, ha0 = 0, ha1 = 0, hb0 = 0, hb1 = 0;   // adjoint row sums
@@ -309,6 +310,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             const T n2 = (T)(-3.0 * rho_d * rd * dd * dd);
             const T n3 = (T)(-2.0 * rho_d * rho_d * rd * rd);
             T rBh = 0, rBz = 0, rRR = 0, rRZ = 0, rZZ = 0, rA = 0;
+            T rRRs = 0, rRRc = 0;   // double: sum w cos s dL and sum w cos dL, rRR = rho rRRs - dm rRRc per row
             T eR = 0, eZ = 0;   // adjoint: end rings, sum w cos (z1 K1^3 + z2 K2^3), sum w (rho - r cos)(K1^3 + K2^3)
 #pragma unroll kStableUnroll<T, WA>
             for (int i = 0; i < na; ++i) {
@@ -354,8 +356,15 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                         // K1^3 - K2^3 = (K1 - K2)(K1^2 + K1 K2 + K2^2), the second factor as (K1 + K2)^2 - K1 K2
                         // (the subtraction removes at most a quarter: no cancellation)
                         const T dL = dK * fma_<T>(Ks, Ks, -K12);
-                        const T rcm = fma_<T>(rho, s, -dm);                 // r - rho cos
-                        rRR = fma_<T>(wc, rcm * dL, rRR);
+                        if constexpr (sizeof(T) == 8) {
+                            // r - rho cos = rho s - dm: two moments, each one FMA with a tabulated weight; their
+                            // combination per row is bounded by the same sum of magnitudes as the per-node terms
+                            rRRs = fma_<T>((T)wcst[i], dL, rRRs);
+                            rRRc = fma_<T>(wc, dL, rRRc);
+                        } else {
+                            const T rcm = fma_<T>(rho, s, -dm);             // r - rho cos
+                            rRR = fma_<T>(wc, rcm * dL, rRR);
+                        }
                         rZZ = fma_<T>(wr, dL, rZZ);
                         // hM = (z1 K1^3 - z2 K2^3) / M = dgM K1^2 + z2 K2 (K1^2 - K2^2) / M, K1^2 - K2^2 = -tP2:
                         // with z >= 0 both terms are positive when z2 < 0, and dgM carries no cancellation when
@@ -378,6 +387,8 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                 }
                 if (WA) rA = fma_<T>(wc, log1p_pos<P>(numr * nsel * Ra), rA);
             }
+            if constexpr (sizeof(T) == 8)
+                if (WG) rRR = fma_<T>(rho, rRRs, -dm * rRRc);
             const double W = c.cf * wt[j] * rho_d;
             aBh += W * (double)rBh;
             aBz += W * (double)rBz;
