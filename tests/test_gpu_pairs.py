@@ -1,6 +1,6 @@
 """GPU parity of the pair path (ccoil_pairs through the C ABI, and the c_coil drop-in API) against the FP64
 oracle. Tolerances: CCOIL_FP64 and CCOIL_FP64_IEEE relative error <= 1e-10; CCOIL_FP32C <= 3e-5 for M and |F|, |T|
-(secondary nodes in the primary's near-winding shell are evaluated in FP64 node by node; far from the primary the
+(secondary nodes in the primary's near-winding shell are evaluated with the FP64 arithmetic; far from the primary the
 FP32 node error is amplified by the angular sums, as in the field path)."""
 import numpy as np
 import pytest
