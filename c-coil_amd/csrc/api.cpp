@@ -168,6 +168,7 @@ Tables ctx_tables(ccoil_ctx ctx)
     t.ws = ctx->gl_ws;
     t.ws2 = ctx->gl_ws2;
     t.wcs = ctx->gl_wcs;
+    t.wcsf = ctx->gl_wcsf;
     t.rsin = ctx->gl_rsin;
     return t;
 }
@@ -209,7 +210,8 @@ ccoil_status ccoil_create(int device, ccoil_ctx *out)
     const size_t nd = sizeof(double) * kTableSize, nf = sizeof(float) * kTableSize;
     double **dp[11] = {&ctx->gl_x, &ctx->gl_w, &ctx->gl_cos, &ctx->gl_omc, &ctx->gl_rcos, &ctx->gl_rsin, &ctx->gl_wcos,
                        &ctx->gl_opc, &ctx->gl_ws, &ctx->gl_ws2, &ctx->gl_wcs};
-    float **fp[6] = {&ctx->gl_xf, &ctx->gl_wf, &ctx->gl_cosf, &ctx->gl_omcf, &ctx->gl_wcosf, &ctx->gl_opcf};
+    float **fp[7] = {&ctx->gl_xf, &ctx->gl_wf, &ctx->gl_cosf, &ctx->gl_omcf, &ctx->gl_wcosf, &ctx->gl_opcf,
+                     &ctx->gl_wcsf};
     for (auto p : dp)
         if ((e = hipMalloc((void **)p, nd)) != hipSuccess) return bail(e, "hipMalloc(tables)");
     for (auto p : fp)
@@ -218,7 +220,7 @@ ccoil_status ccoil_create(int device, ccoil_ctx *out)
                      ctx->gl_cosf, ctx->gl_omcf, ctx->gl_rcos, ctx->gl_rsin);
     if ((e = hipGetLastError()) != hipSuccess) return bail(e, "gl_table_kernel launch");
     launch_angular_tables(ctx->stream, ctx->gl_x, ctx->gl_w, ctx->gl_cos, ctx->gl_omc, ctx->gl_wcos, ctx->gl_wcosf,
-                          ctx->gl_opc, ctx->gl_opcf, ctx->gl_ws, ctx->gl_ws2, ctx->gl_wcs);
+                          ctx->gl_opc, ctx->gl_opcf, ctx->gl_ws, ctx->gl_ws2, ctx->gl_wcs, ctx->gl_wcsf);
     if ((e = hipGetLastError()) != hipSuccess) return bail(e, "angular_table_kernel launch");
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return bail(e, "gl_table_kernel");
     *out = ctx;
@@ -232,7 +234,7 @@ ccoil_status ccoil_destroy(ccoil_ctx ctx)
     if (ctx->own_stream) (void)hipStreamSynchronize(ctx->own_stream);
     double *dp[11] = {ctx->gl_x, ctx->gl_w, ctx->gl_cos, ctx->gl_omc, ctx->gl_rcos, ctx->gl_rsin, ctx->gl_wcos,
                       ctx->gl_opc, ctx->gl_ws, ctx->gl_ws2, ctx->gl_wcs};
-    float *fp[6] = {ctx->gl_xf, ctx->gl_wf, ctx->gl_cosf, ctx->gl_omcf, ctx->gl_wcosf, ctx->gl_opcf};
+    float *fp[7] = {ctx->gl_xf, ctx->gl_wf, ctx->gl_cosf, ctx->gl_omcf, ctx->gl_wcosf, ctx->gl_opcf, ctx->gl_wcsf};
     for (auto p : dp)
         if (p) (void)hipFree(p);
     for (auto p : fp)

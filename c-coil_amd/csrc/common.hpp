@@ -55,6 +55,7 @@ struct Tables {
     const double *ws;     // w (1 - cos): the weight times the first and second power of s = 1 - cos
     const double *ws2;    // w (1 - cos)^2
     const double *wcs;    // w cos (1 - cos)
+    const float *wcsf;
 };
 
 // Quantity bits (mirror of the C ABI enum).

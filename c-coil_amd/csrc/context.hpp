@@ -37,7 +37,7 @@ struct ccoil_ctx_s {
     double *gl_rcos = nullptr, *gl_rsin = nullptr, *gl_wcos = nullptr;
     float *gl_wcosf = nullptr;
     double *gl_opc = nullptr, *gl_ws = nullptr, *gl_ws2 = nullptr, *gl_wcs = nullptr;
-    float *gl_opcf = nullptr;
+    float *gl_opcf = nullptr, *gl_wcsf = nullptr;
     ccoil::DevBuf coils, pts, outA, outB, outE, outG, fpartial;
     ccoil::DevBuf ncounts, noffsets, nkeys, nkeys2, nrecs, nrecs2, nvals, ntemp;   // FP32C near-winding pass
     ccoil::DevBuf pprim, psec, ppairs, ppartial, pcounter, pM, pF, pT, pdM;

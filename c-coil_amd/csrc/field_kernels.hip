@@ -180,7 +180,7 @@ void launch_gl_tables(hipStream_t s, double *x, double *w, double *cosv, double 
 // near phi = pi).
 __global__ void angular_table_kernel(const double *x, const double *w, const double *cosv, const double *omc,
                                      double *wc, float *wcf, double *opc, float *opcf, double *ws, double *ws2,
-                                     double *wcs)
+                                     double *wcs, float *wcsf)
 {
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= kTableSize) return;
@@ -192,13 +192,15 @@ __global__ void angular_table_kernel(const double *x, const double *w, const dou
     ws[e] = w[e] * omc[e];
     ws2[e] = ws[e] * omc[e];
     wcs[e] = wc[e] * omc[e];
+    wcsf[e] = (float)wcs[e];
 }
 
 void launch_angular_tables(hipStream_t s, const double *x, const double *w, const double *cosv, const double *omc,
-                           double *wc, float *wcf, double *opc, float *opcf, double *ws, double *ws2, double *wcs)
+                           double *wc, float *wcf, double *opc, float *opcf, double *ws, double *ws2, double *wcs,
+                           float *wcsf)
 {
     hipLaunchKernelGGL(angular_table_kernel, dim3((kTableSize + 255) / 256), dim3(256), 0, s, x, w, cosv, omc, wc,
-                       wcf, opc, opcf, ws, ws2, wcs);
+                       wcf, opc, opcf, ws, ws2, wcs, wcsf);
 }
 
 __global__ void fill_zero_kernel(unsigned long long *p, int64_t n)

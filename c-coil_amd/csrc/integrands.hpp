@@ -248,6 +248,10 @@ template <typename T> __device__ inline const T *tab_opc(const Tables &t);
 template <> __device__ inline const double *tab_opc<double>(const Tables &t) { return t.opc; }
 template <> __device__ inline const float *tab_opc<float>(const Tables &t) { return t.opcf; }
 
+template <typename T> __device__ inline const T *tab_wcs(const Tables &t);
+template <> __device__ inline const double *tab_wcs<double>(const Tables &t) { return t.wcs; }
+template <> __device__ inline const float *tab_wcs<float>(const Tables &t) { return t.wcsf; }
+
 template <typename T> __device__ inline const T *tab_omc(const Tables &t);
 template <> __device__ inline const double *tab_omc<double>(const Tables &t) { return t.omc; }
 template <> __device__ inline const float *tab_omc<float>(const Tables &t) { return t.omcf; }
@@ -279,7 +283,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
     const T *wa = tab_w<T>(tab) + table_offset(c.n_ang);
     const T *wct = tab_wcos<T>(tab) + table_offset(c.n_ang);   // w cos
     const T *opt = tab_opc<T>(tab) + table_offset(c.n_ang);    // 1 + cos
-    const double *wcst = tab.wcs + table_offset(c.n_ang);        // w cos (1 - cos), double instances only
+    const T *wcst = tab_wcs<T>(tab) + table_offset(c.n_ang);     // w cos (1 - cos)
     const int na = c.n_ang, nt = c.n_thk;
     double aA = 0, aBh = 0, aBz = 0, aRR = 0, aRZ = 0, aZZ = 0;
     double hR0 = 0, hR1 = 0, ha0 = 0, ha1 = 0, hb0 = 0, hb1 = 0;   // adjoint row sums
@@ -309,8 +313,8 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             const T n1 = (T)(dd * dd * (2.0 * rho_d * rd - dd * dd));
             const T n2 = (T)(-3.0 * rho_d * rd * dd * dd);
             const T n3 = (T)(-2.0 * rho_d * rho_d * rd * rd);
-            T rBh = 0, rBz = 0, rRR = 0, rRZ = 0, rZZ = 0, rA = 0;
-            T rRRs = 0, rRRc = 0;   // double: sum w cos s dL and sum w cos dL, rRR = rho rRRs - dm rRRc per row
+            T rBh = 0, rBz = 0, rRZ = 0, rZZ = 0, rA = 0;
+            T rRRs = 0, rRRc = 0;   // sum w cos s dL and sum w cos dL: the row's RR is rho rRRs - dm rRRc
             T eR = 0, eZ = 0;   // adjoint: end rings, sum w cos (z1 K1^3 + z2 K2^3), sum w (rho - r cos)(K1^3 + K2^3)
 #pragma unroll kStableUnroll<T, WA>
             for (int i = 0; i < na; ++i) {
@@ -356,15 +360,11 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                         // K1^3 - K2^3 = (K1 - K2)(K1^2 + K1 K2 + K2^2), the second factor as (K1 + K2)^2 - K1 K2
                         // (the subtraction removes at most a quarter: no cancellation)
                         const T dL = dK * fma_<T>(Ks, Ks, -K12);
-                        if constexpr (sizeof(T) == 8) {
-                            // r - rho cos = rho s - dm: two moments, each one FMA with a tabulated weight; their
-                            // combination per row is bounded by the same sum of magnitudes as the per-node terms
-                            rRRs = fma_<T>((T)wcst[i], dL, rRRs);
-                            rRRc = fma_<T>(wc, dL, rRRc);
-                        } else {
-                            const T rcm = fma_<T>(rho, s, -dm);             // r - rho cos
-                            rRR = fma_<T>(wc, rcm * dL, rRR);
-                        }
+                        // r - rho cos = rho s - dm: two moments, each one FMA with a tabulated weight; their
+                        // combination per row (in double) is bounded by the same sum of magnitudes as the per-node
+                        // terms
+                        rRRs = fma_<T>(wcst[i], dL, rRRs);
+                        rRRc = fma_<T>(wc, dL, rRRc);
                         rZZ = fma_<T>(wr, dL, rZZ);
                         // hM = (z1 K1^3 - z2 K2^3) / M = dgM K1^2 + z2 K2 (K1^2 - K2^2) / M, K1^2 - K2^2 = -tP2:
                         // with z >= 0 both terms are positive when z2 < 0, and dgM carries no cancellation when
@@ -387,19 +387,18 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                 }
                 if (WA) rA = fma_<T>(wc, log1p_pos<P>(numr * nsel * Ra), rA);
             }
-            if constexpr (sizeof(T) == 8)
-                if (WG) rRR = fma_<T>(rho, rRRs, -dm * rRRc);
+            const double rowRR = WG ? fma(rho_d, (double)rRRs, -dd * (double)rRRc) : 0.0;
             const double W = c.cf * wt[j] * rho_d;
             aBh += W * (double)rBh;
             aBz += W * (double)rBz;
-            aRR += W * (double)rRR;
+            aRR += W * rowRR;
             aZZ += W * (double)rZZ;
             aRZ += W * (double)rRZ;
             aA += W * (double)rA;
             if constexpr (AD) {
                 const double Wr = c.cf * wt[j], hx = 0.5 * (1.0 + xt[j]);
                 const double e0 = 0.5 * (double)eR, e1 = 0.5 * (double)eZ;
-                const double g0 = rd * (double)rRR + zm * (double)rRZ + c.b * e0;   // -d f_rho / d rho
+                const double g0 = rd * rowRR + zm * (double)rRZ + c.b * e0;         // -d f_rho / d rho
                 const double g1 = rd * (double)rRZ + zm * (double)rZZ + c.b * e1;   // -d f_z / d rho
                 hR0 += Wr * g0;
                 hR1 += Wr * g1;

@@ -57,9 +57,10 @@ void launch_field_grad(hipStream_t s, const GradParams &p, int precision);
 void launch_gl_tables(hipStream_t s, double *x, double *w, double *cosv, double *omc, float *xf, float *wf,
                       float *cosf, float *omcf, double *rcos, double *rsin);
 // Per-entry products of the tables (after launch_gl_tables on the same stream): wc = w cos, opc = 1 + cos,
-// ws = w (1 - cos), ws2 = w (1 - cos)^2, wcs = w cos (1 - cos).
+// ws = w (1 - cos), ws2 = w (1 - cos)^2, wcs = w cos (1 - cos) (and its float copy wcsf).
 void launch_angular_tables(hipStream_t s, const double *x, const double *w, const double *cosv, const double *omc,
-                           double *wc, float *wcf, double *opc, float *opcf, double *ws, double *ws2, double *wcs);
+                           double *wc, float *wcf, double *opc, float *opcf, double *ws, double *ws2, double *wcs,
+                           float *wcsf);
 // split: 1, 4 or 16 lanes per point.
 // Zeroes n_words 8-byte words with a kernel (no hipMemsetAsync, so callers may capture it in a graph).
 void launch_fill_zero(hipStream_t s, void *p, int64_t n_words);
