@@ -284,6 +284,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
     const T *wct = tab_wcos<T>(tab) + table_offset(c.n_ang);   // w cos
     const T *opt = tab_opc<T>(tab) + table_offset(c.n_ang);    // 1 + cos
     const T *wcst = tab_wcs<T>(tab) + table_offset(c.n_ang);     // w cos (1 - cos)
+    const double *wst2 = tab.ws + table_offset(c.n_ang);         // w (1 - cos), double fast instances without G
     const int na = c.n_ang, nt = c.n_thk;
     double aA = 0, aBh = 0, aBz = 0, aRR = 0, aRZ = 0, aZZ = 0;
     double hR0 = 0, hR1 = 0, ha0 = 0, ha1 = 0, hb0 = 0, hb1 = 0;   // adjoint row sums
@@ -315,6 +316,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             const T n3 = (T)(-2.0 * rho_d * rho_d * rd * rd);
             T rBh = 0, rBz = 0, rRZ = 0, rZZ = 0, rA = 0;
             T rRRs = 0, rRRc = 0;   // sum w cos s dL and sum w cos dL: the row's RR is rho rRRs - dm rRRc
+            T rBzs = 0;             // double without the gradient: sum w s dgM (rBz then holds sum w dgM)
             T eR = 0, eZ = 0;   // adjoint: end rings, sum w cos (z1 K1^3 + z2 K2^3), sum w (rho - r cos)(K1^3 + K2^3)
 #pragma unroll kStableUnroll<T, WA>
             for (int i = 0; i < na; ++i) {
@@ -351,10 +353,20 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                     const T dK = -tP2 * S;                              // K1 - K2
                     const T invM = gsel * Rr;
                     const T dgM = (same ? tP2 * M : fma_<T>(z1, K1, -g2)) * Rr;   // (z1 K1 - z2 K2) / M
-                    const T rmc = fma_<T>(rf, s, dm);                       // rho - r cos
-                    const T wr = w * rmc;                                   // shared by B_z and dB_z/dz
                     rBh = fma_<T>(wc, -dK, rBh);       // B_rho, also the gradient's rho-phi term
-                    if (WB) rBz = fma_<T>(wr, dgM, rBz);
+                    constexpr bool kBzMoments = sizeof(T) == 8 && !WG;
+                    T wr = 0, rmc = 0;
+                    if constexpr (kBzMoments) {
+                        // without the gradient, B_z as dm sum w dgM + r sum w s dgM: two FMAs with tabulated weights
+                        if (WB) {
+                            rBz = fma_<T>(w, dgM, rBz);
+                            rBzs = fma_<T>((T)wst2[i], dgM, rBzs);
+                        }
+                    } else {
+                        rmc = fma_<T>(rf, s, dm);                           // rho - r cos
+                        wr = w * rmc;                                       // shared by B_z and dB_z/dz
+                        if (WB) rBz = fma_<T>(wr, dgM, rBz);
+                    }
                     if (WG) {
                         const T K1s = K1 * K1;
                         // K1^3 - K2^3 = (K1 - K2)(K1^2 + K1 K2 + K2^2), the second factor as (K1 + K2)^2 - K1 K2
@@ -388,9 +400,10 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                 if (WA) rA = fma_<T>(wc, log1p_pos<P>(numr * nsel * Ra), rA);
             }
             const double rowRR = WG ? fma(rho_d, (double)rRRs, -dd * (double)rRRc) : 0.0;
+            const double rowBz = (sizeof(T) == 8 && !WG) ? fma(rd, (double)rBzs, dd * (double)rBz) : (double)rBz;
             const double W = c.cf * wt[j] * rho_d;
             aBh += W * (double)rBh;
-            aBz += W * (double)rBz;
+            aBz += W * rowBz;
             aRR += W * rowRR;
             aZZ += W * (double)rZZ;
             aRZ += W * (double)rRZ;
