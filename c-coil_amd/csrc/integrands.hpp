@@ -450,6 +450,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             const T k1 = (T)(2.0 * dd * dd + 2.0 * rho_d * rd - zz);
             const T k2 = (T)(rho_d * rd);
             T rBh = 0, rBz = 0, rRR = 0, rRZ = 0, rZZ = 0, rA = 0;
+            T sRRs = 0, sRRc = 0;   // float: sum w cos s K^5 and sum w cos K^5
             if constexpr (sizeof(T) == 8) {
                 // Double: the angular sums as moments in s = 1 - cos of positive terms, b_k = sum w s^k K^3 and
                 // g_k = sum w s^k K^5, combined once per row: (1 - s)(rho s - d), d + r s and the rho-z polynomial
@@ -497,8 +498,10 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                         if (WB) rBz = fma_<T>(wr, K3, rBz);
                         if (WG) {
                             const T K5 = K3 * K2;
-                            const T rcm = fma_<T>(rho, s, -dm); // r - rho cos
-                            rRR = fma_<T>(wc, rcm * K5, rRR);
+                            // r - rho cos = rho s - dm: two moments, combined per row in double (as in the fast
+                            // method; the rho-z term keeps its per-node polynomial, whose moments lose accuracy)
+                            sRRs = fma_<T>(wcst[i], K5, sRRs);
+                            sRRc = fma_<T>(wc, K5, sRRc);
                             rZZ = fma_<T>(wr, K5, rZZ);
                             const T poly = k0 - s * fma_<T>(k2, s, k1);
                             rRZ = fma_<T>(w, poly * K5, rRZ);
@@ -507,16 +510,17 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                     if (WA) rA = fma_<T>(wc, K, rA);
                 }
             }
+            const double rowRR = sizeof(T) == 8 ? (double)rRR : fma(rho_d, (double)sRRs, -dd * (double)sRRc);
             const double W = c.cf * wt[j] * rho_d;
             aBh += W * (double)rBh;
             aBz += W * (double)rBz;
-            aRR += W * (double)rRR;
+            aRR += W * rowRR;
             aZZ += W * (double)rZZ;
             aRZ += W * (double)rRZ;
             aA += W * (double)rA;
             if constexpr (AD) {
                 const double Wr = c.cf * wt[j], hx = 0.5 * (1.0 + xt[j]);
-                const double g0 = zd * ((double)rBh - 3.0 * rd * (double)rRR + (double)rRZ);     // -d f_rho / d rho
+                const double g0 = zd * ((double)rBh - 3.0 * rd * rowRR + (double)rRZ);           // -d f_rho / d rho
                 const double g1 = (double)rBz + rd * (double)rRZ - 3.0 * zd * zd * (double)rZZ;  // -d f_z / d rho
                 hR0 += Wr * g0;
                 hR1 += Wr * g1;
