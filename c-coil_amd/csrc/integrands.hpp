@@ -284,7 +284,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
     const T *wct = tab_wcos<T>(tab) + table_offset(c.n_ang);   // w cos
     const T *opt = tab_opc<T>(tab) + table_offset(c.n_ang);    // 1 + cos
     const T *wcst = tab_wcs<T>(tab) + table_offset(c.n_ang);     // w cos (1 - cos)
-    const double *wst2 = tab.ws + table_offset(c.n_ang);         // w (1 - cos), double fast instances without G
+    const double *wsd = tab.ws + table_offset(c.n_ang);          // w (1 - cos), double fast instances without G
     const int na = c.n_ang, nt = c.n_thk;
     double aA = 0, aBh = 0, aBz = 0, aRR = 0, aRZ = 0, aZZ = 0;
     double hR0 = 0, hR1 = 0, ha0 = 0, ha1 = 0, hb0 = 0, hb1 = 0;   // adjoint row sums
@@ -360,7 +360,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                         // without the gradient, B_z as dm sum w dgM + r sum w s dgM: two FMAs with tabulated weights
                         if (WB) {
                             rBz = fma_<T>(w, dgM, rBz);
-                            rBzs = fma_<T>((T)wst2[i], dgM, rBzs);
+                            rBzs = fma_<T>((T)wsd[i], dgM, rBzs);
                         }
                     } else {
                         rmc = fma_<T>(rf, s, dm);                           // rho - r cos
