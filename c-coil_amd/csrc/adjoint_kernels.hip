@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kGradBlock) void field_grad_kernel(GradParams p)
         } else if constexpr (PREC == kFP64IEEE) {
             eval_stable<double, false, true, true, true>(c, p.tab, z, r, 0, 1, L, &ad);
         } else {
-            if (is_near(c, z, r)) eval_stable<f64s, false, true, true, true>(c, p.tab, z, r, 0, 1, L, &ad);
+            if (is_near(c, z, r, true)) eval_stable<f64s, false, true, true, true>(c, p.tab, z, r, 0, 1, L, &ad);
             else eval_stable<float, false, true, true, true>(c, p.tab, z, r, 0, 1, L, &ad);
         }
         // local Cartesian B, gradient (the same on-axis rule as the forward path) and position

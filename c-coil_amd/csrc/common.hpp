@@ -146,13 +146,19 @@ __device__ inline void accumulate_global(const CoilDev &c, const Local &L, doubl
     }
 }
 
-// Near-winding shell: distance from (r, z) to the cross-section rectangle [R, R+a] x [-b/2, b/2]
-// below one cross-section width (max(a, b), floored at 1e-3 R for filaments).
-__device__ inline bool is_near(const CoilDev &c, double z, double r)
+// Pairs that FP32C evaluates in FP64:
+//  - the near-winding shell: distance from (r, z) to the cross-section rectangle [R, R+a] x [-b/2, b/2] below one
+//    cross-section width (max(a, b), floored at 1e-3 R for filaments);
+//  - with axis (outputs that divide by r: the gradient's hoop term B_rho / r, the pair gradient's A / r), the cone
+//    and cylinder around the axis, r < 0.1 sqrt(r^2 + z^2 + R^2): the float angular sums of B_rho and A vanish like
+//    r while their rounding does not.
+constexpr double kAxisCone2 = 0.01;
+__device__ inline bool is_near(const CoilDev &c, double z, double r, bool axis)
 {
     double dr = fmax(fmax(c.R - r, r - (c.R + c.a)), 0.0);
     double dz = fmax(fabs(z) - 0.5 * c.b, 0.0);
-    return dr * dr + dz * dz < c.near_w2;
+    const double r2 = r * r;
+    return dr * dr + dz * dz < c.near_w2 || (axis && r2 < kAxisCone2 * (r2 + z * z + c.R * c.R));
 }
 
 }  // namespace ccoil

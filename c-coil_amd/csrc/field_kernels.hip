@@ -124,7 +124,7 @@ void field_kernel(FieldParams p)
         } else if constexpr (PREC == kFP32) {
             eval_plain<float, WA, WB, WG>(c, p.tab, z, r, lane, SPLIT, L);
         } else {
-            const bool skip = p.near_fp64 && is_near(c, z, r);
+            const bool skip = p.near_fp64 && is_near(c, z, r, WG);
             if (!skip) eval_stable<float, WA, WB, WG>(c, p.tab, z, r, lane, SPLIT, L);
             else ++n_near;
         }

@@ -27,7 +27,7 @@ __global__ __launch_bounds__(256) void near_fill_kernel(FieldParams p)
         const CoilDev &c = p.coils[ci];
         double z, r, cph, sph;
         to_local(c, px, py, pz, z, r, cph, sph);
-        if (is_near(c, z, r)) {   // the field kernel's own test
+        if (is_near(c, z, r, p.q & (Q_G | Q_GJ))) {   // the field kernel's own test
             p.near_keys[slot] = (uint32_t)ci;
             p.near_recs[slot] = hi | (uint64_t)slot;
             ++slot;

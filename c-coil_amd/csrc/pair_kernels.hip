@@ -36,7 +36,7 @@ __device__ inline void eval_prim(const CoilDev &pc, const Tables &tab, double z,
         eval_plain<float, WM, WF, false>(pc, tab, z, r, row0, rstep, L);
     } else if constexpr (PREC == 3) {
         eval_plain<double, WM, WF, false>(pc, tab, z, r, row0, rstep, L);
-    } else if (is_near(pc, z, r)) {
+    } else if (is_near(pc, z, r, false)) {
         eval_stable<f64s, WM, WF, false>(pc, tab, z, r, row0, rstep, L);
     } else {
         eval_stable<float, WM, WF, false>(pc, tab, z, r, row0, rstep, L);
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
                 bool near = false;
                 if (k < n1) {
                     node(k, n);
-                    near = is_near(pc, n.z, n.r);
+                    near = is_near(pc, n.z, n.r, WD);   // the position gradient divides A by r
                     if (!near) {
                         Local L = {0, 0, 0, 0, 0, 0, 0};
                         eval_stable<float, WM || WD, WF || WD, false>(pc, p.tab, n.z, n.r, rg, pd.row_groups, L);

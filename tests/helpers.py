@@ -38,6 +38,23 @@ def rel_err_vec(x, ref, floor=0.0):
     return out
 
 
+def fp64_pair_count(params, pts, width_floor=1e-3):
+    """(coil, point) pairs that CCOIL_FP32C evaluates in FP64 when the gradient is requested: the near-winding shell
+    of near_mask, and the cone and cylinder around each coil's axis, r < 0.1 sqrt(r^2 + z^2 + R^2)."""
+    n = 0
+    for p in params:
+        c = oracle_coil(p)
+        T = np.array(c.Tinv).reshape(3, 3)
+        l = (pts - np.array(p["pos"])) @ T.T
+        r = np.hypot(l[:, 0], l[:, 1])
+        z = l[:, 2]
+        dr = np.maximum(np.maximum(c.R - r, r - (c.R + c.a)), 0.0)
+        dz = np.maximum(np.abs(z) - 0.5 * c.b, 0.0)
+        w = max(c.a, c.b, width_floor * c.R)
+        n += int(np.sum((dr * dr + dz * dz < w * w) | (r * r < 0.01 * (r * r + z * z + c.R * c.R))))
+    return n
+
+
 def near_mask(params, pts, width_floor=1e-3):
     """Points within one cross-section width (max(a, b), floored at 1e-3 R) of any coil's winding."""
     near = np.zeros(len(pts), dtype=bool)
