@@ -6,7 +6,7 @@ sizes in the other test files):
         square root, the same integrands): two arithmetic paths of the same rule;
       * superposition: the group equals the sum of its two halves, to rounding of the sum of the coils' magnitudes;
       * CCOIL_FP32C against CCOIL_FP64 relative to the sum of the coils' magnitudes at each point (the
-        superposition's own conditioning), and bitwise reproducibility of FP32C (its near-winding pass has no
+        superposition's own conditioning), B and the gradient within 3e-6 at every point, and bitwise reproducibility of FP32C (its near-winding pass has no
         floating-point atomics);
   - 65,536 random pairs of mixed type with increments ~ U{4, ..., 80} (M, F, T):
       * CCOIL_FP64 against CCOIL_FP64_IEEE;
@@ -73,7 +73,8 @@ def test_group_full_size(ctx):
     print(f"FP32C vs FP64 relative to sum_k |.|: B max {cb.max():.3g} p99 {np.percentile(cb, 99):.3g}; "
           f"G max {cg.max():.3g} p99 {np.percentile(cg, 99):.3g} p99.9 {np.percentile(cg, 99.9):.3g}")
     assert cb.max() < 2e-6
-    assert np.percentile(cg, 99) < 1e-6 and np.percentile(cg, 99.9) < 2e-6
+    # the hoop term B_rho/r of pairs near a coil's axis goes to the FP64 pass, so the gradient has no 1/r blow-up
+    assert cg.max() < 3e-6 and np.percentile(cg, 99) < 1e-6 and np.percentile(cg, 99.9) < 2e-6
 
 
 def test_pairs_full_size(ctx):
