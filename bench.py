@@ -145,7 +145,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         log("bench.py measures one GPU: the field and pair paths shard by points / pairs with no exchange, and "
-            "multi-GPU runs are not part of this build. Run with --gpus 1.")
+            "this benchmark has no multi-GPU mode. Run with --gpus 1.")
         sys.exit(2)
 
     import torch

@@ -1,7 +1,7 @@
 /*
  * ccoil_oracle.c -- FP64 CPU restatement of C-Coil's algorithms (see ccoil_oracle.h).
  *
- * TEST INFRASTRUCTURE ONLY: the checker and the timed CPU baseline. The product never links it.
+ * TEST INFRASTRUCTURE ONLY: the correctness reference of the tests and the timed CPU baseline. The product never links it.
  * Built twice by oracle/Makefile: liboracle.so (IEEE, the correctness oracle) and
  * liboracle_fast.so (-O3 -mavx2 -ffast-math, the reference's own CPU flags from CMakeLists.txt:32,
  * used only as the timed CPU baseline).

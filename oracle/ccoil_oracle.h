@@ -1,7 +1,7 @@
 /*
  * ccoil_oracle.h -- FP64 CPU restatement of C-Coil's field, planning and pair algorithms.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the checker for the HIP product path: only the test
+ * TEST INFRASTRUCTURE ONLY. This library is what the tests check the HIP product path against: only the test
  * suite, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and then only to
  * compare against or to time as the CPU baseline. The product library (libccoil_hip.so and the
  * c_coil module) never links, loads or calls it.

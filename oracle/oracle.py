@@ -1,7 +1,7 @@
 """ctypes front end of the FP64 oracle (oracle/ccoil_oracle.c).
 
 TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
-import this module, and only as the checker or as the timed CPU baseline. The product (the HIP
+import this module, and only as the correctness reference or as the timed CPU baseline. The product (the HIP
 library and the c_coil module) never imports it.
 
 Each wrapper names the reference routine it restates; the C file cites file:line.
