@@ -282,7 +282,13 @@ def main():
         em = np.abs(o32["M"] - o64["M"]) / np.abs(o64["M"])
         ef, _ = rel_err(o32["F"], o64["F"])
         et, _ = rel_err(o32["T"], o64["T"])
-        pres["accuracy_fp32c_vs_fp64"] = {"M": stats(em), "F": stats(ef), "T": stats(et)}
+        # M of a random pose can be a small remainder of cancelling contributions; |M| + |grad M| |d| (grad M =
+        # F / (I1 I2), d the centre distance) is the size of M over the pose's neighbourhood
+        cur = np.array([p.get("current", 1.0) * s.get("current", 1.0) for p, s in zip(prow, srow)])
+        dist = np.linalg.norm(np.array([s["pos"] for s in srow]) - np.array([p["pos"] for p in prow]), axis=1)
+        mscale = np.abs(o64["M"]) + np.linalg.norm(o64["F"], axis=1) / np.abs(cur) * dist
+        emc = np.abs(o32["M"] - o64["M"]) / mscale
+        pres["accuracy_fp32c_vs_fp64"] = {"M": stats(em), "M_conditioned": stats(emc), "F": stats(ef), "T": stats(et)}
         pres["workload"] = (f"{len(prow)} random mixed pairs, increments ~ U{{4,...,80}} per active dimension, "
                             f"{pnodes:.3g} node-pairs, M+F+T")
         result["pairs"] = pres
