@@ -332,13 +332,23 @@ ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, const double 
         return CCOIL_OK;
     }
 
-    CC_HIP(ctx, ensure(ctx->coils, sizeof(CoilDev) * hc.size()));
-    CC_HIP(ctx, hipMemcpyAsync(ctx->coils.p, hc.data(), sizeof(CoilDev) * hc.size(), hipMemcpyHostToDevice,
-                               ctx->stream));
+    // The FP32C near pass sorts its (coil, point) records by near_key: coils with the most nodes first, so its
+    // longest waves start first. near_coil (near_key -> coil) follows the coils in the same buffer.
+    std::vector<uint32_t> near_coil(hc.size());
+    for (size_t i = 0; i < hc.size(); ++i) near_coil[i] = (uint32_t)i;
+    auto cost = [&](uint32_t i) { return (int64_t)hc[i].n_ang * hc[i].n_thk; };   // field nodes of the coil
+    std::stable_sort(near_coil.begin(), near_coil.end(), [&](uint32_t x, uint32_t y) { return cost(x) > cost(y); });
+    for (size_t k = 0; k < hc.size(); ++k) hc[near_coil[k]].near_key = (int32_t)k;
+    const size_t coil_bytes = sizeof(CoilDev) * hc.size();
+    CC_HIP(ctx, ensure(ctx->coils, coil_bytes + sizeof(uint32_t) * hc.size()));
+    CC_HIP(ctx, hipMemcpyAsync(ctx->coils.p, hc.data(), coil_bytes, hipMemcpyHostToDevice, ctx->stream));
+    CC_HIP(ctx, hipMemcpyAsync((char *)ctx->coils.p + coil_bytes, near_coil.data(), sizeof(uint32_t) * hc.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
     FieldParams p;
     std::memset(&p, 0, sizeof p);
     p.coils = (const CoilDev *)ctx->coils.p;
     p.n_coils = (int32_t)hc.size();
+    p.near_coil = (const uint32_t *)((const char *)ctx->coils.p + coil_bytes);
     p.n_pts = n;
     p.q = quantities;
     p.tab = ctx_tables(ctx);

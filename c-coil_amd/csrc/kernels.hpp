@@ -25,7 +25,8 @@ struct FieldParams {
     // point's slots in order. No floating-point atomics: the results are deterministic.
     int64_t *near_counts;
     const int64_t *near_offsets;
-    uint32_t *near_keys, *near_keys_sorted;
+    uint32_t *near_keys, *near_keys_sorted;   // keys: CoilDev::near_key, so the sort puts the costliest coils first
+    const uint32_t *near_coil;               // near_key -> coil index
     uint64_t *near_recs, *near_recs_sorted;
     double *near_vals;
     int32_t near_width;

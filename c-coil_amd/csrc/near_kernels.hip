@@ -28,7 +28,7 @@ __global__ __launch_bounds__(256) void near_fill_kernel(FieldParams p)
         double z, r, cph, sph;
         to_local(c, px, py, pz, z, r, cph, sph);
         if (is_near(c, z, r, p.q & (Q_G | Q_GJ))) {   // the field kernel's own test
-            p.near_keys[slot] = (uint32_t)ci;
+            p.near_keys[slot] = (uint32_t)c.near_key;
             p.near_recs[slot] = hi | (uint64_t)slot;
             ++slot;
         }
@@ -47,15 +47,15 @@ __global__ __launch_bounds__(256) void near_eval_kernel(FieldParams p, int64_t c
 {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= count) return;
-    const uint32_t ci = p.near_keys_sorted[e];
+    const uint32_t key = p.near_keys_sorted[e];
     const uint64_t rec = p.near_recs_sorted[e];
     const int64_t pi = (int64_t)(rec >> 32), slot = (int64_t)(rec & 0xffffffffu);
     double *o = p.near_vals + slot * p.near_width;
-    if (ci >= (uint32_t)p.n_coils) {
+    if (key >= (uint32_t)p.n_coils) {
         for (int k = 0; k < p.near_width; ++k) o[k] = 0.0;
         return;
     }
-    const CoilDev &c = p.coils[ci];
+    const CoilDev &c = p.coils[p.near_coil[key]];
     const double px = p.pts[3 * pi], py = p.pts[3 * pi + 1], pz = p.pts[3 * pi + 2];
     double z, r, cph, sph;
     to_local(c, px, py, pz, z, r, cph, sph);
