@@ -55,19 +55,20 @@ enum {
                                  from the hardware FP64 approximations refined by one Newton step (a few ulp):
                                  the accurate mode */
     CCOIL_FP32C = 1,          /* the same integrands with FP32 per-node arithmetic, FP64 per-row work and row
-                                 accumulation, FP64 re-evaluation of near-winding (coil, point) pairs */
+                                 accumulation, re-evaluation with CCOIL_FP64's arithmetic of near-winding (coil,
+                                 point) pairs and, for outputs that divide by r (the gradient, the pair position
+                                 gradient, the adjoint), of pairs near the coil's axis */
     CCOIL_FP32 = 2,           /* the reference's integrands in float (what a TYPE=float build computes) */
     CCOIL_FP64_REFERENCE = 3, /* the reference's integrands verbatim in double (a TYPE=double build): the
                                  parity mode and the plain-FP64 throughput comparator; its gradient terms
                                  lose up to ~3e-5 relative to cancellation near the winding */
-    CCOIL_FP64_IEEE = 4       /* CCOIL_FP64's integrands with IEEE division and square root (also the
-                                 evaluator of FP32C's near-winding pairs) */
+    CCOIL_FP64_IEEE = 4       /* CCOIL_FP64's integrands with IEEE division and square root */
 };
 
 /* Flags. */
 enum {
     CCOIL_DEVICE_POINTERS = 1, /* points/outputs are device pointers on the context's device */
-    CCOIL_NO_NEAR_FP64 = 2,    /* FP32C: do not re-evaluate near-winding pairs in FP64 */
+    CCOIL_NO_NEAR_FP64 = 2,    /* FP32C: do not re-evaluate near-winding or near-axis pairs in FP64 */
     CCOIL_FORCE_REFERENCE_SCALE = 4 /* ccoil_pairs: force/torque scaled by 2*pi*N2/I1 as the reference's
                                        single-pair arrangement path does (CalculateForceArrangements.cxx:125,
                                        SupportFunctions.cxx:238-241); default is the physical 2*pi*N2*I2 */
@@ -152,7 +153,7 @@ ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coi
  *   dL/d(current, R, a, b, px, py, pz, alpha, beta)   (grad, 9 per coil, coil-major)
  * of sum_x dL_dB(x) . B_c(x), with each coil's quadrature orders (n_ang, n_thk) held fixed. Needs coil current
  * and turns (B is linear in the current through J = N I / (a b)). precision: CCOIL_FP64, CCOIL_FP64_IEEE or
- * CCOIL_FP32C (near-winding pairs in FP64). Deterministic: partial sums are reduced in a fixed order. */
+ * CCOIL_FP32C (near-winding and near-axis pairs in FP64). Deterministic: partial sums are reduced in a fixed order. */
 ccoil_status ccoil_field_gradient(ccoil_ctx ctx, const ccoil_coils *coils, const double *points, int64_t n_points,
                                   const double *dL_dB, int32_t precision, uint32_t flags, double *grad,
                                   ccoil_report *report);
