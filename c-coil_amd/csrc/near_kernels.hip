@@ -51,24 +51,33 @@ __global__ __launch_bounds__(256) void near_eval_kernel(FieldParams p, int64_t c
     const uint64_t rec = p.near_recs_sorted[e];
     const int64_t pi = (int64_t)(rec >> 32), slot = (int64_t)(rec & 0xffffffffu);
     double *o = p.near_vals + slot * p.near_width;
-    if (key >= (uint32_t)p.n_coils) {
-        for (int k = 0; k < p.near_width; ++k) o[k] = 0.0;
-        return;
-    }
-    const CoilDev &c = p.coils[p.near_coil[key]];
+    // A wave holds one coil's records, or two or more at the boundaries of the sorted keys: it takes its keys one
+    // at a time (the key of the lowest lane still to do, read into a scalar register), and every lane evaluates
+    // that coil at its point, keeping the value only if the key is its own. The control flow stays uniform and no
+    // store precedes the loads, so the coil's fields and the quadrature tables are scalar loads, as in
+    // field_kernel; a divergent branch would take the same issue time. todo is wave-uniform and loses at least
+    // one lane on every pass.
     const double px = p.pts[3 * pi], py = p.pts[3 * pi + 1], pz = p.pts[3 * pi + 2];
-    double z, r, cph, sph;
-    to_local(c, px, py, pz, z, r, cph, sph);
-    Local L;
-    eval_stable<f64s, WA, WB, WG>(c, p.tab, z, r, 0, 1, L);
     Cart acc;
-    for (int k = 0; k < 3; ++k) { acc.A[k] = 0.0; acc.E[k] = 0.0; acc.B[k] = 0.0; }
-    for (int k = 0; k < 9; ++k) acc.G[k] = 0.0;
-    accumulate_global(c, L, r, cph, sph, p.q, acc);
-    if (p.A) { for (int k = 0; k < 3; ++k) o[k] = acc.A[k]; o += 3; }
-    if (p.E) { for (int k = 0; k < 3; ++k) o[k] = acc.E[k]; o += 3; }
-    if (p.B) { for (int k = 0; k < 3; ++k) o[k] = acc.B[k]; o += 3; }
-    if (p.G) for (int k = 0; k < 9; ++k) o[k] = acc.G[k];
+    for (int j = 0; j < 3; ++j) { acc.A[j] = 0.0; acc.E[j] = 0.0; acc.B[j] = 0.0; }
+    for (int j = 0; j < 9; ++j) acc.G[j] = 0.0;
+    uint64_t todo = __ballot(1);
+    while (todo) {
+        const uint32_t k = __builtin_amdgcn_readfirstlane(__shfl(key, __builtin_ctzll(todo)));
+        const bool mine = key == k;
+        todo &= ~(uint64_t)__ballot(mine);
+        if (k >= (uint32_t)p.n_coils) continue;   // inert records (sorted last): their value is zero
+        const CoilDev &c = p.coils[p.near_coil[k]];
+        double z, r, cph, sph;
+        to_local(c, px, py, pz, z, r, cph, sph);
+        Local L;
+        eval_stable<f64s, WA, WB, WG>(c, p.tab, z, r, 0, 1, L);
+        if (mine) accumulate_global(c, L, r, cph, sph, p.q, acc);
+    }
+    if (p.A) { for (int j = 0; j < 3; ++j) o[j] = acc.A[j]; o += 3; }
+    if (p.E) { for (int j = 0; j < 3; ++j) o[j] = acc.E[j]; o += 3; }
+    if (p.B) { for (int j = 0; j < 3; ++j) o[j] = acc.B[j]; o += 3; }
+    if (p.G) for (int j = 0; j < 9; ++j) o[j] = acc.G[j];
 }
 
 __global__ __launch_bounds__(256) void near_add_kernel(FieldParams p)
