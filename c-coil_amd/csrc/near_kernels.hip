@@ -87,13 +87,24 @@ __global__ __launch_bounds__(256) void near_add_kernel(FieldParams p)
     const int64_t count = p.near_counts[pi];
     if (count == 0) return;
     const int64_t e0 = p.near_offsets[pi];
+    // the point's outputs are held in registers and each slot added in turn: the same order of additions as a
+    // read-modify-write per slot, with one load and one store per output
+    double va[3] = {}, ve[3] = {}, vb[3] = {}, vg[9] = {};
+    if (p.A) for (int k = 0; k < 3; ++k) va[k] = p.A[3 * pi + k];
+    if (p.E) for (int k = 0; k < 3; ++k) ve[k] = p.E[3 * pi + k];
+    if (p.B) for (int k = 0; k < 3; ++k) vb[k] = p.B[3 * pi + k];
+    if (p.G) for (int k = 0; k < 9; ++k) vg[k] = p.G[9 * pi + k];
     for (int64_t e = e0; e < e0 + count; ++e) {
         const double *o = p.near_vals + e * p.near_width;
-        if (p.A) { for (int k = 0; k < 3; ++k) p.A[3 * pi + k] += o[k]; o += 3; }
-        if (p.E) { for (int k = 0; k < 3; ++k) p.E[3 * pi + k] += o[k]; o += 3; }
-        if (p.B) { for (int k = 0; k < 3; ++k) p.B[3 * pi + k] += o[k]; o += 3; }
-        if (p.G) for (int k = 0; k < 9; ++k) p.G[9 * pi + k] += o[k];
+        if (p.A) { for (int k = 0; k < 3; ++k) va[k] += o[k]; o += 3; }
+        if (p.E) { for (int k = 0; k < 3; ++k) ve[k] += o[k]; o += 3; }
+        if (p.B) { for (int k = 0; k < 3; ++k) vb[k] += o[k]; o += 3; }
+        if (p.G) for (int k = 0; k < 9; ++k) vg[k] += o[k];
     }
+    if (p.A) for (int k = 0; k < 3; ++k) p.A[3 * pi + k] = va[k];
+    if (p.E) for (int k = 0; k < 3; ++k) p.E[3 * pi + k] = ve[k];
+    if (p.B) for (int k = 0; k < 3; ++k) p.B[3 * pi + k] = vb[k];
+    if (p.G) for (int k = 0; k < 9; ++k) p.G[9 * pi + k] = vg[k];
 }
 
 hipError_t near_scan(hipStream_t s, const int64_t *counts, int64_t *offsets, int64_t n, void *temp,
