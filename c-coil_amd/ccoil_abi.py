@@ -17,6 +17,7 @@ A, B, E, G, G_JACOBIAN = 1, 2, 4, 8, 16
 FP64, FP32C, FP32, FP64_REFERENCE, FP64_IEEE = 0, 1, 2, 3, 4
 DEVICE_POINTERS, NO_NEAR_FP64, FORCE_REFERENCE_SCALE = 1, 2, 4
 PAIR_M, PAIR_FT, PAIR_DM = 1, 2, 4
+HIP_STREAM_LEGACY = 1   # hipStreamLegacy: the default (null) stream as a stream handle
 
 EXPORTED = ["ccoil_create", "ccoil_destroy", "ccoil_last_error", "ccoil_version", "ccoil_set_stream",
             "ccoil_synchronize", "ccoil_gauss_legendre", "ccoil_fields", "ccoil_pairs", "ccoil_field_gradient",
@@ -147,7 +148,13 @@ class Context:
             raise CCoilError(st, lib().ccoil_last_error(self.h).decode())
 
     def set_stream(self, stream_ptr):
-        self._check(lib().ccoil_set_stream(self.h, ctypes.c_void_p(stream_ptr) if stream_ptr else None))
+        """Orders the context's work on a HIP stream: None restores the context's own stream; 0 (torch's
+        cuda_stream for the default stream) is the default (null) stream, passed as hipStreamLegacy."""
+        if stream_ptr is None:
+            handle = None
+        else:
+            handle = ctypes.c_void_p(stream_ptr if stream_ptr else HIP_STREAM_LEGACY)
+        self._check(lib().ccoil_set_stream(self.h, handle))
 
     def synchronize(self):
         self._check(lib().ccoil_synchronize(self.h))

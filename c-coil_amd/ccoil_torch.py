@@ -4,6 +4,11 @@
 The quadrature orders of each coil (its PrecisionArguments) are frozen when the model is built, as the reference's
 integer planning is piecewise constant in the geometry; J = N I / (a b) is re-derived from the parameters on every
 call. Parameters and points are float64 tensors on the context's device.
+
+The library's kernels run on torch's current stream (the context is bound to it on every call), so they are ordered
+after the work that produced the points and the incoming gradient without a device-wide synchronisation. The
+parameter values are copied to the host once per forward (n_coils x 9 doubles), because the coil constants are
+formed there.
 """
 import numpy as np
 import torch
@@ -46,7 +51,7 @@ class CoilFieldB(torch.autograd.Function):
         pts = points.detach().contiguous()
         cs = model.coilset(params)
         B = torch.empty((pts.shape[0], 3), dtype=torch.float64, device=pts.device)
-        torch.cuda.synchronize(pts.device)
+        model.context.set_stream(torch.cuda.current_stream(pts.device).cuda_stream)
         model.context.fields(cs, pts, abi.B, model.precision, abi.DEVICE_POINTERS, out={"B": B})
         ctx.model, ctx.cs = model, cs
         ctx.save_for_backward(pts)
@@ -57,7 +62,7 @@ class CoilFieldB(torch.autograd.Function):
         (pts,) = ctx.saved_tensors
         g = gB.contiguous().to(torch.float64)
         grad = torch.empty((ctx.cs.n, len(PARAM_NAMES)), dtype=torch.float64, device=pts.device)
-        torch.cuda.synchronize(pts.device)
+        ctx.model.context.set_stream(torch.cuda.current_stream(pts.device).cuda_stream)
         ctx.model.context.field_gradient(ctx.cs, pts, g, ctx.model.precision, abi.DEVICE_POINTERS, out=grad)
         return grad, None, None
 

@@ -173,6 +173,110 @@ Tables ctx_tables(ccoil_ctx ctx)
     return t;
 }
 
+// Launch-shape rule of a field call. Split the thickness rows over 4 lanes when there are too few points to give
+// every SIMD of the 256 CUs several waves, and over 16 lanes when even one chunk per coil would leave the launch
+// below one wave per SIMD (few points and few coils: then each lane's serial chain of nodes, not the issue rate,
+// sets the kernel's duration; one coil at 4,096 points measured 46 us at 4 lanes and 25 us at 16, while at 65,536
+// points 16 lanes were slower). Split the coils into chunks (grid y, summed in order by field_reduce_kernel) while
+// the launch still has fewer than about 8 waves per SIMD. Large launches take 1-4 chunks, whichever makes the last
+// round of resident waves fullest: every wave does the same work, so with R rounds of resident waves the launch
+// runs ceil(R) rounds.
+FieldShape field_shape_rule(ccoil_ctx ctx, int64_t n, int64_t n_coils, uint32_t q, int precision)
+{
+    FieldShape sh;
+    const int64_t target = (int64_t)256 * 4 * 8;
+    sh.split = n < (int64_t)256 * 4 * 64 * 4 ? 4 : 1;
+    if (sh.split == 4 && (n * 4 + 63) / 64 * n_coils < (int64_t)256 * 4) sh.split = 16;
+    const int64_t waves = (n * sh.split + 63) / 64;
+    int64_t chunks = waves >= target ? 1 : std::min<int64_t>(n_coils, (target + waves - 1) / waves);
+    if (waves >= target && n * 18 < ((int64_t)1 << 31)) {   // the reduction grid stays below 2^31 threads
+        const int bpc = field_blocks_per_cu(q, precision, sh.split);
+        const double slots = (double)ctx->cu_count * bpc * (kFieldBlock / 64);
+        if (slots > 0) {
+            double best = 0.0;
+            for (int64_t c = 1; c <= std::min<int64_t>(4, n_coils); ++c) {
+                const double R = (double)waves * c / slots, eff = R / std::ceil(R);
+                if (eff > best + 0.02) {
+                    best = eff;
+                    chunks = c;
+                }
+            }
+        }
+    }
+    sh.chunks = (int)std::max<int64_t>(1, std::min<int64_t>(chunks, 65535));
+    return sh;
+}
+
+// Sizes the near pass's buffers of L for cap records and points L.p at them (the context's grow-only buffers).
+ccoil_status near_buffers(ccoil_ctx ctx, FieldLaunch &L, int64_t cap, int width)
+{
+    FieldParams &p = L.p;
+    const int64_t n = p.n_pts;
+    CC_HIP(ctx, ensure(ctx->ncounts, sizeof(int64_t) * (size_t)n));
+    CC_HIP(ctx, ensure(ctx->noffsets, sizeof(int64_t) * (size_t)n));
+    CC_HIP(ctx, ensure(ctx->nhist, 2 * sizeof(uint32_t) * ((size_t)p.n_coils + 2)));
+    CC_HIP(ctx, ensure(ctx->nstate, sizeof(NearState)));
+    size_t tb = 0;
+    CC_HIP(ctx, near_scan(ctx->stream, nullptr, nullptr, n, nullptr, tb));
+    CC_HIP(ctx, ensure(ctx->ntemp, tb > 0 ? tb : 1));
+    cap = std::max<int64_t>(cap, 1);
+    CC_HIP(ctx, ensure(ctx->nkeys, sizeof(uint32_t) * (size_t)cap));
+    CC_HIP(ctx, ensure(ctx->nrecs, sizeof(uint64_t) * (size_t)cap));
+    CC_HIP(ctx, ensure(ctx->nvals, sizeof(double) * (size_t)width * (size_t)cap));
+    ctx->near_cap = std::max(ctx->near_cap, cap);
+    p.near_fp64 = 1;
+    p.near_counts = (int64_t *)ctx->ncounts.p;
+    p.near_offsets = (const int64_t *)ctx->noffsets.p;
+    p.near_hist = (uint32_t *)ctx->nhist.p;
+    p.near_cursor = p.near_hist + ((size_t)p.n_coils + 2);
+    p.near_state = (NearState *)ctx->nstate.p;
+    p.near_cap = cap;
+    p.near_keys = (uint32_t *)ctx->nkeys.p;
+    p.near_recs = (uint64_t *)ctx->nrecs.p;
+    p.near_vals = (double *)ctx->nvals.p;
+    p.near_width = width;
+    L.scan_temp = ctx->ntemp.p;
+    L.scan_bytes = tb;
+    return CCOIL_OK;
+}
+
+// Checks the near pass's outcome after the call's work is done.
+ccoil_status near_outcome(ccoil_ctx ctx, const NearState &st)
+{
+    if (st.mismatch)
+        return fail(ctx, CCOIL_ERR_DEVICE, "FP32C near pass: the fill classified a (coil, point) pair differently "
+                                           "from the field kernel; the near contributions are incomplete");
+    return CCOIL_OK;
+}
+
+void enqueue_near_tail(hipStream_t s, const FieldLaunch &L, NearState *host_state)
+{
+    launch_near_key_scan(s, L.p);
+    launch_near_fill(s, L.p);
+    launch_near_eval(s, L.p);
+    launch_near_add(s, L.p);
+    if (host_state) (void)hipMemcpyAsync(host_state, L.p.near_state, sizeof(NearState), hipMemcpyDeviceToHost, s);
+}
+
+void enqueue_fields(hipStream_t s, const FieldLaunch &L, hipEvent_t evm0, hipEvent_t evm1, NearState *host_state)
+{
+    const FieldParams &p = L.p;
+    if (L.near) {
+        launch_fill_zero(s, p.near_counts, p.n_pts);
+        launch_fill_zero(s, p.near_hist, ((int64_t)p.n_coils + 2) / 2);
+        launch_fill_zero(s, p.near_state, sizeof(NearState) / 8);
+    }
+    if (evm0) (void)hipEventRecord(evm0, s);
+    launch_field(s, p, L.precision, L.split);
+    if (evm1) (void)hipEventRecord(evm1, s);
+    launch_field_reduce(s, p);
+    if (L.near) {
+        size_t tb = L.scan_bytes;
+        (void)near_scan(s, p.near_counts, const_cast<int64_t *>(p.near_offsets), p.n_pts, L.scan_temp, tb);
+        enqueue_near_tail(s, L, host_state);
+    }
+}
+
 }  // namespace ccoil
 
 extern "C" {
@@ -207,6 +311,8 @@ ccoil_status ccoil_create(int device, ccoil_ctx *out)
     if ((e = hipEventCreate(&ctx->ev1)) != hipSuccess) return bail(e, "hipEventCreate");
     if ((e = hipEventCreate(&ctx->evm0)) != hipSuccess) return bail(e, "hipEventCreate");
     if ((e = hipEventCreate(&ctx->evm1)) != hipSuccess) return bail(e, "hipEventCreate");
+    if ((e = hipHostMalloc((void **)&ctx->near_host, sizeof(NearState), hipHostMallocDefault)) != hipSuccess)
+        return bail(e, "hipHostMalloc");
     const size_t nd = sizeof(double) * kTableSize, nf = sizeof(float) * kTableSize;
     double **dp[11] = {&ctx->gl_x, &ctx->gl_w, &ctx->gl_cos, &ctx->gl_omc, &ctx->gl_rcos, &ctx->gl_rsin, &ctx->gl_wcos,
                        &ctx->gl_opc, &ctx->gl_ws, &ctx->gl_ws2, &ctx->gl_wcs};
@@ -240,10 +346,11 @@ ccoil_status ccoil_destroy(ccoil_ctx ctx)
     for (auto p : fp)
         if (p) (void)hipFree(p);
     DevBuf *bufs[] = {&ctx->coils, &ctx->fpartial, &ctx->pts, &ctx->outA, &ctx->outB, &ctx->outE, &ctx->outG,
-                      &ctx->ncounts, &ctx->noffsets, &ctx->nkeys, &ctx->nkeys2, &ctx->nrecs, &ctx->nrecs2,
+                      &ctx->ncounts, &ctx->noffsets, &ctx->nhist, &ctx->nstate, &ctx->nkeys, &ctx->nrecs,
                       &ctx->nvals, &ctx->ntemp, &ctx->pprim, &ctx->psec, &ctx->ppairs, &ctx->ppartial, &ctx->pcounter, &ctx->pM, &ctx->pF, &ctx->pT, &ctx->pdM, &ctx->acoils, &ctx->apts, &ctx->agB,
                       &ctx->apartial, &ctx->agrad, &ctx->zitems, &ctx->zpartial, &ctx->zout};
     for (auto b : bufs) free_buf(*b);
+    if (ctx->near_host) (void)hipHostFree(ctx->near_host);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->evm0) (void)hipEventDestroy(ctx->evm0);
@@ -344,7 +451,8 @@ ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, const double 
     CC_HIP(ctx, hipMemcpyAsync(ctx->coils.p, hc.data(), coil_bytes, hipMemcpyHostToDevice, ctx->stream));
     CC_HIP(ctx, hipMemcpyAsync((char *)ctx->coils.p + coil_bytes, near_coil.data(), sizeof(uint32_t) * hc.size(),
                                hipMemcpyHostToDevice, ctx->stream));
-    FieldParams p;
+    FieldLaunch L;
+    FieldParams &p = L.p;
     std::memset(&p, 0, sizeof p);
     p.coils = (const CoilDev *)ctx->coils.p;
     p.n_coils = (int32_t)hc.size();
@@ -367,111 +475,63 @@ ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, const double 
         if (quantities & CCOIL_E) { CC_HIP(ctx, ensure(ctx->outE, s3)); p.E = (double *)ctx->outE.p; }
         if (quantities & (CCOIL_G | CCOIL_G_JACOBIAN)) { CC_HIP(ctx, ensure(ctx->outG, s9)); p.G = (double *)ctx->outG.p; }
     }
-
-    // Launch shape: split the thickness rows over 4 lanes when there are too few points to give every
-    // SIMD of the 256 CUs several waves, and over 16 lanes when even one chunk per coil would leave the launch
-    // below one wave per SIMD (few points and few coils: then each lane's serial chain of nodes, not the issue
-    // rate, sets the kernel's duration; one coil at 4,096 points measured 46 us at 4 lanes and 25 us at 16, while
-    // at 65,536 points 16 lanes were slower). Split the coils into chunks (grid y, summed in order by
-    // field_reduce_kernel) while the launch still has fewer than about 8 waves per SIMD. Large launches take
-    // 1-4 chunks, whichever makes the last round of resident waves fullest: every wave does the same work, so
-    // with R rounds of resident waves the launch runs ceil(R) rounds.
-    const int64_t target = (int64_t)256 * 4 * 8;
-    int split = n < (int64_t)256 * 4 * 64 * 4 ? 4 : 1;
-    if (split == 4 && (n * 4 + 63) / 64 * (int64_t)hc.size() < (int64_t)256 * 4) split = 16;
-    {
-        const int64_t waves = (n * split + 63) / 64;
-        int64_t chunks = waves >= target ? 1 : std::min<int64_t>((int64_t)hc.size(), (target + waves - 1) / waves);
-        if (waves >= target && n * 18 < ((int64_t)1 << 31)) {   // the reduction grid stays below 2^31 threads
-            const int bpc = field_blocks_per_cu(quantities, precision, split);
-            const double slots = (double)ctx->cu_count * bpc * (kFieldBlock / 64);
-            if (slots > 0) {
-                double best = 0.0;
-                for (int64_t c = 1; c <= std::min<int64_t>(4, (int64_t)hc.size()); ++c) {
-                    const double R = (double)waves * c / slots, eff = R / std::ceil(R);
-                    if (eff > best + 0.02) {
-                        best = eff;
-                        chunks = c;
-                    }
-                }
-            }
-        }
-        chunks = std::max<int64_t>(1, std::min<int64_t>(chunks, 65535));
-        const int64_t cpc = ((int64_t)hc.size() + chunks - 1) / chunks;
-        p.coils_per_chunk = (int32_t)cpc;
-        p.n_chunks = (int32_t)(((int64_t)hc.size() + cpc - 1) / cpc);
-        if (p.n_chunks > 1) {
-            p.partial_width = (p.A ? 3 : 0) + (p.E ? 3 : 0) + (p.B ? 3 : 0) + (p.G ? 9 : 0);
-            CC_HIP(ctx, ensure(ctx->fpartial, sizeof(double) * p.partial_width * (size_t)n * (size_t)p.n_chunks));
-            p.partial = (double *)ctx->fpartial.p;
-        }
+    const int width = (p.A ? 3 : 0) + (p.E ? 3 : 0) + (p.B ? 3 : 0) + (p.G ? 9 : 0);
+    const FieldShape shape = field_shape_rule(ctx, n, (int64_t)hc.size(), quantities, precision);
+    L.precision = precision;
+    L.split = shape.split;
+    p.coils_per_chunk = (int32_t)(((int64_t)hc.size() + shape.chunks - 1) / shape.chunks);
+    p.n_chunks = (int32_t)(((int64_t)hc.size() + p.coils_per_chunk - 1) / p.coils_per_chunk);
+    if (p.n_chunks > 1) {
+        p.partial_width = width;
+        CC_HIP(ctx, ensure(ctx->fpartial, sizeof(double) * p.partial_width * (size_t)n * (size_t)p.n_chunks));
+        p.partial = (double *)ctx->fpartial.p;
     }
-    int64_t near = 0;
-    const bool near_fp64 = precision == CCOIL_FP32C && !(flags & CCOIL_NO_NEAR_FP64);
+    L.near = precision == CCOIL_FP32C && !(flags & CCOIL_NO_NEAR_FP64);
+    if (L.near) {
+        // Buffers for the records: every (coil, point) pair when that is small, else a capacity kept from earlier
+        // calls (at least two per point); a call that finds more grows them and runs the near pass again.
+        const int64_t all_pairs = n * (int64_t)hc.size();
+        int64_t cap = std::max<int64_t>(ctx->near_cap, std::min<int64_t>(all_pairs, 2 * n + 65536));
+        cap = std::min<int64_t>(cap, INT32_MAX);
+        ccoil_status st = near_buffers(ctx, L, cap, width);
+        if (st != CCOIL_OK) return st;
+    }
     CC_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    if (near_fp64) {
-        p.near_fp64 = 1;
-        CC_HIP(ctx, ensure(ctx->ncounts, sizeof(int64_t) * (size_t)n));
-        p.near_counts = (int64_t *)ctx->ncounts.p;
-        launch_fill_zero(ctx->stream, p.near_counts, n);
-    }
-    CC_HIP(ctx, hipEventRecord(ctx->evm0, ctx->stream));
-    launch_field(ctx->stream, p, precision, split);
+    enqueue_fields(ctx->stream, L, ctx->evm0, ctx->evm1, L.near ? ctx->near_host : nullptr);
     CC_HIP(ctx, hipGetLastError());
-    CC_HIP(ctx, hipEventRecord(ctx->evm1, ctx->stream));
-    launch_field_reduce(ctx->stream, p);
-    CC_HIP(ctx, hipGetLastError());
-    if (near_fp64) {
-        // offsets of each point's near slots, then the (coil, point | slot) records in (point, coil) order, sorted
-        // by coil, their FP64 values, and the in-order sums (field_kernel counted the near pairs it skipped)
-        CC_HIP(ctx, ensure(ctx->noffsets, sizeof(int64_t) * (size_t)n));
-        p.near_offsets = (const int64_t *)ctx->noffsets.p;
-        size_t tb = 0;
-        CC_HIP(ctx, near_scan(ctx->stream, p.near_counts, (int64_t *)ctx->noffsets.p, n, nullptr, tb));
-        CC_HIP(ctx, ensure(ctx->ntemp, tb > 0 ? tb : 1));
-        CC_HIP(ctx, near_scan(ctx->stream, p.near_counts, (int64_t *)ctx->noffsets.p, n, ctx->ntemp.p, tb));
-        int64_t last[2] = {0, 0};
-        CC_HIP(ctx, hipMemcpyAsync(&last[0], p.near_offsets + (n - 1), sizeof(int64_t), hipMemcpyDeviceToHost,
-                                   ctx->stream));
-        CC_HIP(ctx, hipMemcpyAsync(&last[1], p.near_counts + (n - 1), sizeof(int64_t), hipMemcpyDeviceToHost,
-                                   ctx->stream));
-        CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        near = last[0] + last[1];
-        if (near > (int64_t)INT32_MAX)
-            return fail(ctx, CCOIL_ERR_OUT_OF_MEMORY, "FP32C: " + std::to_string(near) +
-                        " near-winding (coil, point) pairs exceed the near pass's limit of 2^31 - 1; split the points");
-        if (near > 0) {
-            p.near_width = (p.A ? 3 : 0) + (p.E ? 3 : 0) + (p.B ? 3 : 0) + (p.G ? 9 : 0);
-            CC_HIP(ctx, ensure(ctx->nkeys, sizeof(uint32_t) * (size_t)near));
-            CC_HIP(ctx, ensure(ctx->nkeys2, sizeof(uint32_t) * (size_t)near));
-            CC_HIP(ctx, ensure(ctx->nrecs, sizeof(uint64_t) * (size_t)near));
-            CC_HIP(ctx, ensure(ctx->nrecs2, sizeof(uint64_t) * (size_t)near));
-            CC_HIP(ctx, ensure(ctx->nvals, sizeof(double) * (size_t)p.near_width * (size_t)near));
-            p.near_keys = (uint32_t *)ctx->nkeys.p;
-            p.near_keys_sorted = (uint32_t *)ctx->nkeys2.p;
-            p.near_recs = (uint64_t *)ctx->nrecs.p;
-            p.near_recs_sorted = (uint64_t *)ctx->nrecs2.p;
-            p.near_vals = (double *)ctx->nvals.p;
-            launch_near_fill(ctx->stream, p);
-            CC_HIP(ctx, hipGetLastError());
-            size_t sb = 0;
-            CC_HIP(ctx, near_sort(ctx->stream, p, near, nullptr, sb));
-            CC_HIP(ctx, ensure(ctx->ntemp, sb > 0 ? sb : 1));
-            CC_HIP(ctx, near_sort(ctx->stream, p, near, ctx->ntemp.p, sb));
-            launch_near_eval(ctx->stream, p, near);
-            CC_HIP(ctx, hipGetLastError());
-            launch_near_add(ctx->stream, p);
-            CC_HIP(ctx, hipGetLastError());
-        }
-    }
     CC_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    if (!dev) {
+    auto copy_out = [&]() -> ccoil_status {
+        if (dev) return CCOIL_OK;
         if (p.A) CC_HIP(ctx, hipMemcpyAsync(A, p.A, s3, hipMemcpyDeviceToHost, ctx->stream));
         if (p.B) CC_HIP(ctx, hipMemcpyAsync(B, p.B, s3, hipMemcpyDeviceToHost, ctx->stream));
         if (p.E) CC_HIP(ctx, hipMemcpyAsync(E, p.E, s3, hipMemcpyDeviceToHost, ctx->stream));
         if (p.G) CC_HIP(ctx, hipMemcpyAsync(G, p.G, s9, hipMemcpyDeviceToHost, ctx->stream));
-    }
+        return CCOIL_OK;
+    };
+    ccoil_status st = copy_out();
+    if (st != CCOIL_OK) return st;
     CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int64_t near = 0;
+    if (L.near) {
+        st = near_outcome(ctx, *ctx->near_host);
+        if (st != CCOIL_OK) return st;
+        near = (int64_t)ctx->near_host->total;
+        if (ctx->near_host->overflow) {
+            // more near pairs than the buffers hold: the near pass did nothing; grow and run it again
+            st = near_buffers(ctx, L, std::min<int64_t>(near + near / 8 + 4096, INT32_MAX), width);
+            if (st != CCOIL_OK) return st;
+            enqueue_near_tail(ctx->stream, L, ctx->near_host);
+            CC_HIP(ctx, hipGetLastError());
+            CC_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+            st = copy_out();
+            if (st != CCOIL_OK) return st;
+            CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            st = near_outcome(ctx, *ctx->near_host);
+            if (st != CCOIL_OK) return st;
+            if (ctx->near_host->overflow)
+                return fail(ctx, CCOIL_ERR_DEVICE, "FP32C near pass: capacity still exceeded after growing");
+        }
+    }
     if (report) {
         float ms = 0.f, mms = 0.f;
         CC_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -483,7 +543,7 @@ ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, const double 
         report->kernel_ms = ms;
         report->main_kernel_ms = mms;
         report->block_size = kFieldBlock;
-        report->split = split;
+        report->split = L.split;
     }
     return CCOIL_OK;
 }

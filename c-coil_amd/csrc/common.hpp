@@ -79,12 +79,16 @@ __device__ inline void mat_vec(const double *M, const double *v, double *o)
 }
 
 // Point -> coil-local cylindrical (z, r) and the azimuth's cos/sin (cos(atan2(y,x)) = x/r).
+// Every operation is rounded on its own (no FMA contraction), so each kernel that classifies a (coil, point) pair
+// with is_near gets bitwise the same z and r from the same inputs, whatever the inlining context.
 __device__ inline void to_local(const CoilDev &c, double px, double py, double pz, double &z, double &r,
                                 double &cph, double &sph)
 {
-    double d[3] = {px - c.pos[0], py - c.pos[1], pz - c.pos[2]};
-    double l[3];
-    mat_vec(c.Tinv, d, l);
+#pragma clang fp contract(off)
+    const double d[3] = {px - c.pos[0], py - c.pos[1], pz - c.pos[2]};
+    const double *M = c.Tinv;   // written out here rather than through mat_vec, so the pragma covers it
+    const double l[3] = {M[0] * d[0] + M[1] * d[1] + M[2] * d[2], M[3] * d[0] + M[4] * d[1] + M[5] * d[2],
+                         M[6] * d[0] + M[7] * d[1] + M[8] * d[2]};
     z = l[2];
     r = sqrt(l[0] * l[0] + l[1] * l[1]);
     if (r > 0.0) {
@@ -155,6 +159,7 @@ __device__ inline void accumulate_global(const CoilDev &c, const Local &L, doubl
 constexpr double kAxisCone2 = 0.01;
 __device__ inline bool is_near(const CoilDev &c, double z, double r, bool axis)
 {
+#pragma clang fp contract(off)
     double dr = fmax(fmax(c.R - r, r - (c.R + c.a)), 0.0);
     double dz = fmax(fabs(z) - 0.5 * c.b, 0.0);
     const double r2 = r * r;

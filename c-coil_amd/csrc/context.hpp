@@ -39,7 +39,9 @@ struct ccoil_ctx_s {
     double *gl_opc = nullptr, *gl_ws = nullptr, *gl_ws2 = nullptr, *gl_wcs = nullptr;
     float *gl_opcf = nullptr, *gl_wcsf = nullptr;
     ccoil::DevBuf coils, pts, outA, outB, outE, outG, fpartial;
-    ccoil::DevBuf ncounts, noffsets, nkeys, nkeys2, nrecs, nrecs2, nvals, ntemp;   // FP32C near-winding pass
+    ccoil::DevBuf ncounts, noffsets, nhist, nstate, nkeys, nrecs, nvals, ntemp;   // FP32C near-winding pass
+    int64_t near_cap = 0;                         // records the near buffers hold (grow-only)
+    ccoil::NearState *near_host = nullptr;        // pinned copy of the near pass's outcome
     ccoil::DevBuf pprim, psec, ppairs, ppartial, pcounter, pM, pF, pT, pdM;
     ccoil::DevBuf acoils, apts, agB, apartial, agrad;   // field adjoint
     ccoil::DevBuf zitems, zpartial, zout;               // coaxial rules
@@ -48,6 +50,30 @@ struct ccoil_ctx_s {
 };
 
 namespace ccoil {
+
+// Launch shape of a field call: lanes per point (1, 4 or 16) and coil chunks (grid rows summed in order).
+struct FieldShape {
+    int split = 1;
+    int chunks = 1;
+};
+// The fixed rule (few points: split rows over lanes; few waves: chunk the coils; large: fill the last round).
+FieldShape field_shape_rule(ccoil_ctx ctx, int64_t n_points, int64_t n_coils, uint32_t q, int precision);
+
+// Everything a field call launches, with its buffers already sized.
+struct FieldLaunch {
+    FieldParams p;
+    int precision = 0;
+    int split = 1;
+    bool near = false;          // FP32C near pass
+    void *scan_temp = nullptr;  // hipCUB scan storage of the near pass
+    size_t scan_bytes = 0;
+};
+// Enqueues the field kernel, the chunk reduction and (FP32C) the whole near pass on s, with no host
+// synchronisation; evm0/evm1 (may be NULL) bracket the field kernel. The near pass's outcome is copied to
+// host_state (pinned) when given.
+void enqueue_fields(hipStream_t s, const FieldLaunch &L, hipEvent_t evm0, hipEvent_t evm1, NearState *host_state);
+// The near pass from the per-coil cursors on (after a grow-on-overflow retry): key scan, fill, eval, add.
+void enqueue_near_tail(hipStream_t s, const FieldLaunch &L, NearState *host_state);
 
 ccoil_status fail(ccoil_ctx ctx, ccoil_status code, const std::string &msg);
 Tables ctx_tables(ccoil_ctx ctx);

@@ -9,6 +9,13 @@ namespace ccoil {
 enum { kFP64 = 0, kFP32C = 1, kFP32 = 2, kFP64Reference = 3, kFP64IEEE = 4 };
 constexpr int kFieldBlock = 256;
 
+// Device-side outcome of the FP32C near pass (read back by the host once the call's work is done).
+struct NearState {
+    unsigned long long total;   // near (coil, point) pairs counted by field_kernel
+    unsigned int overflow;      // total > near_cap: the pass was skipped
+    unsigned int mismatch;      // near_fill classified a pair differently from field_kernel
+};
+
 struct FieldParams {
     const CoilDev *coils;
     int32_t n_coils;
@@ -18,16 +25,26 @@ struct FieldParams {
     uint32_t q;            // Q_* mask
     Tables tab;
     int32_t near_fp64;     // FP32C: skip near pairs here, re-evaluate in FP64
-    // FP32C near-winding pass: field_kernel counts each point's near coils (near_counts); an exclusive scan gives
-    // near_offsets; near_fill writes per slot, in point-then-coil order, the coil (near_keys) and point << 32 |
-    // slot (near_recs); a radix sort by coil gives near_keys_sorted / near_recs_sorted; near_eval writes each
-    // slot's FP64 contribution (near_width doubles: the requested A, E, B, G in that order); near_add adds a
-    // point's slots in order. No floating-point atomics: the results are deterministic.
+    // FP32C near-winding pass, stream-ordered with no host readback (so a caller may capture it in a graph):
+    // field_kernel counts each point's near coils (near_counts) and each coil's near points (near_hist, indexed by
+    // CoilDev::near_key, so the costliest coils come first); an exclusive scan of near_counts gives each point's
+    // slots (near_offsets); near_key_scan turns near_hist into per-coil cursors, and stores the total and whether
+    // it exceeds near_cap in near_state; near_fill places one record per (coil, point) pair at its coil's cursor:
+    // the coil key (near_keys) and point << 32 | slot (near_recs); near_eval writes each record's FP64
+    // contribution into its slot of near_vals (near_width doubles: the requested A, E, B, G in that order);
+    // near_add adds a point's slots in slot (coil) order. The order of records within a coil varies from run to
+    // run, but a slot's value depends only on its (coil, point) and every sum has a fixed order, so FP32C results
+    // are bitwise reproducible. When the total exceeds near_cap, near_fill, near_eval and near_add do nothing and
+    // the host grows the buffers and runs them again.
     int64_t *near_counts;
     const int64_t *near_offsets;
-    uint32_t *near_keys, *near_keys_sorted;   // keys: CoilDev::near_key, so the sort puts the costliest coils first
+    uint32_t *near_hist;      // [n_coils + 1] counts per near_key
+    uint32_t *near_cursor;    // [n_coils + 1] next free record of each key
+    NearState *near_state;
+    int64_t near_cap;
+    uint32_t *near_keys;
     const uint32_t *near_coil;               // near_key -> coil index
-    uint64_t *near_recs, *near_recs_sorted;
+    uint64_t *near_recs;
     double *near_vals;
     int32_t near_width;
     // Coil chunks (grid y): each block row sums its chunk of coils into partial ([chunk][point][width]: the
@@ -72,10 +89,10 @@ int field_blocks_per_cu(uint32_t q, int precision, int split);
 // Exclusive prefix sum of n counts into offsets; with temp == NULL only sets temp_bytes.
 hipError_t near_scan(hipStream_t s, const int64_t *counts, int64_t *offsets, int64_t n, void *temp,
                      size_t &temp_bytes);
-// Sorts count near records by coil (stable: point order within a coil); with temp == NULL only sets temp_bytes.
-hipError_t near_sort(hipStream_t s, const FieldParams &p, int64_t count, void *temp, size_t &temp_bytes);
+// near_key_scan: per-coil cursors from near_hist, and near_state's total and overflow (one small kernel).
+void launch_near_key_scan(hipStream_t s, const FieldParams &p);
 void launch_near_fill(hipStream_t s, const FieldParams &p);
-void launch_near_eval(hipStream_t s, const FieldParams &p, int64_t count);
+void launch_near_eval(hipStream_t s, const FieldParams &p);
 void launch_near_add(hipStream_t s, const FieldParams &p);
 
 }  // namespace ccoil

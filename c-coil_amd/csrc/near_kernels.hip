@@ -1,11 +1,14 @@
-// FP32C near-winding pass (gfx950): (coil, point) pairs inside a coil's near shell are re-evaluated in FP64.
+// FP32C near-winding pass (gfx950): (coil, point) pairs inside a coil's near shell (and, for outputs that divide by
+// r, its axis cone) are re-evaluated in FP64.
 //
-// field_kernel<kFP32C> skips those pairs and counts them per point; an exclusive scan of the counts gives each
-// point's slots; near_fill_kernel writes one (coil key, point | slot) record per slot in (point, coil) order; a
-// radix sort by coil groups the records so that the lanes of a wave evaluate the same coil (uniform loop bounds,
+// field_kernel<kFP32C> skips those pairs and counts them per point and per coil; an exclusive scan of the point
+// counts gives each point's slots; near_key_scan_kernel turns the per-coil counts into cursors (a counting sort by
+// coil) and checks the total against the buffers' capacity; near_fill_kernel places one (coil key, point | slot)
+// record per pair at its coil's cursor, so that the lanes of a wave evaluate the same coil (uniform loop bounds,
 // one branch of the fast / slow rule); near_eval_kernel evaluates one record per thread into its slot;
 // near_add_kernel adds a point's slots to the outputs in coil order. A slot's value depends only on its (coil,
-// point), and every sum has a fixed order, so FP32C results are bitwise reproducible.
+// point), and every sum has a fixed order, so FP32C results are bitwise reproducible although the order of the
+// records within a coil is not. Nothing here is read back by the host before the work is done.
 #include "kernels.hpp"
 #include "integrands.hpp"
 
@@ -13,43 +16,77 @@
 
 namespace ccoil {
 
+// One workgroup: cursor = exclusive prefix sum of near_hist over the n_coils + 1 keys, and the call's total.
+__global__ __launch_bounds__(256) void near_key_scan_kernel(FieldParams p)
+{
+    __shared__ uint32_t part[256];
+    const int t = threadIdx.x, nk = p.n_coils + 1;
+    uint32_t carry = 0;
+    for (int base = 0; base < nk; base += 256) {
+        const uint32_t v = base + t < nk ? p.near_hist[base + t] : 0u;
+        part[t] = v;
+        __syncthreads();
+        for (int d = 1; d < 256; d <<= 1) {   // inclusive Hillis-Steele scan
+            const uint32_t add = t >= d ? part[t - d] : 0u;
+            __syncthreads();
+            part[t] += add;
+            __syncthreads();
+        }
+        if (base + t < nk) p.near_cursor[base + t] = carry + part[t] - v;
+        carry += part[255];
+        __syncthreads();
+    }
+    if (t == 0) {
+        const unsigned long long total =
+            (unsigned long long)(p.near_offsets[p.n_pts - 1] + p.near_counts[p.n_pts - 1]);
+        p.near_state->total = total;
+        p.near_state->overflow = total > (unsigned long long)p.near_cap ? 1u : 0u;
+    }
+}
+
 __global__ __launch_bounds__(256) void near_fill_kernel(FieldParams p)
 {
+    if (p.near_state->overflow) return;
     const int64_t pi = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (pi >= p.n_pts) return;
     const int64_t count = p.near_counts[pi];
     if (count == 0) return;
+    const uint64_t total = p.near_state->total;
     const double px = p.pts[3 * pi], py = p.pts[3 * pi + 1], pz = p.pts[3 * pi + 2];
     int64_t slot = p.near_offsets[pi];
     const int64_t end = slot + count;
     const uint64_t hi = (uint64_t)pi << 32;
-    for (int ci = 0; ci < p.n_coils && slot < end; ++ci) {
+    bool bad = false;
+    for (int ci = 0; ci < p.n_coils; ++ci) {
         const CoilDev &c = p.coils[ci];
         double z, r, cph, sph;
         to_local(c, px, py, pz, z, r, cph, sph);
-        if (is_near(c, z, r, p.q & (Q_G | Q_GJ))) {   // the field kernel's own test
-            p.near_keys[slot] = (uint32_t)c.near_key;
-            p.near_recs[slot] = hi | (uint64_t)slot;
-            ++slot;
+        if (!is_near(c, z, r, p.q & (Q_G | Q_GJ))) continue;   // the field kernel's own test, bitwise
+        const uint32_t pos = atomicAdd(&p.near_cursor[c.near_key], 1u);
+        if (slot >= end || pos >= total) {   // classified differently from field_kernel: report, write nothing
+            bad = true;
+            continue;
         }
+        p.near_keys[pos] = (uint32_t)c.near_key;
+        p.near_recs[pos] = hi | (uint64_t)slot;
+        ++slot;
     }
-    // the same test on the same inputs finds the same coils; should it ever find fewer, the rest stay inert
-    // (key n_coils, sorted last, evaluated as zero)
-    for (; slot < end; ++slot) {
-        p.near_keys[slot] = (uint32_t)p.n_coils;
-        p.near_recs[slot] = hi | (uint64_t)slot;
-    }
+    if (bad || slot != end) atomicOr(&p.near_state->mismatch, 1u);
 }
 
 // Records sorted by coil: lane e takes the e-th record; its value goes to the record's slot.
 template <bool WA, bool WB, bool WG>
-__global__ __launch_bounds__(256) void near_eval_kernel(FieldParams p, int64_t count)
+__global__ __launch_bounds__(256) void near_eval_kernel(FieldParams p)
 {
+    if (p.near_state->overflow) return;
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= count) return;
-    const uint32_t key = p.near_keys_sorted[e];
-    const uint64_t rec = p.near_recs_sorted[e];
+    const int64_t total = (int64_t)p.near_state->total;
+    if (e >= total) return;
+    const uint32_t key = p.near_keys[e];
+    const uint64_t rec = p.near_recs[e];
     const int64_t pi = (int64_t)(rec >> 32), slot = (int64_t)(rec & 0xffffffffu);
+    // only a near_fill that reported a mismatch leaves records unwritten; never follow one out of bounds
+    if (pi >= p.n_pts || slot >= total) return;
     double *o = p.near_vals + slot * p.near_width;
     // A wave holds one coil's records, or two or more at the boundaries of the sorted keys: it takes its keys one
     // at a time (the key of the lowest lane still to do, read into a scalar register), and every lane evaluates
@@ -66,7 +103,7 @@ __global__ __launch_bounds__(256) void near_eval_kernel(FieldParams p, int64_t c
         const uint32_t k = __builtin_amdgcn_readfirstlane(__shfl(key, __builtin_ctzll(todo)));
         const bool mine = key == k;
         todo &= ~(uint64_t)__ballot(mine);
-        if (k >= (uint32_t)p.n_coils) continue;   // inert records (sorted last): their value is zero
+        if (k >= (uint32_t)p.n_coils) continue;   // never a valid key; only a reported mismatch can leave one
         const CoilDev &c = p.coils[p.near_coil[k]];
         double z, r, cph, sph;
         to_local(c, px, py, pz, z, r, cph, sph);
@@ -82,6 +119,7 @@ __global__ __launch_bounds__(256) void near_eval_kernel(FieldParams p, int64_t c
 
 __global__ __launch_bounds__(256) void near_add_kernel(FieldParams p)
 {
+    if (p.near_state->overflow) return;
     const int64_t pi = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (pi >= p.n_pts) return;
     const int64_t count = p.near_counts[pi];
@@ -113,12 +151,10 @@ hipError_t near_scan(hipStream_t s, const int64_t *counts, int64_t *offsets, int
     return hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, counts, offsets, (size_t)n, s);
 }
 
-hipError_t near_sort(hipStream_t s, const FieldParams &p, int64_t count, void *temp, size_t &temp_bytes)
+void launch_near_key_scan(hipStream_t s, const FieldParams &p)
 {
-    int bits = 1;   // keys are 0 .. n_coils
-    while (bits < 32 && ((uint64_t)p.n_coils >> bits) != 0) ++bits;
-    return hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, p.near_keys, p.near_keys_sorted, p.near_recs,
-                                              p.near_recs_sorted, (int)count, 0, bits, s);
+    if (p.n_pts <= 0) return;
+    hipLaunchKernelGGL(near_key_scan_kernel, dim3(1), dim3(256), 0, s, p);
 }
 
 void launch_near_fill(hipStream_t s, const FieldParams &p)
@@ -127,20 +163,21 @@ void launch_near_fill(hipStream_t s, const FieldParams &p)
     hipLaunchKernelGGL(near_fill_kernel, dim3((unsigned)((p.n_pts + 255) / 256)), dim3(256), 0, s, p);
 }
 
-void launch_near_eval(hipStream_t s, const FieldParams &p, int64_t count)
+// The grid covers the capacity; threads beyond the call's total (read on the device) return at once.
+void launch_near_eval(hipStream_t s, const FieldParams &p)
 {
-    if (count <= 0) return;
-    const unsigned blocks = (unsigned)((count + 255) / 256);
+    if (p.near_cap <= 0) return;
+    const unsigned blocks = (unsigned)((p.near_cap + 255) / 256);
     const bool wa = p.q & (Q_A | Q_E), wb = p.q & Q_B, wg = p.q & (Q_G | Q_GJ);
     const int code = (wa ? 4 : 0) | (wb ? 2 : 0) | (wg ? 1 : 0);
     switch (code) {
-        case 1: hipLaunchKernelGGL((near_eval_kernel<false, false, true>), dim3(blocks), dim3(256), 0, s, p, count); break;
-        case 2: hipLaunchKernelGGL((near_eval_kernel<false, true, false>), dim3(blocks), dim3(256), 0, s, p, count); break;
-        case 3: hipLaunchKernelGGL((near_eval_kernel<false, true, true>), dim3(blocks), dim3(256), 0, s, p, count); break;
-        case 4: hipLaunchKernelGGL((near_eval_kernel<true, false, false>), dim3(blocks), dim3(256), 0, s, p, count); break;
-        case 5: hipLaunchKernelGGL((near_eval_kernel<true, false, true>), dim3(blocks), dim3(256), 0, s, p, count); break;
-        case 6: hipLaunchKernelGGL((near_eval_kernel<true, true, false>), dim3(blocks), dim3(256), 0, s, p, count); break;
-        default: hipLaunchKernelGGL((near_eval_kernel<true, true, true>), dim3(blocks), dim3(256), 0, s, p, count); break;
+        case 1: hipLaunchKernelGGL((near_eval_kernel<false, false, true>), dim3(blocks), dim3(256), 0, s, p); break;
+        case 2: hipLaunchKernelGGL((near_eval_kernel<false, true, false>), dim3(blocks), dim3(256), 0, s, p); break;
+        case 3: hipLaunchKernelGGL((near_eval_kernel<false, true, true>), dim3(blocks), dim3(256), 0, s, p); break;
+        case 4: hipLaunchKernelGGL((near_eval_kernel<true, false, false>), dim3(blocks), dim3(256), 0, s, p); break;
+        case 5: hipLaunchKernelGGL((near_eval_kernel<true, false, true>), dim3(blocks), dim3(256), 0, s, p); break;
+        case 6: hipLaunchKernelGGL((near_eval_kernel<true, true, false>), dim3(blocks), dim3(256), 0, s, p); break;
+        default: hipLaunchKernelGGL((near_eval_kernel<true, true, true>), dim3(blocks), dim3(256), 0, s, p); break;
     }
 }
 

@@ -203,7 +203,9 @@ class Coil {
     [[nodiscard]] double computeSecondaryInducedVoltage(const Coil &secondary,
                                                         const CoilPairArguments &inductanceArguments,
                                                         ComputeMethod computeMethod = CPU_ST) const;
-    // Self inductance uses the reference's CPU z-axis quadrature: throws std::logic_error here.
+    // Self-inductance by the reference's coaxial rule (CalculateMInductanceZAxis.cxx:224-317), evaluated on the
+    // device for THIN and RECTANGULAR coils (ccoil_self_inductance) and through the device field path for FLAT
+    // coils; FILAMENT coils have no self-inductance and throw std::logic_error, as in the reference.
     double computeAndSetSelfInductance(PrecisionFactor precisionFactor);
 
     static std::pair<vec3::Vector3, vec3::Vector3>

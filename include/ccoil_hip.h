@@ -116,7 +116,8 @@ ccoil_status ccoil_create(int device, ccoil_ctx *out);
 ccoil_status ccoil_destroy(ccoil_ctx ctx);
 const char *ccoil_last_error(ccoil_ctx ctx);
 const char *ccoil_version(void);
-/* Use an external stream (hipStream_t cast to void*); NULL restores the context's own stream. */
+/* Use an external stream (hipStream_t cast to void*; hipStreamLegacy binds the default null stream); NULL restores
+ * the context's own (non-blocking) stream. */
 ccoil_status ccoil_set_stream(ccoil_ctx ctx, void *stream);
 ccoil_status ccoil_synchronize(ccoil_ctx ctx);
 /* The stream the context's kernels are ordered on (its own, or the one given to ccoil_set_stream), as hipStream_t

@@ -27,7 +27,8 @@ def gpu_args_rows(params, pf=5.0):
 
 
 def rel_err_vec(x, ref, floor=0.0):
-    """Per-point vector-norm relative error |x - ref| / |ref| (rows), points with |ref| <= floor get NaN."""
+    """Per-point vector-norm relative error |x - ref| / |ref| (rows). Points with |ref| <= floor get NaN (excluded by
+    nanmax); a row whose output x is not finite gets inf wherever it lies, so a NaN written by a kernel fails."""
     x = np.asarray(x).reshape(len(ref), -1)
     ref = np.asarray(ref).reshape(len(ref), -1)
     den = np.linalg.norm(ref, axis=1)
@@ -35,6 +36,7 @@ def rel_err_vec(x, ref, floor=0.0):
     out = np.full(len(ref), np.nan)
     ok = den > floor
     out[ok] = num[ok] / den[ok]
+    out[~np.isfinite(x).all(axis=1)] = np.inf
     return out
 
 
@@ -53,6 +55,19 @@ def fp64_pair_count(params, pts, width_floor=1e-3):
         w = max(c.a, c.b, width_floor * c.R)
         n += int(np.sum((dr * dr + dz * dz < w * w) | (r * r < 0.01 * (r * r + z * z + c.R * c.R))))
     return n
+
+
+def inside_mask(params, pts):
+    """Points inside any coil's winding volume (R <= r <= R + a, |z| <= b/2 in the coil frame), where the reference
+    warns that fields are not meaningful (Coil.h:150) and the quadrature's node values are near-singular."""
+    inside = np.zeros(len(pts), dtype=bool)
+    for p in params:
+        c = oracle_coil(p)
+        T = np.array(c.Tinv).reshape(3, 3)
+        l = (pts - np.array(p["pos"])) @ T.T
+        r = np.hypot(l[:, 0], l[:, 1])
+        inside |= (r >= c.R) & (r <= c.R + c.a) & (np.abs(l[:, 2]) <= 0.5 * c.b)
+    return inside
 
 
 def near_mask(params, pts, width_floor=1e-3):
