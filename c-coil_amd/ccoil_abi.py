@@ -12,17 +12,21 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libccoil_hip.so")
 
-OK, ERR_ARGUMENT, ERR_OUT_OF_MEMORY, ERR_DEVICE, ERR_NO_DEVICE = 0, -1, -2, -3, -4
+OK, ERR_ARGUMENT, ERR_OUT_OF_MEMORY, ERR_DEVICE, ERR_NO_DEVICE, ERR_CAPACITY = 0, -1, -2, -3, -4, -5
 A, B, E, G, G_JACOBIAN = 1, 2, 4, 8, 16
 FP64, FP32C, FP32, FP64_REFERENCE, FP64_IEEE = 0, 1, 2, 3, 4
-DEVICE_POINTERS, NO_NEAR_FP64, FORCE_REFERENCE_SCALE = 1, 2, 4
+DEVICE_POINTERS, NO_NEAR_FP64, FORCE_REFERENCE_SCALE, AUTOTUNE = 1, 2, 4, 8
 PAIR_M, PAIR_FT, PAIR_DM = 1, 2, 4
 HIP_STREAM_LEGACY = 1   # hipStreamLegacy: the default (null) stream as a stream handle
 
 EXPORTED = ["ccoil_create", "ccoil_destroy", "ccoil_last_error", "ccoil_version", "ccoil_set_stream",
             "ccoil_synchronize", "ccoil_gauss_legendre", "ccoil_fields", "ccoil_pairs", "ccoil_field_gradient",
-            "ccoil_pairs_zaxis", "ccoil_self_inductance", "ccoil_get_stream"]
+            "ccoil_pairs_zaxis", "ccoil_self_inductance", "ccoil_get_stream",
+            "ccoil_set_tuning_file", "ccoil_fields_plan", "ccoil_pairs_plan", "ccoil_plan_launch",
+            "ccoil_plan_synchronize", "ccoil_plan_destroy", "ccoil_pairs_grad"]
 GRAD_TERMS = ("current", "R", "a", "b", "px", "py", "pz", "alpha", "beta")
+PAIR_GRAD_TERMS = ("sec_x", "sec_y", "sec_z", "sec_alpha", "sec_beta", "sec_R", "sec_a", "sec_b",
+                   "prim_x", "prim_y", "prim_z", "prim_alpha", "prim_beta", "prim_R", "prim_a", "prim_b")
 
 
 class Coils(ctypes.Structure):
@@ -64,6 +68,17 @@ def lib():
         L.ccoil_gauss_legendre.argtypes = [vp, ctypes.c_int32, dp, dp]
         L.ccoil_fields.argtypes = [vp, ctypes.POINTER(Coils), vp, ctypes.c_int64, ctypes.c_uint32, ctypes.c_int32,
                                    ctypes.c_uint32, vp, vp, vp, vp, ctypes.POINTER(Report)]
+        L.ccoil_set_tuning_file.argtypes = [vp, ctypes.c_char_p]
+        L.ccoil_fields_plan.argtypes = [vp, ctypes.POINTER(Coils), vp, ctypes.c_int64, ctypes.c_uint32, ctypes.c_int32,
+                                        ctypes.c_uint32, vp, vp, vp, vp, ctypes.POINTER(vp)]
+        L.ccoil_pairs_plan.argtypes = [vp, ctypes.POINTER(Coils), ctypes.POINTER(Coils), vp, vp, ctypes.c_int64, vp,
+                                       vp, ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32, vp, vp, vp, vp,
+                                       ctypes.POINTER(vp)]
+        L.ccoil_pairs_grad.argtypes = [vp, ctypes.POINTER(Coils), ctypes.POINTER(Coils), vp, vp, ctypes.c_int64, vp,
+                                       vp, ctypes.c_int32, ctypes.c_uint32, vp, vp, ctypes.POINTER(Report)]
+        L.ccoil_plan_launch.argtypes = [vp, vp]
+        L.ccoil_plan_synchronize.argtypes = [vp, ctypes.POINTER(Report)]
+        L.ccoil_plan_destroy.argtypes = [vp]
         L.ccoil_pairs.argtypes = [vp, ctypes.POINTER(Coils), ctypes.POINTER(Coils), vp, vp, ctypes.c_int64, vp, vp,
                                   ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32, vp, vp, vp, vp, ctypes.POINTER(Report)]
         L.ccoil_field_gradient.argtypes = [vp, ctypes.POINTER(Coils), vp, ctypes.c_int64, vp, ctypes.c_int32,
@@ -73,7 +88,8 @@ def lib():
         L.ccoil_self_inductance.argtypes = [vp, ctypes.POINTER(Coils), vp, ctypes.c_uint32, vp,
                                             ctypes.POINTER(Report)]
         for f in ("ccoil_create", "ccoil_destroy", "ccoil_set_stream", "ccoil_synchronize", "ccoil_gauss_legendre",
-                  "ccoil_fields", "ccoil_pairs", "ccoil_field_gradient", "ccoil_pairs_zaxis",
+                  "ccoil_fields", "ccoil_set_tuning_file", "ccoil_fields_plan",
+                  "ccoil_pairs_plan", "ccoil_plan_launch", "ccoil_plan_synchronize", "ccoil_plan_destroy", "ccoil_pairs_grad", "ccoil_pairs", "ccoil_field_gradient", "ccoil_pairs_zaxis",
                   "ccoil_self_inductance"):
             getattr(L, f).restype = ctypes.c_int32
         _lib = L
@@ -125,6 +141,32 @@ class CoilSet:
                        turns=col("turns", 1.0), frequency=col("frequency", 0.0))
 
 
+class Plan:
+    """A field or pair call captured into a HIP graph (ccoil_fields_plan / ccoil_pairs_plan); launch() replays it."""
+
+    def __init__(self, ctx, handle, keep):
+        self.ctx, self.h, self.keep = ctx, handle, keep
+
+    def launch(self, stream=None):
+        self.ctx._check(lib().ccoil_plan_launch(self.h, ctypes.c_void_p(stream) if stream else None))
+
+    def synchronize(self):
+        rep = Report()
+        self.ctx._check(lib().ccoil_plan_synchronize(self.h, ctypes.byref(rep)))
+        return rep
+
+    def close(self):
+        if self.h:
+            lib().ccoil_plan_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Context:
     def __init__(self, device=0):
         self.h = ctypes.c_void_p()
@@ -156,6 +198,9 @@ class Context:
             handle = ctypes.c_void_p(stream_ptr if stream_ptr else HIP_STREAM_LEGACY)
         self._check(lib().ccoil_set_stream(self.h, handle))
 
+    def set_tuning_file(self, path):
+        self._check(lib().ccoil_set_tuning_file(self.h, None if path is None else str(path).encode()))
+
     def synchronize(self):
         self._check(lib().ccoil_synchronize(self.h))
 
@@ -182,6 +227,29 @@ class Context:
                                        flags, _ptr(out.get("A")), _ptr(out.get("B")), _ptr(out.get("E")),
                                        _ptr(out.get("G")), ctypes.byref(rep)))
         return out, rep
+
+    def fields_plan(self, coils, points, quantities, precision, out, flags=0):
+        """ccoil_fields_plan over torch tensors on the context's device (points (n, 3), out: dict of outputs)."""
+        h = ctypes.c_void_p()
+        self._check(lib().ccoil_fields_plan(self.h, ctypes.byref(coils.struct), _ptr(points), points.shape[0],
+                                            quantities, precision, flags | DEVICE_POINTERS, _ptr(out.get("A")),
+                                            _ptr(out.get("B")), _ptr(out.get("E")), _ptr(out.get("G")),
+                                            ctypes.byref(h)))
+        return Plan(self, h, (coils, points, out))
+
+    def pairs_plan(self, prim, sec, n_pairs, what, precision, out, flags=0, prim_index=None, sec_index=None,
+                   ring_offset=None, half_ring=None):
+        """ccoil_pairs_plan; out: dict of torch tensors M (n,), F, T, dM (n, 3) on the context's device."""
+        pi = None if prim_index is None else np.ascontiguousarray(prim_index, dtype=np.int64)
+        si = None if sec_index is None else np.ascontiguousarray(sec_index, dtype=np.int64)
+        ro = None if ring_offset is None else np.ascontiguousarray(ring_offset, dtype=np.float64)
+        hr = None if half_ring is None else np.ascontiguousarray(half_ring, dtype=np.int32)
+        h = ctypes.c_void_p()
+        self._check(lib().ccoil_pairs_plan(self.h, ctypes.byref(prim.struct), ctypes.byref(sec.struct), _ptr(pi),
+                                           _ptr(si), n_pairs, _ptr(ro), _ptr(hr), what, precision,
+                                           flags | DEVICE_POINTERS, _ptr(out.get("M")), _ptr(out.get("F")),
+                                           _ptr(out.get("T")), _ptr(out.get("dM")), ctypes.byref(h)))
+        return Plan(self, h, (prim, sec, out))
 
     def field_gradient(self, coils, points, dL_dB, precision=FP64, flags=0, out=None):
         """Adjoint of the summed B: (n_coils, 9) gradients in GRAD_TERMS order. Host numpy arrays, or torch tensors
@@ -221,6 +289,21 @@ class Context:
                                       _ptr(out.get("F")), _ptr(out.get("T")), _ptr(out.get("dM")),
                                       ctypes.byref(rep)))
         return out, rep
+
+    def pairs_grad(self, prim, sec, n_pairs, precision=FP64, flags=0, prim_index=None, sec_index=None,
+                   ring_offset=None, half_ring=None):
+        """ccoil_pairs_grad: (M (n,), grad (n, 16) in PAIR_GRAD_TERMS order), host numpy arrays."""
+        rep = Report()
+        M = np.zeros(n_pairs)
+        grad = np.zeros((n_pairs, len(PAIR_GRAD_TERMS)))
+        pi = None if prim_index is None else np.ascontiguousarray(prim_index, dtype=np.int64)
+        si = None if sec_index is None else np.ascontiguousarray(sec_index, dtype=np.int64)
+        ro = None if ring_offset is None else np.ascontiguousarray(ring_offset, dtype=np.float64)
+        hr = None if half_ring is None else np.ascontiguousarray(half_ring, dtype=np.int32)
+        self._check(lib().ccoil_pairs_grad(self.h, ctypes.byref(prim.struct), ctypes.byref(sec.struct), _ptr(pi),
+                                           _ptr(si), n_pairs, _ptr(ro), _ptr(hr), precision, flags, _ptr(M),
+                                           _ptr(grad), ctypes.byref(rep)))
+        return M, grad, rep
 
     def pairs_zaxis(self, prim, sec, n_pairs, what, flags=0, prim_index=None, sec_index=None, out=None):
         """Coaxial pairs: {"M": (n,), "Fz": (n,)} (axial force on the secondary)."""

@@ -7,6 +7,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -19,6 +21,27 @@ struct DevBuf {
 
 hipError_t ensure(DevBuf &b, size_t bytes);
 void free_buf(DevBuf &b);
+
+// A fixed piece of device memory handed out front to back in 256-byte aligned parts (the no-allocation entry
+// point and the plans take their buffers from one instead of the context's grow-only buffers).
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0;
+    size_t used = 0;
+    void *take(size_t bytes)
+    {
+        const size_t off = (used + 255) & ~(size_t)255;
+        if (off > cap || bytes > cap - off) return nullptr;
+        used = off + bytes;
+        return base + off;
+    }
+    size_t left() const
+    {
+        const size_t off = (used + 255) & ~(size_t)255;
+        return off >= cap ? 0 : cap - off;
+    }
+};
+
 void rotation(double alpha, double beta, double *T, double *Ti);
 void rotation_generators(double alpha, double beta, double *om_a, double *om_b);
 std::string pack_coils(const ccoil_coils *c, bool need_len, bool need_freq, std::vector<CoilDev> &out);
@@ -47,6 +70,8 @@ struct ccoil_ctx_s {
     ccoil::DevBuf zitems, zpartial, zout;               // coaxial rules
     hipEvent_t ev0 = nullptr, ev1 = nullptr;   // around all compute kernels of a call
     hipEvent_t evm0 = nullptr, evm1 = nullptr; // around the main kernel (field_kernel / pair_kernel)
+    std::map<std::string, std::string> tuning;  // autotuner cache: call key -> chosen launch shape
+    std::string tuning_file;                    // where new tuning entries are appended ("" = memory only)
 };
 
 namespace ccoil {
@@ -65,18 +90,84 @@ struct FieldLaunch {
     int precision = 0;
     int split = 1;
     bool near = false;          // FP32C near pass
-    void *scan_temp = nullptr;  // hipCUB scan storage of the near pass
+    void *scan_temp = nullptr;  // scan storage of the near pass (tile sums)
     size_t scan_bytes = 0;
 };
 // Enqueues the field kernel, the chunk reduction and (FP32C) the whole near pass on s, with no host
-// synchronisation; evm0/evm1 (may be NULL) bracket the field kernel. The near pass's outcome is copied to
-// host_state (pinned) when given.
+// synchronisation and kernel launches and copies only (so it can be captured in a graph); evm0/evm1 (may be NULL)
+// bracket the field kernel. The near pass's outcome is copied to host_state (pinned) when given.
 void enqueue_fields(hipStream_t s, const FieldLaunch &L, hipEvent_t evm0, hipEvent_t evm1, NearState *host_state);
 // The near pass from the per-coil cursors on (after a grow-on-overflow retry): key scan, fill, eval, add.
 void enqueue_near_tail(hipStream_t s, const FieldLaunch &L, NearState *host_state);
 
 ccoil_status fail(ccoil_ctx ctx, ccoil_status code, const std::string &msg);
 Tables ctx_tables(ccoil_ctx ctx);
+// A call's device buffer of `bytes`: the context's grow-only buffer b (may allocate) when arena is NULL, else the
+// arena's next part (never allocates; fails with CCOIL_ERR_ARGUMENT when the arena is too small).
+ccoil_status call_buffer(ccoil_ctx ctx, DevBuf &b, Arena *arena, size_t bytes, void **out);
+
+// A field call made ready to launch: packed coils, the launch parameters and their buffers.
+struct FieldPrep {
+    std::vector<CoilDev> hc;
+    FieldLaunch L;
+    int64_t n = 0;
+    uint32_t q = 0;
+    int width = 0;           // doubles per point of the requested outputs
+    int64_t nodes = 0;       // quadrature nodes summed over the coils
+    bool dev = false;        // points and outputs are device pointers
+};
+// Validates a field call, uploads its coil records and sizes its buffers: the context's grow-only buffers, or
+// parts of arena (then points and outputs must be device pointers). shape: NULL for the launch-shape rule;
+// near_cap > 0: the FP32C near pass's record capacity (else every pair when few, or the context's kept capacity).
+// A call with no points or no coils returns CCOIL_OK with nothing uploaded.
+ccoil_status fields_prepare(ccoil_ctx ctx, const ccoil_coils *coils, const double *points, int64_t n_points,
+                            uint32_t quantities, int32_t precision, uint32_t flags, double *A, double *B, double *E,
+                            double *G, Arena *arena, const FieldShape *shape, int64_t near_cap, FieldPrep &out);
+// Zero-fills the requested outputs of a call with no coils (device pointers).
+void enqueue_zero_outputs(hipStream_t s, const FieldPrep &fp);
+// ccoil_fields with its buffers from arena (NULL: the context's buffers) and a launch shape (NULL: the rule). With an
+// arena an FP32C near pass that needs more records than the arena holds is an error.
+ccoil_status fields_call(ccoil_ctx ctx, const ccoil_coils *coils, const double *points, int64_t n_points,
+                         uint32_t quantities, int32_t precision, uint32_t flags, double *A, double *B, double *E,
+                         double *G, ccoil_report *report, Arena *arena, const FieldShape *shape);
+ccoil_status near_buffers(ccoil_ctx ctx, FieldLaunch &L, int64_t cap, int width, Arena *arena);
+ccoil_status near_outcome(ccoil_ctx ctx, const NearState &st);
+
+// The host side of a pair call: packed coils and the work list, longest work item first.
+struct PairBuild {
+    std::vector<CoilDev> hp, hs;
+    std::vector<PairDev> sorted;
+    int64_t items = 0;
+    int64_t node_pairs = 0;
+    int64_t target = 0;        // primary-node evaluations per lane and work item
+    int max_row_groups = 1;
+};
+// Validates a pair call and builds its work list; target_override > 0 replaces the lane-target rule.
+ccoil_status build_pairs(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coils *sec, const int64_t *prim_index,
+                         const int64_t *sec_index, int64_t n_pairs, const double *ring_offset,
+                         const int32_t *half_ring, uint32_t what, int32_t precision, uint32_t flags, double *M,
+                         double *F, double *T, double *dM, int64_t target_override, PairBuild &out);
+// Uploads a built pair call into bufs (primary coils, secondary coils, work list, partials) and fills the kernel
+// parameters (outputs left NULL).
+ccoil_status pairs_setup(ccoil_ctx ctx, const PairBuild &pb, uint32_t what, int32_t precision, DevBuf *const bufs[4],
+                         PairParams &pp);
+
+// The autotuner: the launch shape of a field call (lanes per point, coil chunks) or the lane target of a pair call,
+// from the context's tuning cache, or, the first time a call of that size is seen, by timing the candidates on the
+// device (each candidate runs the call itself, so its outputs are written) and caching the fastest. Deterministic
+// once cached: the same cache gives the same shape, hence bitwise the same results.
+ccoil_status tune_fields(ccoil_ctx ctx, const ccoil_coils *coils, const double *points, int64_t n_points,
+                         uint32_t quantities, int32_t precision, uint32_t flags, double *A, double *B, double *E,
+                         double *G, FieldShape &out);
+ccoil_status tune_pairs(ccoil_ctx ctx, const PairBuild &rule, uint32_t what, int32_t precision,
+                        const std::function<ccoil_status(int64_t, double &)> &run, int64_t &target);
+// The tuned lane target of a pair call (rule: its build with the rule's target); tuning runs the call.
+ccoil_status pairs_tuned_target(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coils *sec,
+                                const int64_t *prim_index, const int64_t *sec_index, int64_t n_pairs,
+                                const double *ring_offset, const int32_t *half_ring, uint32_t what, int32_t precision,
+                                uint32_t flags, double *M, double *F, double *T, double *dM, const PairBuild &rule,
+                                int64_t &target);
+ccoil_status load_tuning_file(ccoil_ctx ctx, const char *path);
 
 #define CC_HIP(ctx, call)                                                                                   \
     do {                                                                                                    \

@@ -127,12 +127,6 @@ void field_kernel(FieldParams p)
             const bool skip = p.near_fp64 && is_near(c, z, r, WG);
             if (!skip) eval_stable<float, WA, WB, WG>(c, p.tab, z, r, lane, SPLIT, L);
             else ++n_near;
-            if (p.near_fp64) {
-                // the coil's near points in this wave, counted once per point, added by one lane
-                const uint64_t m = __ballot(skip && active && lane == 0);
-                if (m && (threadIdx.x & 63) == __builtin_ctzll(m))
-                    atomicAdd(&p.near_hist[c.near_key], (unsigned)__popcll(m));
-            }
         }
         if (SPLIT > 1) reduce_local<SPLIT>(L);
         accumulate_global(c, L, r, cph, sph, p.q, acc);

@@ -267,15 +267,27 @@ struct AdjLocal {
     double dR[2], da[2], db[2];
 };
 
+// Pair adjoint extras of one (coil, point): derivatives of the local A_phi with respect to the coil's inner radius
+// R, thickness a and length b. Row j of the rule is A_j = W_j S_j with W_j = cf w_j rho_j: for the sheet (fast
+// method) cf = 1e-7 N I pi / (2 b) and S_j = sum w cos (asinh(z1 / sqrt M) - asinh(z2 / sqrt M)), so A_j is
+// homogeneous of degree 0 in (rho, b, r, z); for the ring (slow method) cf = 1e-7 N I pi / 2 and A_j is homogeneous
+// of degree 0 in (rho, r, z). Hence
+//   rho dA_j/drho = -(r dA_j/dr + z dA_j/dz + b dA_j/db),  dA_j/dr = B_z,j - A_j / r,  dA_j/dz = -B_rho,j,
+// with the row's own B, and, for the sheet, dA_j/db = -A_j / b + W_j sum w cos (K1 + K2) / 2 (the end rings).
+struct AdjA {
+    double dR, da, db;
+};
+
 // Angular-loop unrolling of eval_stable: 4, except 2 for the float B / gradient instances (the FP32C field kernel
 // measured 2 % faster with 2 and 12 % slower with 8).
 template <typename T, bool WA> constexpr int kStableUnroll = (sizeof(T) == 4 && !WA) ? 2 : 4;
 
-template <typename P, bool WA, bool WB, bool WG, bool AD = false>
+template <typename P, bool WA, bool WB, bool WG, bool AD = false, bool ADA = false>
 __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double zd, double rd, int row0, int rstep,
-                                   Local &L, AdjLocal *ad = nullptr)
+                                   Local &L, AdjLocal *ad = nullptr, AdjA *ada = nullptr)
 {
     static_assert(!AD || (WB && WG), "the adjoint needs B and the gradient terms");
+    static_assert(!ADA || (WA && WB), "the pair adjoint needs A and B");
     using T = real_t<P>;
     const double *xt = tab.x + table_offset(c.n_thk);
     const double *wt = tab.w + table_offset(c.n_thk);
@@ -288,6 +300,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
     const int na = c.n_ang, nt = c.n_thk;
     double aA = 0, aBh = 0, aBz = 0, aRR = 0, aRZ = 0, aZZ = 0;
     double hR0 = 0, hR1 = 0, ha0 = 0, ha1 = 0, hb0 = 0, hb1 = 0;   // adjoint row sums
+    double qR = 0, qa = 0, qb = 0;                                  // pair adjoint row sums (A_phi)
 
     if (c.use_fast) {
         // The sheet's field is mirror-symmetric in z: B_rho, dB_rho/drho, dB_z/dz (and the adjoint's rho
@@ -318,6 +331,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             T rRRs = 0, rRRc = 0;   // sum w cos s dL and sum w cos dL: the row's RR is rho rRRs - dm rRRc
             T rBzs = 0;             // double without the gradient: sum w s dgM (rBz then holds sum w dgM)
             T eR = 0, eZ = 0;   // adjoint: end rings, sum w cos (z1 K1^3 + z2 K2^3), sum w (rho - r cos)(K1^3 + K2^3)
+            T eA = 0;           // pair adjoint: end rings of A, sum w cos (K1 + K2)
 #pragma unroll kStableUnroll<T, WA>
             for (int i = 0; i < na; ++i) {
                 const T s = st[i], w = wa[i], wc = wct[i];
@@ -354,6 +368,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                     const T invM = gsel * Rr;
                     const T dgM = (same ? tP2 * M : fma_<T>(z1, K1, -g2)) * Rr;   // (z1 K1 - z2 K2) / M
                     rBh = fma_<T>(wc, -dK, rBh);       // B_rho, also the gradient's rho-phi term
+                    if constexpr (ADA) eA = fma_<T>(wc, Ks, eA);
                     constexpr bool kBzMoments = sizeof(T) == 8 && !WG;
                     T wr = 0, rmc = 0;
                     if constexpr (kBzMoments) {
@@ -408,6 +423,13 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             aZZ += W * (double)rZZ;
             aRZ += W * (double)rRZ;
             aA += W * (double)rA;
+            if constexpr (ADA) {
+                const double rowA = W * (double)rA, dAdb = W * 0.5 * (double)eA - rowA / c.b;
+                const double dAdrho = -(rd * W * rowBz - rowA - zm * W * (double)rBh + c.b * dAdb) / rho_d;
+                qR += dAdrho;
+                qa += 0.5 * (1.0 + xt[j]) * dAdrho;
+                qb += dAdb;
+            }
             if constexpr (AD) {
                 const double Wr = c.cf * wt[j], hx = 0.5 * (1.0 + xt[j]);
                 const double e0 = 0.5 * (double)eR, e1 = 0.5 * (double)eZ;
@@ -422,6 +444,11 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             }
         }
         const double sgn = neg ? -1.0 : 1.0;   // odd terms at z < 0
+        if constexpr (ADA) {   // A is even in z
+            ada->dR = qR;
+            ada->da = qa;
+            ada->db = qb;
+        }
         if constexpr (AD) {
             ad->dR[0] = -sgn * hR0;
             ad->dR[1] = -hR1;
@@ -518,6 +545,12 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             aZZ += W * (double)rZZ;
             aRZ += W * (double)rRZ;
             aA += W * (double)rA;
+            if constexpr (ADA) {   // rows are rings: no length dependence
+                const double rowA = W * (double)rA;
+                const double dAdrho = -(rd * W * (double)rBz - rowA - zd * zd * W * (double)rBh) / rho_d;
+                qR += dAdrho;
+                qa += 0.5 * (1.0 + xt[j]) * dAdrho;
+            }
             if constexpr (AD) {
                 const double Wr = c.cf * wt[j], hx = 0.5 * (1.0 + xt[j]);
                 const double g0 = zd * ((double)rBh - 3.0 * rd * rowRR + (double)rRZ);           // -d f_rho / d rho
@@ -535,6 +568,11 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             ad->da[1] = -ha1;
             ad->db[0] = 0.0;
             ad->db[1] = 0.0;
+        }
+        if constexpr (ADA) {
+            ada->dR = qR;
+            ada->da = qa;
+            ada->db = 0.0;
         }
         L.A = aA;
         L.Bh = aBh * zd;

@@ -26,9 +26,9 @@ struct FieldParams {
     Tables tab;
     int32_t near_fp64;     // FP32C: skip near pairs here, re-evaluate in FP64
     // FP32C near-winding pass, stream-ordered with no host readback (so a caller may capture it in a graph):
-    // field_kernel counts each point's near coils (near_counts) and each coil's near points (near_hist, indexed by
-    // CoilDev::near_key, so the costliest coils come first); an exclusive scan of near_counts gives each point's
-    // slots (near_offsets); near_key_scan turns near_hist into per-coil cursors, and stores the total and whether
+    // field_kernel counts each point's near coils (near_counts); an exclusive scan of near_counts gives each point's
+    // slots (near_offsets); near_count counts each coil's near points (near_hist, indexed by CoilDev::near_key, so
+    // the costliest coils come first); near_key_scan turns near_hist into per-coil cursors, and stores the total and whether
     // it exceeds near_cap in near_state; near_fill places one record per (coil, point) pair at its coil's cursor:
     // the coil key (near_keys) and point << 32 | slot (near_recs); near_eval writes each record's FP64
     // contribution into its slot of near_vals (near_width doubles: the requested A, E, B, G in that order);
@@ -86,10 +86,12 @@ void launch_field(hipStream_t s, const FieldParams &p, int precision, int split)
 void launch_field_reduce(hipStream_t s, const FieldParams &p);   // after launch_field when n_chunks > 1
 // Resident 256-thread blocks per CU of the field kernel instance (0 if the runtime cannot tell).
 int field_blocks_per_cu(uint32_t q, int precision, int split);
-// Exclusive prefix sum of n counts into offsets; with temp == NULL only sets temp_bytes.
+// Exclusive prefix sum of n counts into offsets (kernel launches only); with temp == NULL only sets temp_bytes.
 hipError_t near_scan(hipStream_t s, const int64_t *counts, int64_t *offsets, int64_t n, void *temp,
                      size_t &temp_bytes);
-// near_key_scan: per-coil cursors from near_hist, and near_state's total and overflow (one small kernel).
+// near_count: per-coil counts into near_hist (zeroed before); near_key_scan: per-coil cursors from near_hist, and
+// near_state's total and overflow (one small kernel).
+void launch_near_count(hipStream_t s, const FieldParams &p);
 void launch_near_key_scan(hipStream_t s, const FieldParams &p);
 void launch_near_fill(hipStream_t s, const FieldParams &p);
 void launch_near_eval(hipStream_t s, const FieldParams &p);

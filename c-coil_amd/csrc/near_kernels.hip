@@ -1,20 +1,35 @@
 // FP32C near-winding pass (gfx950): (coil, point) pairs inside a coil's near shell (and, for outputs that divide by
 // r, its axis cone) are re-evaluated in FP64.
 //
-// field_kernel<kFP32C> skips those pairs and counts them per point and per coil; an exclusive scan of the point
-// counts gives each point's slots; near_key_scan_kernel turns the per-coil counts into cursors (a counting sort by
-// coil) and checks the total against the buffers' capacity; near_fill_kernel places one (coil key, point | slot)
-// record per pair at its coil's cursor, so that the lanes of a wave evaluate the same coil (uniform loop bounds,
-// one branch of the fast / slow rule); near_eval_kernel evaluates one record per thread into its slot;
-// near_add_kernel adds a point's slots to the outputs in coil order. A slot's value depends only on its (coil,
-// point), and every sum has a fixed order, so FP32C results are bitwise reproducible although the order of the
+// field_kernel<kFP32C> skips those pairs and counts them per point; an exclusive scan of the point counts gives each
+// point's slots; near_count_kernel counts each coil's near points and near_key_scan_kernel turns those counts into
+// cursors (a counting sort by coil) and checks the total against the buffers' capacity; near_fill_kernel places one
+// (coil key, point | slot) record per pair at its coil's cursor, so that the lanes of a wave evaluate the same coil
+// (uniform loop bounds, one branch of the fast / slow rule); near_eval_kernel evaluates one record per thread into
+// its slot; near_add_kernel adds a point's slots to the outputs in coil order. A slot's value depends only on its
+// (coil, point), and every sum has a fixed order, so FP32C results are bitwise reproducible although the order of the
 // records within a coil is not. Nothing here is read back by the host before the work is done.
 #include "kernels.hpp"
 #include "integrands.hpp"
 
-#include <hipcub/hipcub.hpp>
-
 namespace ccoil {
+
+// Per coil key, the near points (points with near pairs only; the coil loop is wave-uniform, so each coil's count
+// in a wave is one ballot and one atomic).
+__global__ __launch_bounds__(256) void near_count_kernel(FieldParams p)
+{
+    const int64_t pi = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (pi >= p.n_pts || p.near_counts[pi] == 0) return;
+    const double px = p.pts[3 * pi], py = p.pts[3 * pi + 1], pz = p.pts[3 * pi + 2];
+    const int lane = threadIdx.x & 63;
+    for (int ci = 0; ci < p.n_coils; ++ci) {
+        const CoilDev &c = p.coils[ci];
+        double z, r, cph, sph;
+        to_local(c, px, py, pz, z, r, cph, sph);
+        const uint64_t m = __ballot(is_near(c, z, r, p.q & (Q_G | Q_GJ)));   // the field kernel's own test, bitwise
+        if (m && lane == __builtin_ctzll(m)) atomicAdd(&p.near_hist[c.near_key], (unsigned)__popcll(m));
+    }
+}
 
 // One workgroup: cursor = exclusive prefix sum of near_hist over the n_coils + 1 keys, and the call's total.
 __global__ __launch_bounds__(256) void near_key_scan_kernel(FieldParams p)
@@ -56,13 +71,22 @@ __global__ __launch_bounds__(256) void near_fill_kernel(FieldParams p)
     int64_t slot = p.near_offsets[pi];
     const int64_t end = slot + count;
     const uint64_t hi = (uint64_t)pi << 32;
+    const int lane = threadIdx.x & 63;
     bool bad = false;
     for (int ci = 0; ci < p.n_coils; ++ci) {
         const CoilDev &c = p.coils[ci];
         double z, r, cph, sph;
         to_local(c, px, py, pz, z, r, cph, sph);
-        if (!is_near(c, z, r, p.q & (Q_G | Q_GJ))) continue;   // the field kernel's own test, bitwise
-        const uint32_t pos = atomicAdd(&p.near_cursor[c.near_key], 1u);
+        const bool near = is_near(c, z, r, p.q & (Q_G | Q_GJ));   // the field kernel's own test, bitwise
+        // the wave's records of this coil take consecutive places from one atomic on the coil's cursor
+        const uint64_t m = __ballot(near);
+        if (!m) continue;
+        const int leader = __builtin_ctzll(m);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&p.near_cursor[c.near_key], (unsigned)__popcll(m));
+        base = __shfl(base, leader);
+        if (!near) continue;
+        const uint64_t pos = base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
         if (slot >= end || pos >= total) {   // classified differently from field_kernel: report, write nothing
             bad = true;
             continue;
@@ -145,16 +169,99 @@ __global__ __launch_bounds__(256) void near_add_kernel(FieldParams p)
     if (p.G) for (int k = 0; k < 9; ++k) p.G[9 * pi + k] = vg[k];
 }
 
+// Exclusive prefix sum of the per-point counts in three kernels (tile sums, a one-workgroup scan of the tile sums,
+// the tiles' own scans): only kernel launches, so a caller may capture it in a graph, and deterministic.
+constexpr int kScanPerThread = 16;
+constexpr int kScanTile = 256 * kScanPerThread;
+
+__device__ inline int64_t block_exclusive_scan(int64_t v, int64_t *lds, int64_t &total)
+{
+    const int t = threadIdx.x;
+    lds[t] = v;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        const int64_t add = t >= d ? lds[t - d] : 0;
+        __syncthreads();
+        lds[t] += add;
+        __syncthreads();
+    }
+    const int64_t incl = lds[t];
+    total = lds[255];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(256) void scan_tile_sums_kernel(const int64_t *counts, int64_t n, int64_t *sums)
+{
+    __shared__ int64_t lds[256];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanPerThread;
+    int64_t v = 0;
+    for (int k = 0; k < kScanPerThread; ++k)
+        if (base + k < n) v += counts[base + k];
+    int64_t total;
+    (void)block_exclusive_scan(v, lds, total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void scan_sums_kernel(int64_t *sums, int64_t nb)
+{
+    __shared__ int64_t lds[256];
+    int64_t carry = 0;
+    for (int64_t base = 0; base < nb; base += 256) {
+        const int64_t i = base + threadIdx.x;
+        const int64_t v = i < nb ? sums[i] : 0;
+        int64_t total;
+        const int64_t ex = block_exclusive_scan(v, lds, total);
+        if (i < nb) sums[i] = carry + ex;
+        carry += total;
+    }
+}
+
+__global__ __launch_bounds__(256) void scan_tiles_kernel(const int64_t *counts, int64_t n, const int64_t *sums,
+                                                         int64_t *offsets)
+{
+    __shared__ int64_t lds[256];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanPerThread;
+    int64_t v[kScanPerThread], t = 0;
+    for (int k = 0; k < kScanPerThread; ++k) {
+        v[k] = base + k < n ? counts[base + k] : 0;
+        t += v[k];
+    }
+    int64_t total;
+    int64_t run = sums[blockIdx.x] + block_exclusive_scan(t, lds, total);
+    for (int k = 0; k < kScanPerThread; ++k) {
+        if (base + k < n) offsets[base + k] = run;
+        run += v[k];
+    }
+}
+
 hipError_t near_scan(hipStream_t s, const int64_t *counts, int64_t *offsets, int64_t n, void *temp,
                      size_t &temp_bytes)
 {
-    return hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, counts, offsets, (size_t)n, s);
+    const int64_t nb = (n + kScanTile - 1) / kScanTile;
+    if (!temp) {
+        temp_bytes = sizeof(int64_t) * (size_t)(nb > 0 ? nb : 1);
+        return hipSuccess;
+    }
+    if (n <= 0) return hipSuccess;
+    int64_t *sums = (int64_t *)temp;
+    hipLaunchKernelGGL(scan_tile_sums_kernel, dim3((unsigned)nb), dim3(256), 0, s, counts, n, sums);
+    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(256), 0, s, sums, nb);
+    hipLaunchKernelGGL(scan_tiles_kernel, dim3((unsigned)nb), dim3(256), 0, s, counts, n, (const int64_t *)sums,
+                       offsets);
+    return hipGetLastError();
 }
 
 void launch_near_key_scan(hipStream_t s, const FieldParams &p)
 {
     if (p.n_pts <= 0) return;
     hipLaunchKernelGGL(near_key_scan_kernel, dim3(1), dim3(256), 0, s, p);
+}
+
+void launch_near_count(hipStream_t s, const FieldParams &p)
+{
+    if (p.n_pts <= 0) return;
+    hipLaunchKernelGGL(near_count_kernel, dim3((unsigned)((p.n_pts + 255) / 256)), dim3(256), 0, s, p);
 }
 
 void launch_near_fill(hipStream_t s, const FieldParams &p)

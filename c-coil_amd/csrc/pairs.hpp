@@ -27,12 +27,19 @@ struct PairParams {
     int32_t n_pairs;
     int64_t n_items;
     double *partial;                 // kPairTerms doubles per item: M, F[3], T[3], dM/dp[3]
-    uint32_t what;                   // 1: M, 2: F and T, 4: dM/d(secondary position)
+    uint32_t what;                   // 1: M, 2: F and T, 4: dM/d(secondary position), 8: the full gradient of M
     int32_t precision;
     Tables tab;
     double *M, *F, *T, *dM;          // outputs (device), indexed by PairDev::out
+    double *grad;                    // what & 8: kPairGradOut doubles per pair
 };
 constexpr int kPairTerms = 10;
+// The gradient of M (what & 8): per item M and 13 derivative sums, dM/d(secondary x, y, z, alpha, beta, R, a, b,
+// primary alpha, beta, R, a, b); per pair 16 outputs, the secondary's 8 then the primary's 8 (x, y, z, alpha, beta,
+// R, a, b), the primary's position derivative being minus the secondary's.
+constexpr int kPairGradTerms = 14;
+constexpr int kPairGradOut = 16;
+constexpr uint32_t kPairGrad = 8;
 
 // One 64-lane workgroup per work item (grid = n_items, at most 2^31 - 1). A pair's items are its chunks of
 // secondary nodes times its primary row groups: item i of a pair takes chunk i / row_groups and row group

@@ -40,11 +40,12 @@ void decompose(int64_t target, int n_ang, int n_thk, int &nodes_per_item, int &r
 
 }  // namespace
 
-extern "C" ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coils *sec,
-                                    const int64_t *prim_index, const int64_t *sec_index, int64_t n_pairs,
-                                    const double *ring_offset, const int32_t *half_ring, uint32_t what,
-                                    int32_t precision, uint32_t flags, double *M, double *F, double *T,
-                                    double *dM, ccoil_report *report)
+namespace ccoil {
+
+ccoil_status build_pairs(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coils *sec, const int64_t *prim_index,
+                         const int64_t *sec_index, int64_t n_pairs, const double *ring_offset,
+                         const int32_t *half_ring, uint32_t what, int32_t precision, uint32_t flags, double *M,
+                         double *F, double *T, double *dM, int64_t target_override, PairBuild &out)
 {
     if (!ctx) return fail(nullptr, CCOIL_ERR_ARGUMENT, "ccoil_pairs: ctx is NULL");
     if (n_pairs < 0) return fail(ctx, CCOIL_ERR_ARGUMENT, "n_pairs is negative");
@@ -59,12 +60,14 @@ extern "C" ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, cons
     if ((what & CCOIL_PAIR_DM) && !dM) return fail(ctx, CCOIL_ERR_ARGUMENT, "dM requested but dM is NULL");
     if ((what & CCOIL_PAIR_DM) && (precision == CCOIL_FP32 || precision == CCOIL_FP64_REFERENCE))
         return fail(ctx, CCOIL_ERR_ARGUMENT, "CCOIL_PAIR_DM needs CCOIL_FP64, CCOIL_FP64_IEEE or CCOIL_FP32C");
-    std::vector<CoilDev> hp, hs;
+    std::vector<CoilDev> &hp = out.hp, &hs = out.hs;
+    out.sorted.clear();
+    out.items = out.node_pairs = 0;
+    out.max_row_groups = 1;
     std::string msg = pack_coils(prim, false, false, hp);
     if (msg.empty()) msg = pack_coils(sec, true, false, hs);
     if (!msg.empty()) return fail(ctx, CCOIL_ERR_ARGUMENT, msg);
     if (!prim->current || !prim->turns) return fail(ctx, CCOIL_ERR_ARGUMENT, "primary coils need current and turns");
-    if (report) std::memset(report, 0, sizeof *report);
     if (n_pairs == 0) return CCOIL_OK;
     if (hp.empty() || hs.empty()) return fail(ctx, CCOIL_ERR_ARGUMENT, "pairs given but a coil array is empty");
 
@@ -79,7 +82,8 @@ extern "C" ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, cons
         const CoilDev &p = hp[(size_t)pi], &s = hs[(size_t)si];
         node_pairs += (int64_t)s.blk_ang * s.n_ang * s.blk_thk * s.n_thk * s.blk_len * s.n_len * p.n_ang * p.n_thk;
     }
-    const int64_t target = lane_target(node_pairs);
+    const int64_t target = target_override > 0 ? target_override : lane_target(node_pairs);
+    out.target = target;
     int max_row_groups = 1;
     for (int64_t k = 0; k < n_pairs; ++k) {
         const int64_t pi = prim_index ? prim_index[k] : k, si = sec_index ? sec_index[k] : k;
@@ -112,7 +116,8 @@ extern "C" ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, cons
     std::vector<int32_t> order((size_t)n_pairs);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return cost[(size_t)x] > cost[(size_t)y]; });
-    std::vector<PairDev> sorted((size_t)n_pairs);
+    std::vector<PairDev> &sorted = out.sorted;
+    sorted.resize((size_t)n_pairs);
     // items are counted below; one workgroup per item must fit a 1-D grid
     int64_t items = 0;
     for (int64_t k = 0; k < n_pairs; ++k) {
@@ -124,27 +129,54 @@ extern "C" ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, cons
 
     if (items > INT32_MAX)
         return fail(ctx, CCOIL_ERR_ARGUMENT, "pair batch needs more than 2^31 - 1 work items; split the batch");
+    out.items = items;
+    out.node_pairs = node_pairs;
+    out.max_row_groups = max_row_groups;
+    return CCOIL_OK;
+}
+
+}  // namespace ccoil
+
+namespace ccoil {
+
+ccoil_status pairs_setup(ccoil_ctx ctx, const PairBuild &pb, uint32_t what, int32_t precision, DevBuf *const bufs[4],
+                         PairParams &pp)
+{
     CC_HIP(ctx, hipSetDevice(ctx->device));
-    const bool dev = flags & CCOIL_DEVICE_POINTERS;
-    CC_HIP(ctx, ensure(ctx->pprim, sizeof(CoilDev) * hp.size()));
-    CC_HIP(ctx, ensure(ctx->psec, sizeof(CoilDev) * hs.size()));
-    CC_HIP(ctx, ensure(ctx->ppairs, sizeof(PairDev) * sorted.size()));
-    CC_HIP(ctx, ensure(ctx->ppartial, sizeof(double) * kPairTerms * (size_t)items));
-    CC_HIP(ctx, hipMemcpyAsync(ctx->pprim.p, hp.data(), sizeof(CoilDev) * hp.size(), hipMemcpyHostToDevice, ctx->stream));
-    CC_HIP(ctx, hipMemcpyAsync(ctx->psec.p, hs.data(), sizeof(CoilDev) * hs.size(), hipMemcpyHostToDevice, ctx->stream));
-    CC_HIP(ctx, hipMemcpyAsync(ctx->ppairs.p, sorted.data(), sizeof(PairDev) * sorted.size(), hipMemcpyHostToDevice,
+    const std::vector<CoilDev> &hp = pb.hp, &hs = pb.hs;
+    const std::vector<PairDev> &sorted = pb.sorted;
+    CC_HIP(ctx, ensure(*bufs[0], sizeof(CoilDev) * hp.size()));
+    CC_HIP(ctx, ensure(*bufs[1], sizeof(CoilDev) * hs.size()));
+    CC_HIP(ctx, ensure(*bufs[2], sizeof(PairDev) * sorted.size()));
+    const int terms = (what & kPairGrad) ? kPairGradTerms : kPairTerms;
+    CC_HIP(ctx, ensure(*bufs[3], sizeof(double) * terms * (size_t)pb.items));
+    CC_HIP(ctx, hipMemcpyAsync(bufs[0]->p, hp.data(), sizeof(CoilDev) * hp.size(), hipMemcpyHostToDevice, ctx->stream));
+    CC_HIP(ctx, hipMemcpyAsync(bufs[1]->p, hs.data(), sizeof(CoilDev) * hs.size(), hipMemcpyHostToDevice, ctx->stream));
+    CC_HIP(ctx, hipMemcpyAsync(bufs[2]->p, sorted.data(), sizeof(PairDev) * sorted.size(), hipMemcpyHostToDevice,
                                ctx->stream));
-    PairParams pp;
     std::memset(&pp, 0, sizeof pp);
-    pp.prim = (const CoilDev *)ctx->pprim.p;
-    pp.sec = (const CoilDev *)ctx->psec.p;
-    pp.pairs = (const PairDev *)ctx->ppairs.p;
-    pp.n_pairs = (int32_t)n_pairs;
-    pp.n_items = items;
-    pp.partial = (double *)ctx->ppartial.p;
+    pp.prim = (const CoilDev *)bufs[0]->p;
+    pp.sec = (const CoilDev *)bufs[1]->p;
+    pp.pairs = (const PairDev *)bufs[2]->p;
+    pp.n_pairs = (int32_t)sorted.size();
+    pp.n_items = pb.items;
+    pp.partial = (double *)bufs[3]->p;
     pp.what = what;
     pp.precision = precision;
     pp.tab = ctx_tables(ctx);
+    return CCOIL_OK;
+}
+
+// One pair call from a built work list, on the context's buffers; synchronous.
+static ccoil_status pairs_run(ccoil_ctx ctx, const PairBuild &pb, uint32_t what, int32_t precision, uint32_t flags,
+                              double *M, double *F, double *T, double *dM, ccoil_report *report)
+{
+    const int64_t n_pairs = (int64_t)pb.sorted.size();
+    PairParams pp;
+    DevBuf *const bufs[4] = {&ctx->pprim, &ctx->psec, &ctx->ppairs, &ctx->ppartial};
+    ccoil_status st = pairs_setup(ctx, pb, what, precision, bufs, pp);
+    if (st != CCOIL_OK) return st;
+    const bool dev = flags & CCOIL_DEVICE_POINTERS;
     const size_t s1 = sizeof(double) * (size_t)n_pairs, s3 = 3 * s1;
     if (dev) {
         pp.M = M;
@@ -181,11 +213,114 @@ extern "C" ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, cons
         float ms = 0.f, mms = 0.f;
         CC_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         CC_HIP(ctx, hipEventElapsedTime(&mms, ctx->ev0, ctx->evm1));
-        report->node_evaluations = node_pairs;
+        report->node_evaluations = pb.node_pairs;
         report->kernel_ms = ms;
         report->main_kernel_ms = mms;
         report->block_size = 64;
-        report->split = max_row_groups;
+        report->split = pb.max_row_groups;
+    }
+    return CCOIL_OK;
+}
+
+ccoil_status pairs_tuned_target(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coils *sec,
+                                const int64_t *prim_index, const int64_t *sec_index, int64_t n_pairs,
+                                const double *ring_offset, const int32_t *half_ring, uint32_t what, int32_t precision,
+                                uint32_t flags, double *M, double *F, double *T, double *dM, const PairBuild &rule,
+                                int64_t &target)
+{
+    auto run = [&](int64_t t, double &ms) -> ccoil_status {
+        PairBuild pb;
+        ccoil_status st = build_pairs(ctx, prim, sec, prim_index, sec_index, n_pairs, ring_offset, half_ring, what,
+                                      precision, flags, M, F, T, dM, t, pb);
+        if (st != CCOIL_OK) return st;
+        ccoil_report rep;
+        st = pairs_run(ctx, pb, what, precision, flags, M, F, T, dM, &rep);
+        ms = rep.main_kernel_ms;
+        return st;
+    };
+    return tune_pairs(ctx, rule, what, precision, run, target);
+}
+
+}  // namespace ccoil
+
+extern "C" ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coils *sec,
+                                    const int64_t *prim_index, const int64_t *sec_index, int64_t n_pairs,
+                                    const double *ring_offset, const int32_t *half_ring, uint32_t what,
+                                    int32_t precision, uint32_t flags, double *M, double *F, double *T,
+                                    double *dM, ccoil_report *report)
+{
+    if (report) std::memset(report, 0, sizeof *report);
+    PairBuild pb;
+    ccoil_status st = build_pairs(ctx, prim, sec, prim_index, sec_index, n_pairs, ring_offset, half_ring, what,
+                                  precision, flags, M, F, T, dM, 0, pb);
+    if (st != CCOIL_OK || n_pairs == 0) return st;
+    if (flags & CCOIL_AUTOTUNE) {
+        int64_t target = pb.target;
+        st = pairs_tuned_target(ctx, prim, sec, prim_index, sec_index, n_pairs, ring_offset, half_ring, what,
+                                precision, flags, M, F, T, dM, pb, target);
+        if (st != CCOIL_OK) return st;
+        if (target != pb.target) {
+            st = build_pairs(ctx, prim, sec, prim_index, sec_index, n_pairs, ring_offset, half_ring, what, precision,
+                             flags, M, F, T, dM, target, pb);
+            if (st != CCOIL_OK) return st;
+        }
+    }
+    return pairs_run(ctx, pb, what, precision, flags, M, F, T, dM, report);
+}
+
+extern "C" ccoil_status ccoil_pairs_grad(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coils *sec,
+                                         const int64_t *prim_index, const int64_t *sec_index, int64_t n_pairs,
+                                         const double *ring_offset, const int32_t *half_ring, int32_t precision,
+                                         uint32_t flags, double *M, double *grad, ccoil_report *report)
+{
+    if (report) std::memset(report, 0, sizeof *report);
+    if (!ctx) return fail(nullptr, CCOIL_ERR_ARGUMENT, "ccoil_pairs_grad: ctx is NULL");
+    if (precision != CCOIL_FP64 && precision != CCOIL_FP64_IEEE && precision != CCOIL_FP32C)
+        return fail(ctx, CCOIL_ERR_ARGUMENT, "ccoil_pairs_grad supports CCOIL_FP64, CCOIL_FP64_IEEE and CCOIL_FP32C");
+    if (!grad) return fail(ctx, CCOIL_ERR_ARGUMENT, "grad is NULL");
+    PairBuild pb;
+    ccoil_status st = build_pairs(ctx, prim, sec, prim_index, sec_index, n_pairs, ring_offset, half_ring,
+                                  CCOIL_PAIR_M, precision, flags, M ? M : grad, nullptr, nullptr, nullptr, 0, pb);
+    if (st != CCOIL_OK || n_pairs == 0) return st;
+    PairParams pp;
+    DevBuf *const bufs[4] = {&ctx->pprim, &ctx->psec, &ctx->ppairs, &ctx->ppartial};
+    const uint32_t what = CCOIL_PAIR_M | kPairGrad;
+    st = pairs_setup(ctx, pb, what, precision, bufs, pp);
+    if (st != CCOIL_OK) return st;
+    const bool dev = flags & CCOIL_DEVICE_POINTERS;
+    const size_t s1 = sizeof(double) * (size_t)n_pairs, sg = kPairGradOut * s1;
+    if (dev) {
+        pp.M = M;
+        pp.grad = grad;
+    } else {
+        CC_HIP(ctx, ensure(ctx->pdM, sg));
+        pp.grad = (double *)ctx->pdM.p;
+        if (M) {
+            CC_HIP(ctx, ensure(ctx->pM, s1));
+            pp.M = (double *)ctx->pM.p;
+        }
+    }
+    CC_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    launch_pairs(ctx->stream, pp);
+    CC_HIP(ctx, hipGetLastError());
+    CC_HIP(ctx, hipEventRecord(ctx->evm1, ctx->stream));
+    launch_pair_reduce(ctx->stream, pp);
+    CC_HIP(ctx, hipGetLastError());
+    CC_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    if (!dev) {
+        CC_HIP(ctx, hipMemcpyAsync(grad, pp.grad, sg, hipMemcpyDeviceToHost, ctx->stream));
+        if (M) CC_HIP(ctx, hipMemcpyAsync(M, pp.M, s1, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (report) {
+        float ms = 0.f, mms = 0.f;
+        CC_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        CC_HIP(ctx, hipEventElapsedTime(&mms, ctx->ev0, ctx->evm1));
+        report->node_evaluations = pb.node_pairs;
+        report->kernel_ms = ms;
+        report->main_kernel_ms = mms;
+        report->block_size = 64;
+        report->split = pb.max_row_groups;
     }
     return CCOIL_OK;
 }

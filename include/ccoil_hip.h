@@ -37,7 +37,8 @@ enum {
     CCOIL_ERR_ARGUMENT = -1,      /* invalid sizes, pointers or precision values; nothing was launched */
     CCOIL_ERR_OUT_OF_MEMORY = -2, /* device or host allocation failed (reference: std::bad_alloc) */
     CCOIL_ERR_DEVICE = -3,        /* a HIP runtime call failed; see ccoil_last_error */
-    CCOIL_ERR_NO_DEVICE = -4      /* no HIP device available (reference: "GPU functions are disabled") */
+    CCOIL_ERR_NO_DEVICE = -4,     /* no HIP device available (reference: "GPU functions are disabled") */
+    CCOIL_ERR_CAPACITY = -5       /* a plan's launch needed more FP32C near-pass records than the plan holds */
 };
 
 /* Quantity mask for ccoil_fields. */
@@ -69,9 +70,15 @@ enum {
 enum {
     CCOIL_DEVICE_POINTERS = 1, /* points/outputs are device pointers on the context's device */
     CCOIL_NO_NEAR_FP64 = 2,    /* FP32C: do not re-evaluate near-winding or near-axis pairs in FP64 */
-    CCOIL_FORCE_REFERENCE_SCALE = 4 /* ccoil_pairs: force/torque scaled by 2*pi*N2/I1 as the reference's
+    CCOIL_FORCE_REFERENCE_SCALE = 4, /* ccoil_pairs: force/torque scaled by 2*pi*N2/I1 as the reference's
                                        single-pair arrangement path does (CalculateForceArrangements.cxx:125,
                                        SupportFunctions.cxx:238-241); default is the physical 2*pi*N2*I2 */
+    CCOIL_AUTOTUNE = 8         /* ccoil_fields, ccoil_pairs and the plans: take the launch shape (field: lanes per
+                                  point and coil chunks; pairs: primary-node evaluations per lane and work item)
+                                  from the context's tuning cache; the first call of a size not in the cache times
+                                  every candidate shape on the device (the call runs once per candidate) and caches
+                                  the fastest. A cached shape is replayed as is, so results are bitwise
+                                  reproducible for a given cache. */
 };
 
 /* Pair quantity mask for ccoil_pairs. CCOIL_PAIR_DM: gradient of M with respect to the secondary's position,
@@ -120,6 +127,11 @@ const char *ccoil_version(void);
  * the context's own (non-blocking) stream. */
 ccoil_status ccoil_set_stream(ccoil_ctx ctx, void *stream);
 ccoil_status ccoil_synchronize(ccoil_ctx ctx);
+/* Autotuner cache file (no counterpart in the reference): loads the entries of path (a text file, one "key<TAB>shape"
+ * line per tuned call size; missing file: empty) into the context's cache and appends every new entry to it. NULL
+ * keeps the cache in memory only. A context also loads the file named by the environment variable
+ * CCOIL_TUNING_FILE when it is created. */
+ccoil_status ccoil_set_tuning_file(ccoil_ctx ctx, const char *path);
 /* The stream the context's kernels are ordered on (its own, or the one given to ccoil_set_stream), as hipStream_t
  * cast to void*. No counterpart in the reference: zero-copy callers hand it to a producer (DLPack __dlpack__(stream))
  * so that the producer's pending writes are ordered before the context's kernels. */
@@ -147,6 +159,19 @@ ccoil_status ccoil_pairs(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coi
                          const double *ring_offset, const int32_t *half_ring,
                          uint32_t what, int32_t precision, uint32_t flags,
                          double *M, double *F, double *T, double *dM, ccoil_report *report);
+
+/* Gradient of each pair's mutual inductance with respect to both coils' parameters: the adjoint of M (no counterpart
+ * in the reference, whose force and torque are its physical counterparts: F = I1 I2 dM/d(secondary position),
+ * CalculateForceGeneral.cxx:114-128, and the torque about the secondary's centre is I1 I2 times the derivative of M
+ * with respect to a rotation of the secondary). Same pairs, rule and scale as ccoil_pairs with CCOIL_PAIR_M; the
+ * derivatives are those of that quadrature, with every quadrature order and the coplanar half-ring rule held fixed.
+ * grad: 16 per pair, dM/d(secondary x, y, z, alpha, beta, R, a, b) then dM/d(primary x, y, z, alpha, beta, R, a, b)
+ * (alpha: yAxisAngle, beta: zAxisAngle; a, b: the thickness and length as given). M does not depend on either coil's
+ * current. M (nullable): 1 per pair. precision: CCOIL_FP64, CCOIL_FP64_IEEE or CCOIL_FP32C. */
+ccoil_status ccoil_pairs_grad(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coils *sec,
+                              const int64_t *prim_index, const int64_t *sec_index, int64_t n_pairs,
+                              const double *ring_offset, const int32_t *half_ring, int32_t precision, uint32_t flags,
+                              double *M, double *grad, ccoil_report *report);
 
 /* Adjoint of the summed B field (no counterpart in the reference's CUDA interface; it backs the gradient of
  * scalar objectives of the field, e.g. a torch.autograd.Function over ccoil_fields). Given dL_dB (3 per point,
@@ -177,6 +202,29 @@ ccoil_status ccoil_pairs_zaxis(ccoil_ctx ctx, const ccoil_coils *prim, const cco
  * Replaces Coil::calculateSelfInductance (CalculateMInductanceZAxis.cxx:224-317). */
 ccoil_status ccoil_self_inductance(ccoil_ctx ctx, const ccoil_coils *coils, const int32_t *radial, uint32_t flags,
                                    double *L, ccoil_report *report);
+
+/* Plans (no counterpart in the reference): one field or pair call recorded once and captured into a HIP graph that
+ * every launch replays with one submission, no host work and no host synchronisation. The coil records, the pair
+ * work list, the launch shape (with CCOIL_AUTOTUNE, the tuned one) and every buffer are fixed when the plan is
+ * made; points and outputs must be device pointers (CCOIL_DEVICE_POINTERS), and their contents may change between
+ * launches while their addresses may not. Making a plan runs the call once (FP32C field plans: to size the near
+ * pass, which then holds a quarter more (coil, point) pairs than that call found). The context must outlive its
+ * plans, and a plan's launches must not overlap one another. */
+typedef struct ccoil_plan_s *ccoil_plan;
+ccoil_status ccoil_fields_plan(ccoil_ctx ctx, const ccoil_coils *coils, const double *points, int64_t n_points,
+                               uint32_t quantities, int32_t precision, uint32_t flags,
+                               double *A, double *B, double *E, double *G, ccoil_plan *plan);
+ccoil_status ccoil_pairs_plan(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coils *sec,
+                              const int64_t *prim_index, const int64_t *sec_index, int64_t n_pairs,
+                              const double *ring_offset, const int32_t *half_ring, uint32_t what, int32_t precision,
+                              uint32_t flags, double *M, double *F, double *T, double *dM, ccoil_plan *plan);
+/* Enqueues the plan's graph on stream (a hipStream_t cast to void*; NULL: the context's stream) and returns. */
+ccoil_status ccoil_plan_launch(ccoil_plan plan, void *stream);
+/* Waits for the plan's last launch and checks it: CCOIL_ERR_CAPACITY when an FP32C field launch found more near
+ * pairs than the plan holds (its outputs then lack the near contributions; make a new plan). report: the device
+ * time of the whole launch (kernel_ms = main_kernel_ms) and the launch's node evaluations and near pairs. */
+ccoil_status ccoil_plan_synchronize(ccoil_plan plan, ccoil_report *report);
+ccoil_status ccoil_plan_destroy(ccoil_plan plan);
 
 #ifdef __cplusplus
 }
