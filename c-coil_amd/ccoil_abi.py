@@ -21,7 +21,7 @@ HIP_STREAM_LEGACY = 1   # hipStreamLegacy: the default (null) stream as a stream
 
 EXPORTED = ["ccoil_create", "ccoil_destroy", "ccoil_last_error", "ccoil_version", "ccoil_set_stream",
             "ccoil_synchronize", "ccoil_gauss_legendre", "ccoil_fields", "ccoil_pairs", "ccoil_field_gradient",
-            "ccoil_pairs_zaxis", "ccoil_self_inductance", "ccoil_get_stream",
+            "ccoil_pairs_zaxis", "ccoil_self_inductance", "ccoil_get_stream", "ccoil_fields_noalloc",
             "ccoil_set_tuning_file", "ccoil_fields_plan", "ccoil_pairs_plan", "ccoil_plan_launch",
             "ccoil_plan_synchronize", "ccoil_plan_destroy", "ccoil_pairs_grad"]
 GRAD_TERMS = ("current", "R", "a", "b", "px", "py", "pz", "alpha", "beta")
@@ -68,6 +68,7 @@ def lib():
         L.ccoil_gauss_legendre.argtypes = [vp, ctypes.c_int32, dp, dp]
         L.ccoil_fields.argtypes = [vp, ctypes.POINTER(Coils), vp, ctypes.c_int64, ctypes.c_uint32, ctypes.c_int32,
                                    ctypes.c_uint32, vp, vp, vp, vp, ctypes.POINTER(Report)]
+        L.ccoil_fields_noalloc.argtypes = L.ccoil_fields.argtypes
         L.ccoil_set_tuning_file.argtypes = [vp, ctypes.c_char_p]
         L.ccoil_fields_plan.argtypes = [vp, ctypes.POINTER(Coils), vp, ctypes.c_int64, ctypes.c_uint32, ctypes.c_int32,
                                         ctypes.c_uint32, vp, vp, vp, vp, ctypes.POINTER(vp)]
@@ -88,7 +89,7 @@ def lib():
         L.ccoil_self_inductance.argtypes = [vp, ctypes.POINTER(Coils), vp, ctypes.c_uint32, vp,
                                             ctypes.POINTER(Report)]
         for f in ("ccoil_create", "ccoil_destroy", "ccoil_set_stream", "ccoil_synchronize", "ccoil_gauss_legendre",
-                  "ccoil_fields", "ccoil_set_tuning_file", "ccoil_fields_plan",
+                  "ccoil_fields", "ccoil_fields_noalloc", "ccoil_set_tuning_file", "ccoil_fields_plan",
                   "ccoil_pairs_plan", "ccoil_plan_launch", "ccoil_plan_synchronize", "ccoil_plan_destroy", "ccoil_pairs_grad", "ccoil_pairs", "ccoil_field_gradient", "ccoil_pairs_zaxis",
                   "ccoil_self_inductance"):
             getattr(L, f).restype = ctypes.c_int32
@@ -226,6 +227,15 @@ class Context:
         self._check(lib().ccoil_fields(self.h, ctypes.byref(coils.struct), _ptr(points), n, quantities, precision,
                                        flags, _ptr(out.get("A")), _ptr(out.get("B")), _ptr(out.get("E")),
                                        _ptr(out.get("G")), ctypes.byref(rep)))
+        return out, rep
+
+    def fields_noalloc(self, coils, points, quantities, precision, out):
+        """ccoil_fields_noalloc: torch tensors on the context's device (points (n, 3), out: dict of outputs)."""
+        rep = Report()
+        self._check(lib().ccoil_fields_noalloc(self.h, ctypes.byref(coils.struct), _ptr(points), points.shape[0],
+                                               quantities, precision, DEVICE_POINTERS, _ptr(out.get("A")),
+                                               _ptr(out.get("B")), _ptr(out.get("E")), _ptr(out.get("G")),
+                                               ctypes.byref(rep)))
         return out, rep
 
     def fields_plan(self, coils, points, quantities, precision, out, flags=0):

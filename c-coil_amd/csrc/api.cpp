@@ -224,6 +224,7 @@ ccoil_status ccoil_create(int device, ccoil_ctx *out)
     if ((e = hipEventCreate(&ctx->evm1)) != hipSuccess) return bail(e, "hipEventCreate");
     if ((e = hipHostMalloc((void **)&ctx->near_host, sizeof(NearState), hipHostMallocDefault)) != hipSuccess)
         return bail(e, "hipHostMalloc");
+    if ((e = scratch_address(device, &ctx->scratch)) != hipSuccess) return bail(e, "hipGetSymbolAddress(scratch)");
     if (const char *tf = std::getenv("CCOIL_TUNING_FILE")) load_tuning_file(ctx, tf);
     const size_t nd = sizeof(double) * kTableSize, nf = sizeof(float) * kTableSize;
     double **dp[11] = {&ctx->gl_x, &ctx->gl_w, &ctx->gl_cos, &ctx->gl_omc, &ctx->gl_rcos, &ctx->gl_rsin, &ctx->gl_wcos,

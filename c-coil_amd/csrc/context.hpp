@@ -65,6 +65,7 @@ struct ccoil_ctx_s {
     ccoil::DevBuf ncounts, noffsets, nhist, nstate, nkeys, nrecs, nvals, ntemp;   // FP32C near-winding pass
     int64_t near_cap = 0;                         // records the near buffers hold (grow-only)
     ccoil::NearState *near_host = nullptr;        // pinned copy of the near pass's outcome
+    void *scratch = nullptr;                      // the device's no-allocation scratch (noalloc.hip)
     ccoil::DevBuf pprim, psec, ppairs, ppartial, pcounter, pM, pF, pT, pdM;
     ccoil::DevBuf acoils, apts, agB, apartial, agrad;   // field adjoint
     ccoil::DevBuf zitems, zpartial, zout;               // coaxial rules
@@ -168,6 +169,10 @@ ccoil_status pairs_tuned_target(ccoil_ctx ctx, const ccoil_coils *prim, const cc
                                 uint32_t flags, double *M, double *F, double *T, double *dM, const PairBuild &rule,
                                 int64_t &target);
 ccoil_status load_tuning_file(ccoil_ctx ctx, const char *path);
+
+// The no-allocation scratch of the current device (noalloc.hip): its device address, and its size.
+hipError_t scratch_address(int device, void **out);
+size_t scratch_bytes();
 
 #define CC_HIP(ctx, call)                                                                                   \
     do {                                                                                                    \

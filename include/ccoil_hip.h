@@ -149,6 +149,19 @@ ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, const double 
                           uint32_t quantities, int32_t precision, uint32_t flags,
                           double *A, double *B, double *E, double *G, ccoil_report *report);
 
+/* ccoil_fields that allocates nothing. No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). Every buffer a field
+ * call needs beyond its inputs and outputs (coil records, coil-chunk partials, the FP32C near pass) comes from a
+ * fixed 64 MiB scratch array per device, held from the library's load until the process ends. Points and outputs
+ * must be device pointers (flags must hold CCOIL_DEVICE_POINTERS). A call whose buffers do not fit, or whose FP32C
+ * near pass finds more near pairs than the scratch holds, returns CCOIL_ERR_ARGUMENT (in the second case the
+ * outputs lack the near contributions): split the points. Results are bitwise those of ccoil_fields. All calls on
+ * one device share its scratch, which keeps the last call's contents: calls that overlap on one device (from two
+ * contexts or threads) race. */
+ccoil_status ccoil_fields_noalloc(ccoil_ctx ctx, const ccoil_coils *coils, const double *points, int64_t n_points,
+                                  uint32_t quantities, int32_t precision, uint32_t flags,
+                                  double *A, double *B, double *E, double *G, ccoil_report *report);
+
 /* Mutual inductance M and force/torque (F, T about the secondary centre) for n_pairs (primary, secondary)
  * pairs, each pair in its own pose. prim_index/sec_index select coils of prim/sec (NULL: identity).
  * ring_offset/half_ring (per pair, NULL: 0) reproduce the coplanar improved-precision rule of
