@@ -181,12 +181,12 @@ def main():
         return {"B": torch.empty((n, 3), dtype=torch.float64, device=dev),
                 "G": torch.empty((n, 9), dtype=torch.float64, device=dev)}
 
-    def timed(prec, steps, warmup, out):
+    def timed(prec, steps, warmup, out, flags=0):
         for _ in range(warmup):
-            ctx.fields(cs, d_pts, q, prec, abi.DEVICE_POINTERS, out=out)
+            ctx.fields(cs, d_pts, q, prec, abi.DEVICE_POINTERS | flags, out=out)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        reps = [ctx.fields(cs, d_pts, q, prec, abi.DEVICE_POINTERS, out=out)[1] for _ in range(steps)]
+        reps = [ctx.fields(cs, d_pts, q, prec, abi.DEVICE_POINTERS | flags, out=out)[1] for _ in range(steps)]
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / steps
         main_ms = float(np.mean([r.main_kernel_ms for r in reps]))
@@ -203,8 +203,9 @@ def main():
 
     fp64_head = args.precision == "fp64"
     head_prec = abi.FP64 if fp64_head else abi.FP32C
+    head_flags = 0
     head = new_out()
-    dt, main_ms, all_ms, rep = timed(head_prec, args.steps, args.warmup, head)
+    dt, main_ms, all_ms, rep = timed(head_prec, args.steps, args.warmup, head, head_flags)
     value = len(rows) * n / dt
     log(f"{args.precision}: {dt * 1e3:.2f} ms/step, field kernel {main_ms:.2f} ms, all kernels {all_ms:.2f} ms")
     rl = roofline(main_ms, RM.PEAK_FP64_TFLOPS if fp64_head else RM.PEAK_FP32_TFLOPS)
@@ -219,17 +220,21 @@ def main():
 
     if not args.no_side:
         side, outs = {}, {}
-        for name, prec, peak in (("fp32c", abi.FP32C, RM.PEAK_FP32_TFLOPS),
-                                 ("fp64_ieee", abi.FP64_IEEE, RM.PEAK_FP64_TFLOPS),
-                                 ("fp64_reference", abi.FP64_REFERENCE, RM.PEAK_FP64_TFLOPS)):
-            if prec == head_prec:
+        # modes that evaluate every pair with the rule carry the rule's roofline; the far-field series modes (FP32C
+        # by default, FP64 with CCOIL_FAR_FIELD) skip most of the rule's nodes, so the node model does not apply
+        for name, prec, flags, peak in (("fp32c", abi.FP32C, 0, None),
+                                        ("fp32c_direct", abi.FP32C, abi.DIRECT, RM.PEAK_FP32_TFLOPS),
+                                        ("fp64_far_field", abi.FP64, abi.FAR_FIELD, None),
+                                        ("fp64_ieee", abi.FP64_IEEE, 0, RM.PEAK_FP64_TFLOPS),
+                                        ("fp64_reference", abi.FP64_REFERENCE, 0, RM.PEAK_FP64_TFLOPS)):
+            if prec == head_prec and flags == head_flags:
                 continue
             outs[name] = new_out()
-            sdt, sms, sall, srep = timed(prec, args.side_steps, 1, outs[name])
+            sdt, sms, sall, srep = timed(prec, args.side_steps, 1, outs[name], flags)
             side[name] = {"value": len(rows) * n / sdt, "ms_per_step": sdt * 1e3, "kernel_ms": sms, "all_ms": sall,
-                          "roofline_frac": roofline(sms, peak)["frac"], "near_pairs": int(srep.near_pairs),
-                          "value_over_headline": (len(rows) * n / sdt) / value}
-            log(f"{name}: {sdt * 1e3:.1f} ms/step, field kernel {sms:.2f} ms")
+                          "roofline_frac": roofline(sms, peak)["frac"] if peak else None,
+                          "near_pairs": int(srep.near_pairs), "value_over_headline": (len(rows) * n / sdt) / value}
+            log(f"{name}: {sdt * 1e3:.1f} ms/step, field kernel {sms:.2f} ms, all kernels {sall:.2f} ms")
         result["side"] = side
         if fp64_head:
             b64, g64 = head["B"].cpu().numpy(), head["G"].cpu().numpy()
@@ -247,7 +252,7 @@ def main():
             mb, mg = mb.cpu().numpy(), mg.cpu().numpy()
             acc["conditioned"] = ("error over the sum of the coils' magnitudes at the point, sum_k |B_k| "
                                   "(sum_k |G_k| for the gradient)")
-            for name in ("fp32c", "fp64_ieee"):
+            for name in ("fp32c", "fp32c_direct", "fp64_far_field", "fp64_ieee"):
                 ob, og = outs[name]["B"].cpu().numpy(), outs[name]["G"].cpu().numpy()
                 eb, okb = rel_err(ob, b64)
                 eg, okg = rel_err(og, g64)

@@ -20,7 +20,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_SOURCES = ["csrc/field_kernels.hip", "csrc/pair_kernels.hip", "csrc/adjoint_kernels.hip", "csrc/api.cpp",
                "csrc/pairs_api.cpp", "csrc/adjoint_api.cpp", "csrc/zaxis_kernels.hip", "csrc/zaxis_api.cpp", "csrc/near_kernels.hip",
                "csrc/noalloc.hip", "csrc/fields_api.cpp", "csrc/tune.cpp",
-               "csrc/plan_api.cpp"]
+               "csrc/plan_api.cpp", "csrc/multipole_kernels.hip"]
 HOST_SOURCES = ["host/Tensor.cpp", "host/Coil.cpp", "host/CoilGroup.cpp", "host/Planning.cpp", "host/GpuBridge.cpp"]
 PY_SOURCES = ["python/bindings.cpp"]
 

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libccoil_hip.so")
 OK, ERR_ARGUMENT, ERR_OUT_OF_MEMORY, ERR_DEVICE, ERR_NO_DEVICE, ERR_CAPACITY = 0, -1, -2, -3, -4, -5
 A, B, E, G, G_JACOBIAN = 1, 2, 4, 8, 16
 FP64, FP32C, FP32, FP64_REFERENCE, FP64_IEEE = 0, 1, 2, 3, 4
-DEVICE_POINTERS, NO_NEAR_FP64, FORCE_REFERENCE_SCALE, AUTOTUNE = 1, 2, 4, 8
+DEVICE_POINTERS, NO_NEAR_FP64, FORCE_REFERENCE_SCALE, AUTOTUNE, DIRECT, FAR_FIELD = 1, 2, 4, 8, 16, 32
 PAIR_M, PAIR_FT, PAIR_DM = 1, 2, 4
 HIP_STREAM_LEGACY = 1   # hipStreamLegacy: the default (null) stream as a stream handle
 

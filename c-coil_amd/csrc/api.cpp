@@ -159,6 +159,9 @@ std::string pack_coils(const ccoil_coils *c, bool need_len, bool need_freq, std:
         rotation_generators(c->alpha[i], c->beta[i], d.om_a, d.om_b);
         d.cur_used = d.current;
         d.adj_scale = 1.0;
+        d.ni = c->J[i] * d.a * d.b;
+        d.rmax = std::hypot(d.R + d.a, d.use_fast ? 0.5 * d.b : 0.0);   // the slow rule's winding is a flat sheet
+        d.far_d2 = 0.0;
         out[(size_t)i] = d;
     }
     return "";
@@ -260,7 +263,7 @@ ccoil_status ccoil_destroy(ccoil_ctx ctx)
         if (p) (void)hipFree(p);
     DevBuf *bufs[] = {&ctx->coils, &ctx->fpartial, &ctx->pts, &ctx->outA, &ctx->outB, &ctx->outE, &ctx->outG,
                       &ctx->ncounts, &ctx->noffsets, &ctx->nhist, &ctx->nstate, &ctx->nkeys, &ctx->nrecs,
-                      &ctx->nvals, &ctx->ntemp, &ctx->pprim, &ctx->psec, &ctx->ppairs, &ctx->ppartial, &ctx->pcounter, &ctx->pM, &ctx->pF, &ctx->pT, &ctx->pdM, &ctx->acoils, &ctx->apts, &ctx->agB,
+                      &ctx->nvals, &ctx->ntemp, &ctx->fmp, &ctx->forder, &ctx->ftemp, &ctx->pprim, &ctx->psec, &ctx->ppairs, &ctx->ppartial, &ctx->pcounter, &ctx->pM, &ctx->pF, &ctx->pT, &ctx->pdM, &ctx->acoils, &ctx->apts, &ctx->agB,
                       &ctx->apartial, &ctx->agrad, &ctx->zitems, &ctx->zpartial, &ctx->zout};
     for (auto b : bufs) free_buf(*b);
     if (ctx->near_host) (void)hipHostFree(ctx->near_host);

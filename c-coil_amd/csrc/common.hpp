@@ -30,6 +30,9 @@ struct alignas(64) CoilDev {
     double om_a[3], om_b[3]; // T^T dT/dalpha and T^T dT/dbeta as axial vectors (Omega v = om x v)
     double cur_used;         // current the constFactor was formed with (1 for a zero-current coil)
     double adj_scale;        // I / cur_used: scales every adjoint term except d/dI
+    double ni;               // N I: the coil's total current (J a b)
+    double rmax;             // radius of the sphere about the centre that holds the winding, |(R + a, b / 2)|
+    double far_d2;           // squared distance beyond which the far-field series replaces the rule (0: never)
     int32_t use_fast;
     int32_t n_ang, n_thk, n_len;
     int32_t blk_ang, blk_thk, blk_len;
@@ -157,6 +160,15 @@ __device__ inline void accumulate_global(const CoilDev &c, const Local &L, doubl
 //    and cylinder around the axis, r < 0.1 sqrt(r^2 + z^2 + R^2): the float angular sums of B_rho and A vanish like
 //    r while their rounding does not.
 constexpr double kAxisCone2 = 0.01;
+
+// Whether the far-field series replaces the quadrature rule for a point at local (z, r). Rounded operation by
+// operation, like to_local, so every kernel that classifies a pair gets the same answer.
+__device__ inline bool is_far(const CoilDev &c, double z, double r)
+{
+#pragma clang fp contract(off)
+    return c.far_d2 > 0.0 && r * r + z * z > c.far_d2;
+}
+
 __device__ inline bool is_near(const CoilDev &c, double z, double r, bool axis)
 {
 #pragma clang fp contract(off)
@@ -164,6 +176,12 @@ __device__ inline bool is_near(const CoilDev &c, double z, double r, bool axis)
     double dz = fmax(fabs(z) - 0.5 * c.b, 0.0);
     const double r2 = r * r;
     return dr * dr + dz * dz < c.near_w2 || (axis && r2 < kAxisCone2 * (r2 + z * z + c.R * c.R));
+}
+
+// The pairs of the FP32C near pass: near and not far (the far-field series is FP64 and exact on the axis).
+__device__ inline bool near_pass_pair(const CoilDev &c, double z, double r, bool axis)
+{
+    return !is_far(c, z, r) && is_near(c, z, r, axis);
 }
 
 }  // namespace ccoil

@@ -63,6 +63,7 @@ struct ccoil_ctx_s {
     float *gl_opcf = nullptr, *gl_wcsf = nullptr;
     ccoil::DevBuf coils, pts, outA, outB, outE, outG, fpartial;
     ccoil::DevBuf ncounts, noffsets, nhist, nstate, nkeys, nrecs, nvals, ntemp;   // FP32C near-winding pass
+    ccoil::DevBuf fmp, forder, ftemp;                                            // FP32C far field
     int64_t near_cap = 0;                         // records the near buffers hold (grow-only)
     ccoil::NearState *near_host = nullptr;        // pinned copy of the near pass's outcome
     void *scratch = nullptr;                      // the device's no-allocation scratch (noalloc.hip)
@@ -93,7 +94,18 @@ struct FieldLaunch {
     bool near = false;          // FP32C near pass
     void *scan_temp = nullptr;  // scan storage of the near pass (tile sums)
     size_t scan_bytes = 0;
+    void *order_temp = nullptr; // FP32C far field: storage of the visit-order sort
+    size_t order_scan_bytes = 0;
+    int order_bits = 0;
+    size_t far_bytes = 0;       // device bytes of the far-field buffers (moments, order, sort storage)
 };
+// Far field: the series replaces the rule where a bound on the angular rule's error is below kFarTolerance
+// (FP32C) or kFarToleranceFP64 (FP64 with CCOIL_FAR_FIELD) of the coil's field (far_threshold); a spatial visit order
+// from kOrderMinPoints points on.
+constexpr double kFarTolerance = 1e-11;
+constexpr double kFarToleranceFP64 = 1e-14;
+constexpr int64_t kOrderMinPoints = 16384;
+double far_threshold(int n_ang, double q, double tol);
 // Enqueues the field kernel, the chunk reduction and (FP32C) the whole near pass on s, with no host
 // synchronisation and kernel launches and copies only (so it can be captured in a graph); evm0/evm1 (may be NULL)
 // bracket the field kernel. The near pass's outcome is copied to host_state (pinned) when given.

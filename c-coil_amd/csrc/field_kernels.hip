@@ -8,6 +8,7 @@
 // the result array each time (FieldBGroup.cu:175-181).
 #include "kernels.hpp"
 #include "integrands.hpp"
+#include "multipole.hpp"
 
 namespace ccoil {
 
@@ -98,11 +99,12 @@ __global__ __launch_bounds__(kFieldBlock) __attribute__((amdgpu_waves_per_eu(kFi
 void field_kernel(FieldParams p)
 {
     const int64_t t = (int64_t)blockIdx.x * kFieldBlock + threadIdx.x;
-    const int64_t pi = t / SPLIT;
+    const int64_t vi = t / SPLIT;   // place in the visit order
     const int lane = (int)(t % SPLIT);
-    const bool active = pi < p.n_pts;
-    const int64_t pc = active ? pi : (p.n_pts - 1);   // inactive lanes shadow a valid point
-    const double px = p.pts[3 * pc], py = p.pts[3 * pc + 1], pz = p.pts[3 * pc + 2];
+    const bool active = vi < p.n_pts;
+    const int64_t vc = active ? vi : (p.n_pts - 1);   // inactive lanes shadow a valid point
+    const int64_t pi = p.order ? (int64_t)p.order[vc] : vc;
+    const double px = p.pts[3 * pi], py = p.pts[3 * pi + 1], pz = p.pts[3 * pi + 2];
 
     Cart acc;
     for (int k = 0; k < 3; ++k) { acc.A[k] = 0.0; acc.E[k] = 0.0; acc.B[k] = 0.0; }
@@ -115,18 +117,32 @@ void field_kernel(FieldParams p)
         double z, r, cph, sph;
         to_local(c, px, py, pz, z, r, cph, sph);
         Local L = {0, 0, 0, 0, 0, 0, 0};
-        if constexpr (PREC == kFP64) {
-            eval_stable<f64s, WA, WB, WG>(c, p.tab, z, r, lane, SPLIT, L);
-        } else if constexpr (PREC == kFP64IEEE) {
+        if constexpr (PREC == kFP64IEEE) {
             eval_stable<double, WA, WB, WG>(c, p.tab, z, r, lane, SPLIT, L);
         } else if constexpr (PREC == kFP64Reference) {
             eval_plain<double, WA, WB, WG>(c, p.tab, z, r, lane, SPLIT, L);
         } else if constexpr (PREC == kFP32) {
             eval_plain<float, WA, WB, WG>(c, p.tab, z, r, lane, SPLIT, L);
         } else {
-            const bool skip = p.near_fp64 && is_near(c, z, r, WG);
-            if (!skip) eval_stable<float, WA, WB, WG>(c, p.tab, z, r, lane, SPLIT, L);
-            else ++n_near;
+            // FP64 and FP32C: beyond the coil's far-field distance (set only when the call uses the series) the FP64
+            // series; FP32C: inside the near shell (or axis cone) nothing here (the near pass adds those pairs in
+            // FP64); elsewhere the rule
+            const bool far = is_far(c, z, r);
+            const bool skip = PREC == kFP32C && !far && p.near_fp64 && is_near(c, z, r, WG);
+            if (!far && !skip) {
+                if constexpr (PREC == kFP64) eval_stable<f64s, WA, WB, WG>(c, p.tab, z, r, lane, SPLIT, L);
+                else eval_stable<float, WA, WB, WG>(c, p.tab, z, r, lane, SPLIT, L);
+            }
+            if (skip) ++n_near;
+            if (__ballot(far)) {
+                // the loop runs to the wave's largest term count (uniform bounds and coefficient loads); each lane
+                // stops adding at its own count, so its value does not depend on the other points of its wave
+                const int own = far ? multipole_terms(r * r + z * z, c.rmax) : 0;
+                const int terms = wave_max_int(own);
+                if (far && lane == 0)
+                    eval_multipole<WA, WB, WG>(p.mp + kMpHeader + (size_t)ci * kMpCoil, p.mp, c.rmax, z, r, terms,
+                                               own, L);
+            }
         }
         if (SPLIT > 1) reduce_local<SPLIT>(L);
         accumulate_global(c, L, r, cph, sph, p.q, acc);

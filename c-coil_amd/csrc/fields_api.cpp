@@ -2,9 +2,11 @@
 // the coil records, the launch shape, the buffers (grow-only, or parts of a fixed arena), the kernels and the FP32C
 // near pass.
 #include "context.hpp"
+#include "multipole.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
 
 using namespace ccoil;
@@ -43,6 +45,40 @@ FieldShape field_shape_rule(ccoil_ctx ctx, int64_t n, int64_t n_coils, uint32_t 
     }
     sh.chunks = (int)std::max<int64_t>(1, std::min<int64_t>(chunks, 65535));
     return sh;
+}
+
+// Distance, in units of the coil's r_max, beyond which the FP32C far-field series replaces the rule (0: never).
+//
+// The series sums the rule's own rings exactly, so it departs from the rule by the angular rule's error and its
+// truncation (below 1e-16 from D = 2 r_max on). The angular integrand over phi in [0, pi] (Gauss-Legendre on
+// phi = pi (1 + x) / 2, n_ang nodes) is singular where the distance from the point to the ring vanishes, at
+// phi = +-i eta with cosh(eta) = 1 + d^2 / (2 r rho), d the point's distance from the ring in the meridian plane;
+// for every point at distance D from the centre and every ring of the winding, d >= D - r_max and r rho <= D (R + a),
+// so eta >= acosh(1 + (D - r_max)^2 / (2 D (R + a))). The singularity sits at x_s = -1 + 2 i eta / pi, outside the
+// Bernstein ellipse of parameter rho_B = |x_s - sqrt(x_s^2 - 1)| (the root beyond 1), and the rule's error falls like
+// rho_B^(-2 n_ang). Relative to the coil's field at D the error is at most 1000 y^2 rho_B^(-2 n_ang) (y = D / r_max;
+// the y^2 for the node terms that cancel down to the far field, the constant 10x above what a filament, a thin and a
+// thick coil measured in FP64 against the series at n_ang = 4..24 and y = 2..50). The threshold is the least y in
+// [2, 100] with that bound below tol: for n_ang >= 5 the bound falls monotonically there, so a bisection finds it.
+double far_threshold(int n_ang, double q, double tol)
+{
+    auto bound = [&](double y) {
+        const double d = y - 1.0;
+        const double eta = std::acosh(1.0 + d * d * q / (2.0 * y));
+        const std::complex<double> xs(-1.0, 2.0 * eta / kPi);
+        const std::complex<double> sq = std::sqrt(xs * xs - 1.0);
+        const double rho = std::max(std::abs(xs + sq), std::abs(xs - sq));
+        return 1000.0 * y * y * std::pow(rho, -2.0 * n_ang);
+    };
+    if (n_ang < 5) return 0.0;
+    if (bound(2.0) <= tol) return 2.0;
+    if (bound(100.0) > tol) return 0.0;
+    double lo = 2.0, hi = 100.0;
+    for (int it = 0; it < 40; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        (bound(mid) <= tol ? hi : lo) = mid;
+    }
+    return hi;
 }
 
 // Sizes the near pass's buffers of L for cap records and points L.p at them: the context's grow-only buffers, or
@@ -106,6 +142,10 @@ void enqueue_near_tail(hipStream_t s, const FieldLaunch &L, NearState *host_stat
 void enqueue_fields(hipStream_t s, const FieldLaunch &L, hipEvent_t evm0, hipEvent_t evm1, NearState *host_state)
 {
     const FieldParams &p = L.p;
+    if (p.mp) launch_multipole_moments(s, p.coils, p.n_coils, p.tab, const_cast<double *>(p.mp));
+    if (p.order)
+        (void)launch_visit_order(s, p.pts, p.n_pts, L.order_bits, L.order_temp, L.order_scan_bytes,
+                                 const_cast<uint32_t *>(p.order));
     if (L.near) {
         launch_fill_zero(s, p.near_counts, p.n_pts);
         launch_fill_zero(s, p.near_hist, ((int64_t)p.n_coils + 2) / 2);
@@ -159,6 +199,9 @@ ccoil_status fields_prepare(ccoil_ctx ctx, const ccoil_coils *coils, const doubl
     p.q = quantities;
     L.precision = precision;
     L.near = precision == CCOIL_FP32C && !(flags & CCOIL_NO_NEAR_FP64);
+    const bool far = (precision == CCOIL_FP32C && !(flags & CCOIL_DIRECT)) ||
+                     (precision == CCOIL_FP64 && (flags & CCOIL_FAR_FIELD));
+    const double far_tol = precision == CCOIL_FP64 ? kFarToleranceFP64 : kFarTolerance;
     if (dev) {
         p.pts = points;
         p.A = (quantities & CCOIL_A) ? A : nullptr;
@@ -169,7 +212,13 @@ ccoil_status fields_prepare(ccoil_ctx ctx, const ccoil_coils *coils, const doubl
     out.width = ((quantities & CCOIL_A) ? 3 : 0) + ((quantities & CCOIL_E) ? 3 : 0) +
                 ((quantities & CCOIL_B) ? 3 : 0) + ((quantities & (CCOIL_G | CCOIL_G_JACOBIAN)) ? 9 : 0);
     out.nodes = 0;
-    for (const auto &c : hc) out.nodes += (int64_t)c.n_ang * c.n_thk;
+    for (auto &c : hc) {
+        out.nodes += (int64_t)c.n_ang * c.n_thk;
+        if (far) {
+            const double y = far_threshold(c.n_ang, c.rmax / (c.R + c.a), far_tol);
+            c.far_d2 = y > 0.0 ? (y * c.rmax) * (y * c.rmax) : 0.0;
+        }
+    }
     if (n == 0 || hc.empty()) return CCOIL_OK;   // nothing to evaluate, or every field zero
 
     // The FP32C near pass sorts its (coil, point) records by near_key: coils with the most nodes first, so its
@@ -213,6 +262,31 @@ ccoil_status fields_prepare(ccoil_ctx ctx, const ccoil_coils *coils, const doubl
                          &partial);
         if (st != CCOIL_OK) return st;
         p.partial = (double *)partial;
+    }
+    if (far) {
+        // the series' moments, and the visit order when there are enough points for the spatial sort to pay
+        L.far_bytes = sizeof(double) * ((size_t)kMpHeader + (size_t)kMpCoil * hc.size()) + 256;
+        void *mp = nullptr;
+        st = call_buffer(ctx, ctx->fmp, arena, sizeof(double) * ((size_t)kMpHeader + (size_t)kMpCoil * hc.size()), &mp);
+        if (st != CCOIL_OK) return st;
+        p.mp = (const double *)mp;
+        if (n >= kOrderMinPoints) {
+            // about 16-32 points per cell: 2^bits cells per axis, bits = log8(n / 16), between 3 and 6
+            int bits = 3;
+            while (bits < 6 && ((int64_t)1 << (3 * (bits + 1))) * 16 <= n) ++bits;
+            const int64_t cells = (int64_t)1 << (3 * bits);
+            size_t sb = 0;
+            CC_HIP(ctx, near_scan(ctx->stream, nullptr, nullptr, cells, nullptr, sb));
+            const size_t ob = sizeof(uint32_t) * (size_t)n, tb = visit_order_bytes(n, bits, sb);
+            void *order = nullptr, *temp = nullptr;
+            if ((st = call_buffer(ctx, ctx->forder, arena, ob, &order)) != CCOIL_OK) return st;
+            if ((st = call_buffer(ctx, ctx->ftemp, arena, tb, &temp)) != CCOIL_OK) return st;
+            L.far_bytes += ob + tb + 2 * 256;
+            p.order = (const uint32_t *)order;
+            L.order_temp = temp;
+            L.order_scan_bytes = sb;
+            L.order_bits = bits;
+        }
     }
     if (L.near) {
         // Buffers for the records: every (coil, point) pair when that is small, else a capacity kept from earlier

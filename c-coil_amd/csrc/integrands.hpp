@@ -267,15 +267,18 @@ struct AdjLocal {
     double dR[2], da[2], db[2];
 };
 
-// Pair adjoint extras of one (coil, point): derivatives of the local A_phi with respect to the coil's inner radius
-// R, thickness a and length b. Row j of the rule is A_j = W_j S_j with W_j = cf w_j rho_j: for the sheet (fast
-// method) cf = 1e-7 N I pi / (2 b) and S_j = sum w cos (asinh(z1 / sqrt M) - asinh(z2 / sqrt M)), so A_j is
-// homogeneous of degree 0 in (rho, b, r, z); for the ring (slow method) cf = 1e-7 N I pi / 2 and A_j is homogeneous
-// of degree 0 in (rho, r, z). Hence
-//   rho dA_j/drho = -(r dA_j/dr + z dA_j/dz + b dA_j/db),  dA_j/dr = B_z,j - A_j / r,  dA_j/dz = -B_rho,j,
-// with the row's own B, and, for the sheet, dA_j/db = -A_j / b + W_j sum w cos (K1 + K2) / 2 (the end rings).
+// Pair adjoint extras of one (coil, point): the radial derivative of the local A_phi of the same rule, dA/dr (the
+// discrete B_z equals A/r + dA/dr only up to the angular rule's error, so the derivative is summed node by node), and
+// the derivatives of A_phi with respect to the coil's inner radius R, thickness a and length b. Row j of the rule is
+// A_j = W_j S_j with W_j = cf w_j rho_j: for the sheet (fast method) cf = 1e-7 N I pi / (2 b) and
+// S_j = sum w cos (asinh(z1 / sqrt M) - asinh(z2 / sqrt M)), whose node terms have dS/dM = -(z1 K1 - z2 K2) / (2 M)
+// and dM/dr = 2 (r - rho cos), so dA_j/dr = -W_j sum w cos (r - rho cos) (z1 K1 - z2 K2) / M; A_j is homogeneous of
+// degree 0 in (rho, b, r, z). For the ring (slow method) cf = 1e-7 N I pi / 2, S_j = sum w cos K, dA_j/dr =
+// -W_j sum w cos (r - rho cos) K^3, and A_j is homogeneous of degree 0 in (rho, r, z). Hence
+//   rho dA_j/drho = -(r dA_j/dr + z dA_j/dz + b dA_j/db),  dA_j/dz = -B_rho,j (exact node by node),
+// and, for the sheet, dA_j/db = -A_j / b + W_j sum w cos (K1 + K2) / 2 (the end rings).
 struct AdjA {
-    double dR, da, db;
+    double dR, da, db, Ar;
 };
 
 // Angular-loop unrolling of eval_stable: 4, except 2 for the float B / gradient instances (the FP32C field kernel
@@ -300,7 +303,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
     const int na = c.n_ang, nt = c.n_thk;
     double aA = 0, aBh = 0, aBz = 0, aRR = 0, aRZ = 0, aZZ = 0;
     double hR0 = 0, hR1 = 0, ha0 = 0, ha1 = 0, hb0 = 0, hb1 = 0;   // adjoint row sums
-    double qR = 0, qa = 0, qb = 0;                                  // pair adjoint row sums (A_phi)
+    double qR = 0, qa = 0, qb = 0, qAr = 0;                         // pair adjoint row sums (A_phi)
 
     if (c.use_fast) {
         // The sheet's field is mirror-symmetric in z: B_rho, dB_rho/drho, dB_z/dz (and the adjoint's rho
@@ -332,6 +335,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             T rBzs = 0;             // double without the gradient: sum w s dgM (rBz then holds sum w dgM)
             T eR = 0, eZ = 0;   // adjoint: end rings, sum w cos (z1 K1^3 + z2 K2^3), sum w (rho - r cos)(K1^3 + K2^3)
             T eA = 0;           // pair adjoint: end rings of A, sum w cos (K1 + K2)
+            T rAr = 0;          // pair adjoint: sum w cos (r - rho cos) dgM
 #pragma unroll kStableUnroll<T, WA>
             for (int i = 0; i < na; ++i) {
                 const T s = st[i], w = wa[i], wc = wct[i];
@@ -368,7 +372,10 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                     const T invM = gsel * Rr;
                     const T dgM = (same ? tP2 * M : fma_<T>(z1, K1, -g2)) * Rr;   // (z1 K1 - z2 K2) / M
                     rBh = fma_<T>(wc, -dK, rBh);       // B_rho, also the gradient's rho-phi term
-                    if constexpr (ADA) eA = fma_<T>(wc, Ks, eA);
+                    if constexpr (ADA) {
+                        eA = fma_<T>(wc, Ks, eA);
+                        rAr = fma_<T>(wc * fma_<T>(rho, s, -dm), dgM, rAr);   // r - rho cos = rho s - (rho - r)
+                    }
                     constexpr bool kBzMoments = sizeof(T) == 8 && !WG;
                     T wr = 0, rmc = 0;
                     if constexpr (kBzMoments) {
@@ -425,10 +432,12 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             aA += W * (double)rA;
             if constexpr (ADA) {
                 const double rowA = W * (double)rA, dAdb = W * 0.5 * (double)eA - rowA / c.b;
-                const double dAdrho = -(rd * W * rowBz - rowA - zm * W * (double)rBh + c.b * dAdb) / rho_d;
+                const double dAdr = -W * (double)rAr;
+                const double dAdrho = -(rd * dAdr - zm * W * (double)rBh + c.b * dAdb) / rho_d;
                 qR += dAdrho;
                 qa += 0.5 * (1.0 + xt[j]) * dAdrho;
                 qb += dAdb;
+                qAr += dAdr;
             }
             if constexpr (AD) {
                 const double Wr = c.cf * wt[j], hx = 0.5 * (1.0 + xt[j]);
@@ -448,6 +457,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             ada->dR = qR;
             ada->da = qa;
             ada->db = qb;
+            ada->Ar = qAr;
         }
         if constexpr (AD) {
             ad->dR[0] = -sgn * hR0;
@@ -478,6 +488,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             const T k2 = (T)(rho_d * rd);
             T rBh = 0, rBz = 0, rRR = 0, rRZ = 0, rZZ = 0, rA = 0;
             T sRRs = 0, sRRc = 0;   // float: sum w cos s K^5 and sum w cos K^5
+            T rC2 = 0;              // pair adjoint: sum w cos^2 K^3 (float: accumulated as sum w cos s K^3 first)
             if constexpr (sizeof(T) == 8) {
                 // Double: the angular sums as moments in s = 1 - cos of positive terms, b_k = sum w s^k K^3 and
                 // g_k = sum w s^k K^5, combined once per row: (1 - s)(rho s - d), d + r s and the rho-z polynomial
@@ -485,7 +496,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                 // (the cos-weighted first harmonic). The weights w s and w s^2 are tabulated, so each moment is one
                 // FMA per node.
                 const double *wst = tab.ws + table_offset(c.n_ang), *ws2t = tab.ws2 + table_offset(c.n_ang);
-                T b0 = 0, b1 = 0, g0 = 0, g1 = 0, g2 = 0;
+                T b0 = 0, b1 = 0, b2 = 0, g0 = 0, g1 = 0, g2 = 0;
 #pragma unroll kStableUnroll<T, WA>
                 for (int i = 0; i < na; ++i) {
                     const T s = st[i], w = wa[i];
@@ -496,6 +507,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                         const T K3 = K2 * K;
                         b0 = fma_<T>(w, K3, b0);
                         b1 = fma_<T>(wst[i], K3, b1);
+                        if constexpr (ADA) b2 = fma_<T>(ws2t[i], K3, b2);
                         if (WG) {
                             const T K5 = K3 * K2;
                             g0 = fma_<T>(w, K5, g0);
@@ -506,6 +518,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                     if (WA) rA = fma_<T>(wct[i], K, rA);
                 }
                 rBh = b0 - b1;                                  // sum w cos K^3
+                if constexpr (ADA) rC2 = b0 - 2 * b1 + b2;      // sum w cos^2 K^3
                 rBz = fma_<T>(dm, b0, rf * b1);                 // sum w (rho - r cos) K^3
                 rRR = fma_<T>(-dm, g0, fma_<T>(rho + dm, g1, -rho * g2));   // sum w cos (r - rho cos) K^5
                 rZZ = fma_<T>(dm, g0, rf * g1);                 // sum w (rho - r cos) K^5
@@ -523,6 +536,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                         const T wr = w * rmc;                 // shared by B_z and dB_z/dz
                         rBh = fma_<T>(wc, K3, rBh);        // B_rho, also the gradient's rho-phi term
                         if (WB) rBz = fma_<T>(wr, K3, rBz);
+                        if constexpr (ADA) rC2 = fma_<T>(wcst[i], K3, rC2);
                         if (WG) {
                             const T K5 = K3 * K2;
                             // r - rho cos = rho s - dm: two moments, combined per row in double (as in the fast
@@ -546,10 +560,12 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             aRZ += W * (double)rRZ;
             aA += W * (double)rA;
             if constexpr (ADA) {   // rows are rings: no length dependence
-                const double rowA = W * (double)rA;
-                const double dAdrho = -(rd * W * (double)rBz - rowA - zd * zd * W * (double)rBh) / rho_d;
+                const double c2 = sizeof(T) == 8 ? (double)rC2 : (double)rBh - (double)rC2;   // sum w cos^2 K^3
+                const double dAdr = -W * (rd * (double)rBh - rho_d * c2);
+                const double dAdrho = -(rd * dAdr - zd * zd * W * (double)rBh) / rho_d;
                 qR += dAdrho;
                 qa += 0.5 * (1.0 + xt[j]) * dAdrho;
+                qAr += dAdr;
             }
             if constexpr (AD) {
                 const double Wr = c.cf * wt[j], hx = 0.5 * (1.0 + xt[j]);
@@ -573,6 +589,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
             ada->dR = qR;
             ada->da = qa;
             ada->db = 0.0;
+            ada->Ar = qAr;
         }
         L.A = aA;
         L.Bh = aBh * zd;

@@ -25,6 +25,12 @@ struct FieldParams {
     uint32_t q;            // Q_* mask
     Tables tab;
     int32_t near_fp64;     // FP32C: skip near pairs here, re-evaluate in FP64
+    // FP32C far field: the recurrence header and each coil's moments (kMpHeader + n_coils * kMpCoil doubles,
+    // multipole.hpp); a coil with CoilDev::far_d2 > 0 takes its series beyond that distance. NULL: no coil does.
+    const double *mp;
+    // The order the points are visited in (a spatial sort, so the points of a wave lie close together and a wave
+    // mostly takes one branch per coil); NULL: index order. No result depends on it.
+    const uint32_t *order;
     // FP32C near-winding pass, stream-ordered with no host readback (so a caller may capture it in a graph):
     // field_kernel counts each point's near coils (near_counts); an exclusive scan of near_counts gives each point's
     // slots (near_offsets); near_count counts each coil's near points (near_hist, indexed by CoilDev::near_key, so
@@ -96,5 +102,12 @@ void launch_near_key_scan(hipStream_t s, const FieldParams &p);
 void launch_near_fill(hipStream_t s, const FieldParams &p);
 void launch_near_eval(hipStream_t s, const FieldParams &p);
 void launch_near_add(hipStream_t s, const FieldParams &p);
+// Far-field moments of every coil with CoilDev::far_d2 > 0 and the recurrence header, into mp.
+void launch_multipole_moments(hipStream_t s, const CoilDev *coils, int n_coils, const Tables &tab, double *mp);
+// The visit order: points counting-sorted by the Morton cell of a 2^bits-per-axis grid over their bounding box.
+// temp: visit_order_bytes(n, bits, scan_bytes) bytes, scan_bytes from near_scan for 2^(3 bits) counts.
+size_t visit_order_bytes(int64_t n, int bits, size_t scan_bytes);
+hipError_t launch_visit_order(hipStream_t s, const double *pts, int64_t n, int bits, void *temp, size_t scan_bytes,
+                              uint32_t *order);
 
 }  // namespace ccoil

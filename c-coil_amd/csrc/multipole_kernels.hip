@@ -1,0 +1,218 @@
+// Far-field moments of the coils and the spatial visit order of the points (gfx950).
+//
+// multipole_moments_kernel: one workgroup per coil sums the zonal-harmonic moments of the coil's rings (the field
+// rule's thickness nodes times the length nodes, multipole.hpp) in FP64, with a fixed reduction order.
+//
+// The visit order: the points' bounding box, a cell of a 2^k x 2^k x 2^k grid for every point (Morton order of the
+// cells), a counting sort of the point indices by cell. field_kernel reads its points in that order, so the 64
+// points of a wave lie in one or two neighbouring cells and a wave mostly takes one branch (series or rule) per
+// coil. Each point's arithmetic does not depend on which wave holds it, so the order within a cell (set by
+// atomics) does not change any result.
+#include "kernels.hpp"
+#include "multipole.hpp"
+
+namespace ccoil {
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void multipole_moments_kernel(const CoilDev *coils, Tables tab, double *mp)
+{
+    __shared__ double part[4][kMpTerms + 1];
+    const int ci = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (ci == 0 && t < 64) {   // recurrence coefficients of the Legendre polynomials
+        mp[t] = (2.0 * t + 5.0) / (t + 3.0);
+        mp[64 + t] = (t + 2.0) / (t + 3.0);
+    }
+    const CoilDev &c = coils[ci];
+    double *out = mp + kMpHeader + (size_t)ci * kMpCoil;
+    if (!(c.far_d2 > 0.0)) {   // the coil never uses its series
+        out[t] = 0.0;
+        return;
+    }
+    const double inv = 1.0 / c.rmax;
+    const int nt = c.n_thk, nz = c.use_fast ? kMpLenNodes : 1;
+    const double *xt = tab.x + table_offset(nt), *wt = tab.w + table_offset(nt);
+    const double *xz = tab.x + table_offset(kMpLenNodes), *wz = tab.w + table_offset(kMpLenNodes);
+    double acc[kMpTerms + 1];
+#pragma unroll
+    for (int l = 0; l <= kMpTerms; ++l) acc[l] = 0.0;
+    for (int k = t; k < nt * nz; k += 256) {
+        const int j = k / nz, i = k - j * nz;
+        // ring (rho, z0) in units of r_max, weight: its share of the current (the weights sum to 1)
+        const double rho = (c.R + 0.5 * c.a * (1.0 + xt[j])) * inv;
+        const double z0 = nz > 1 ? 0.5 * c.b * xz[i] * inv : 0.0;
+        const double w = 0.5 * wt[j] * (nz > 1 ? 0.5 * wz[i] : 1.0) * rho * rho;
+        const double c2 = rho * rho + z0 * z0;
+        double Um = 1.0, U = z0;     // U_(l-1), U_l
+        double Vm = 0.0, V = 1.0;    // V_(l-1), V_l
+        acc[1] += w;
+#pragma unroll
+        for (int l = 1; l < kMpTerms; ++l) {
+            const double Un = ((2.0 * l + 1.0) * z0 * U - l * c2 * Um) / (l + 1.0);
+            const double Vn = c2 * Vm + (2.0 * l + 1.0) * U;
+            Um = U;
+            U = Un;
+            Vm = V;
+            V = Vn;
+            acc[l + 1] += w * V;
+        }
+    }
+#pragma unroll
+    for (int l = 1; l <= kMpTerms; ++l) {
+        const double v = wave_sum(acc[l]);
+        if (lane == 0) part[wave][l] = v;
+    }
+    __syncthreads();
+    // m_l = mu0 N I sum / (2 (l + 1)), mu0 = 4 pi 1e-7
+    const double ni = c.ni * (4.0 * kPi * kMuReduced);
+    if (t < 64) {
+        const int l = t;
+        double m = 0.0;
+        if (l >= 1 && l <= kMpTerms) m = ni * (((part[0][l] + part[1][l]) + part[2][l]) + part[3][l]) / (2.0 * (l + 1));
+        out[l] = l >= 1 ? m / l : 0.0;
+        out[64 + l] = m;
+        out[128 + l] = (l + 1) * m;
+        out[192 + l] = (l + 1.0) * (l + 2.0) * m;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Visit order.
+// ---------------------------------------------------------------------------------------------
+constexpr int kBoxBlocks = 512;
+
+__device__ inline void minmax6(double *v, const double *o)
+{
+    for (int k = 0; k < 3; ++k) {
+        v[k] = fmin(v[k], o[k]);
+        v[3 + k] = fmax(v[3 + k], o[3 + k]);
+    }
+}
+
+// Block-wide min/max of 6 values (3 minima, 3 maxima), result in thread 0.
+__device__ inline void block_minmax6(double *v)
+{
+    __shared__ double sh[4][6];
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        double o[6];
+        for (int k = 0; k < 6; ++k) o[k] = __shfl_xor(v[k], m, 64);
+        minmax6(v, o);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < 6; ++k) sh[wave][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < 4; ++w) minmax6(v, sh[w]);
+}
+
+__global__ __launch_bounds__(256) void box_partial_kernel(const double *pts, int64_t n, double *part)
+{
+    double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const double p[6] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+        minmax6(v, p);
+    }
+    block_minmax6(v);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; ++k) part[6 * blockIdx.x + k] = v[k];
+}
+
+__global__ __launch_bounds__(256) void box_final_kernel(const double *part, int n_part, double *box)
+{
+    double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int b = threadIdx.x; b < n_part; b += 256) minmax6(v, part + 6 * b);
+    block_minmax6(v);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; ++k) box[k] = v[k];
+}
+
+// Spreads the low 10 bits of x to every third bit.
+__device__ inline uint32_t spread3(uint32_t x)
+{
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+// Cell (Morton code of the 2^bits grid over the box) of every point, and the points per cell.
+__global__ __launch_bounds__(256) void cell_count_kernel(const double *pts, int64_t n, const double *box, int bits,
+                                                         uint32_t *cell, unsigned long long *count)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int cells = 1 << bits;
+    uint32_t q[3];
+    for (int k = 0; k < 3; ++k) {
+        const double ext = box[3 + k] - box[k];
+        const double f = ext > 0.0 ? (pts[3 * i + k] - box[k]) / ext * cells : 0.0;   // in [0, cells]
+        q[k] = f >= 1.0 ? (uint32_t)fmin(f, cells - 1.0) : 0u;   // NaN and non-finite boxes: cell 0
+    }
+    const uint32_t key = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+    cell[i] = key;
+    atomicAdd(&count[key], 1ull);
+}
+
+// order[cursor[cell]++] = point (cursor: the exclusive prefix sum of the cell counts).
+__global__ __launch_bounds__(256) void cell_place_kernel(const uint32_t *cell, int64_t n, unsigned long long *cursor,
+                                                         uint32_t *order)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long pos = atomicAdd(&cursor[cell[i]], 1ull);
+    if (pos < (unsigned long long)n) order[pos] = (uint32_t)i;
+}
+
+void launch_multipole_moments(hipStream_t s, const CoilDev *coils, int n_coils, const Tables &tab, double *mp)
+{
+    if (n_coils > 0) hipLaunchKernelGGL(multipole_moments_kernel, dim3(n_coils), dim3(256), 0, s, coils, tab, mp);
+}
+
+size_t visit_order_bytes(int64_t n, int bits, size_t scan_bytes)
+{
+    const size_t cells = (size_t)1 << (3 * bits);
+    return sizeof(double) * (6 * kBoxBlocks + 8) + sizeof(uint32_t) * (size_t)n + 2 * sizeof(int64_t) * cells +
+           scan_bytes + 6 * 256;
+}
+
+hipError_t launch_visit_order(hipStream_t s, const double *pts, int64_t n, int bits, void *temp, size_t scan_bytes,
+                              uint32_t *order)
+{
+    const size_t cells = (size_t)1 << (3 * bits);
+    auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    char *t = (char *)temp;
+    double *part = (double *)t;
+    t += align(sizeof(double) * 6 * kBoxBlocks);
+    double *box = (double *)t;
+    t += align(sizeof(double) * 8);
+    uint32_t *cell = (uint32_t *)t;
+    t += align(sizeof(uint32_t) * (size_t)n);
+    int64_t *count = (int64_t *)t;
+    t += align(sizeof(int64_t) * cells);
+    int64_t *cursor = (int64_t *)t;
+    t += align(sizeof(int64_t) * cells);
+    void *scan_temp = t;
+    const int blocks = (int)((n + 255) / 256 < kBoxBlocks ? (n + 255) / 256 : kBoxBlocks);
+    hipLaunchKernelGGL(box_partial_kernel, dim3(blocks), dim3(256), 0, s, pts, n, part);
+    hipLaunchKernelGGL(box_final_kernel, dim3(1), dim3(256), 0, s, part, blocks, box);
+    launch_fill_zero(s, count, (int64_t)cells);
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(cell_count_kernel, dim3(grid), dim3(256), 0, s, pts, n, box, bits, cell,
+                       (unsigned long long *)count);
+    size_t tb = scan_bytes;
+    hipError_t e = near_scan(s, count, cursor, (int64_t)cells, scan_temp, tb);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(cell_place_kernel, dim3(grid), dim3(256), 0, s, cell, n, (unsigned long long *)cursor, order);
+    return hipGetLastError();
+}
+
+}  // namespace ccoil

@@ -18,15 +18,17 @@ namespace ccoil {
 // in a wave is one ballot and one atomic).
 __global__ __launch_bounds__(256) void near_count_kernel(FieldParams p)
 {
-    const int64_t pi = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (pi >= p.n_pts || p.near_counts[pi] == 0) return;
+    const int64_t vi = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (vi >= p.n_pts) return;
+    const int64_t pi = p.order ? (int64_t)p.order[vi] : vi;   // the field kernel's visit order
+    if (p.near_counts[pi] == 0) return;
     const double px = p.pts[3 * pi], py = p.pts[3 * pi + 1], pz = p.pts[3 * pi + 2];
     const int lane = threadIdx.x & 63;
     for (int ci = 0; ci < p.n_coils; ++ci) {
         const CoilDev &c = p.coils[ci];
         double z, r, cph, sph;
         to_local(c, px, py, pz, z, r, cph, sph);
-        const uint64_t m = __ballot(is_near(c, z, r, p.q & (Q_G | Q_GJ)));   // the field kernel's own test, bitwise
+        const uint64_t m = __ballot(near_pass_pair(c, z, r, p.q & (Q_G | Q_GJ)));   // the field kernel's test, bitwise
         if (m && lane == __builtin_ctzll(m)) atomicAdd(&p.near_hist[c.near_key], (unsigned)__popcll(m));
     }
 }
@@ -62,8 +64,9 @@ __global__ __launch_bounds__(256) void near_key_scan_kernel(FieldParams p)
 __global__ __launch_bounds__(256) void near_fill_kernel(FieldParams p)
 {
     if (p.near_state->overflow) return;
-    const int64_t pi = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (pi >= p.n_pts) return;
+    const int64_t vi = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (vi >= p.n_pts) return;
+    const int64_t pi = p.order ? (int64_t)p.order[vi] : vi;
     const int64_t count = p.near_counts[pi];
     if (count == 0) return;
     const uint64_t total = p.near_state->total;
@@ -77,7 +80,7 @@ __global__ __launch_bounds__(256) void near_fill_kernel(FieldParams p)
         const CoilDev &c = p.coils[ci];
         double z, r, cph, sph;
         to_local(c, px, py, pz, z, r, cph, sph);
-        const bool near = is_near(c, z, r, p.q & (Q_G | Q_GJ));   // the field kernel's own test, bitwise
+        const bool near = near_pass_pair(c, z, r, p.q & (Q_G | Q_GJ));   // the field kernel's test, bitwise
         // the wave's records of this coil take consecutive places from one atomic on the coil's cursor
         const uint64_t m = __ballot(near);
         if (!m) continue;
@@ -144,8 +147,9 @@ __global__ __launch_bounds__(256) void near_eval_kernel(FieldParams p)
 __global__ __launch_bounds__(256) void near_add_kernel(FieldParams p)
 {
     if (p.near_state->overflow) return;
-    const int64_t pi = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (pi >= p.n_pts) return;
+    const int64_t vi = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (vi >= p.n_pts) return;
+    const int64_t pi = p.order ? (int64_t)p.order[vi] : vi;
     const int64_t count = p.near_counts[pi];
     if (count == 0) return;
     const int64_t e0 = p.near_offsets[pi];

@@ -28,7 +28,7 @@ template <int PREC, bool WM, bool WF, bool GR>
 __device__ inline void eval_prim(const CoilDev &pc, const Tables &tab, double z, double r, int row0, int rstep,
                                  Local &L, AdjA *ada)
 {
-    if constexpr (GR) {   // the gradient of M: A, B and A's geometry derivatives; FP64, FP64_IEEE or FP32C far nodes
+    if constexpr (GR) {   // gradients of M: A, B and the pair adjoint extras; FP64, FP64_IEEE or FP32C far nodes
         if constexpr (PREC == 4) eval_stable<double, true, true, false, false, true>(pc, tab, z, r, row0, rstep, L, nullptr, ada);
         else if constexpr (PREC == 1) eval_stable<float, true, true, false, false, true>(pc, tab, z, r, row0, rstep, L, nullptr, ada);
         else eval_stable<f64s, true, true, false, false, true>(pc, tab, z, r, row0, rstep, L, nullptr, ada);
@@ -58,21 +58,21 @@ __device__ inline void cross3(const double *a, const double *b, double *o)
 __device__ inline double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
 // Gradient of M with respect to the secondary's position: M = sum_n w A(x_n) . t_n over the secondary nodes x_n,
-// so dM/dp = sum_n w J_A(x_n)^T t_n with J_A the Jacobian of the primary's A. With A = A_phi(r, z) e_phi and the
-// discrete B = curl A of the same rule, dA_phi/dz = -B_rho and dA_phi/dr = B_z - A_phi / r, so J_A follows from
-// (A_phi, B_rho, B_z) at the node (on the axis, A_phi / r and dA_phi/dr tend to B_z / 2).
-__device__ inline void grad_m_term(const CoilDev &pc, const Local &L, double r, double c, double s,
+// so dM/dp = sum_n w J_A(x_n)^T t_n with J_A the Jacobian of the primary's A. With A = A_phi(r, z) e_phi, J_A follows
+// from A_phi / r, dA_phi/dr (Ar: the rule's own radial derivative, summed node by node) and dA_phi/dz = -B_rho
+// (exact node by node); on the axis A_phi / r and dA_phi/dr are equal.
+__device__ inline void grad_m_term(const CoilDev &pc, const Local &L, double Ar, double r, double c, double s,
                                    const double *tan, double *out, double *tl_out = nullptr, double *gl_out = nullptr)
 {
     double tl[3];
     mat_vec(pc.Tinv, tan, tl);
     double aor, dra;
     if (r / pc.R <= kGpuAxisRatio) {
-        aor = 0.5 * L.Bz;
-        dra = aor;
+        aor = Ar;
+        dra = Ar;
     } else {
         aor = L.A / r;
-        dra = L.Bz - aor;
+        dra = Ar;
     }
     const double dd = dra - aor;
     const double g[3] = {tl[0] * (-c * s * dd) + tl[1] * (c * c * dra + s * s * aor),
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
         // with Omega x = om x x; its R, a, b change A_phi by the rows' homogeneity (AdjA).
         auto accumulate_grad = [&](const Node &n, const Local &L, const AdjA &q) {
             double gd[3], tl[3], gl[3];
-            grad_m_term(pc, L, n.r, n.pcph, n.psph, n.tan, gd, tl, gl);
+            grad_m_term(pc, L, q.Ar, n.r, n.pcph, n.psph, n.tan, gd, tl, gl);
             const double Al[3] = {-L.A * n.psph, L.A * n.pcph, 0.0};
             double Ag[3];
             mat_vec(pc.T, Al, Ag);
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
             gs[12] += n.w * tphi * q.db;
         };
         // Adds the node's weighted terms: M from A, F and T from B, the gradient of M from both.
-        auto accumulate = [&](const Node &n, const Local &L) {
+        auto accumulate = [&](const Node &n, const Local &L, const AdjA &q) {
             if (WM) {
                 const double vl[3] = {-L.A * n.psph, L.A * n.pcph, 0.0};
                 double ag[3];
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
             }
             if (WD) {
                 double gd[3];
-                grad_m_term(pc, L, n.r, n.pcph, n.psph, n.tan, gd);
+                grad_m_term(pc, L, q.Ar, n.r, n.pcph, n.psph, n.tan, gd);
                 d0 += n.w * gd[0];
                 d1 += n.w * gd[1];
                 d2 += n.w * gd[2];
@@ -253,8 +253,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
                                                                                L, nullptr, &q);
                             accumulate_grad(n, L, q);
                         } else {
-                            eval_stable<float, WM || WD, WF || WD, false>(pc, p.tab, n.z, n.r, rg, pd.row_groups, L);
-                            accumulate(n, L);
+                            AdjA q;
+                            eval_stable<float, WM || WD, WF || WD, false, false, WD>(pc, p.tab, n.z, n.r, rg,
+                                                                                     pd.row_groups, L, nullptr, &q);
+                            accumulate(n, L, q);
                         }
                     }
                 }
@@ -272,8 +274,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
                                                                               L, nullptr, &q);
                             accumulate_grad(n, L, q);
                         } else {
-                            eval_stable<f64s, WM || WD, WF || WD, false>(pc, p.tab, n.z, n.r, rg, pd.row_groups, L);
-                            accumulate(n, L);
+                            AdjA q;
+                            eval_stable<f64s, WM || WD, WF || WD, false, false, WD>(pc, p.tab, n.z, n.r, rg,
+                                                                                    pd.row_groups, L, nullptr, &q);
+                            accumulate(n, L, q);
                         }
                         qh = (qh + 64) & 127;
                         qn -= 64;
@@ -291,8 +295,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
                                                                       nullptr, &q);
                     accumulate_grad(n, L, q);
                 } else {
-                    eval_stable<f64s, WM || WD, WF || WD, false>(pc, p.tab, n.z, n.r, rg, pd.row_groups, L);
-                    accumulate(n, L);
+                    AdjA q;
+                    eval_stable<f64s, WM || WD, WF || WD, false, false, WD>(pc, p.tab, n.z, n.r, rg, pd.row_groups,
+                                                                            L, nullptr, &q);
+                    accumulate(n, L, q);
                 }
             }
         } else {
@@ -301,9 +307,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
                 node(k, n);
                 Local L = {0, 0, 0, 0, 0, 0, 0};
                 AdjA q;
-                eval_prim<PREC, WM || WD, WF || WD, GR>(pc, p.tab, n.z, n.r, rg, pd.row_groups, L, &q);
+                eval_prim<PREC, WM || WD, WF || WD, GR || WD>(pc, p.tab, n.z, n.r, rg, pd.row_groups, L, &q);
                 if constexpr (GR) accumulate_grad(n, L, q);
-                else accumulate(n, L);
+                else accumulate(n, L, q);
             }
         }
         if constexpr (GR) {

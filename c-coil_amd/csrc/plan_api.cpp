@@ -113,6 +113,7 @@ ccoil_status ccoil_fields_plan(ccoil_ctx ctx, const ccoil_coils *coils, const do
     const size_t nc = probe.hc.size();
     size_t bytes = (sizeof(CoilDev) + sizeof(uint32_t)) * nc + 4096;
     if (pp.n_chunks > 1) bytes += sizeof(double) * (size_t)pp.partial_width * (size_t)probe.n * (size_t)pp.n_chunks + 256;
+    bytes += probe.L.far_bytes;
     if (probe.L.near) {
         bytes += 2 * sizeof(int64_t) * (size_t)probe.n + 2 * sizeof(uint32_t) * (nc + 2) + sizeof(NearState) +
                  probe.L.scan_bytes + (size_t)pp.near_cap * (sizeof(uint32_t) + sizeof(uint64_t) +

@@ -73,12 +73,23 @@ enum {
     CCOIL_FORCE_REFERENCE_SCALE = 4, /* ccoil_pairs: force/torque scaled by 2*pi*N2/I1 as the reference's
                                        single-pair arrangement path does (CalculateForceArrangements.cxx:125,
                                        SupportFunctions.cxx:238-241); default is the physical 2*pi*N2*I2 */
-    CCOIL_AUTOTUNE = 8         /* ccoil_fields, ccoil_pairs and the plans: take the launch shape (field: lanes per
+    CCOIL_AUTOTUNE = 8,        /* ccoil_fields, ccoil_pairs and the plans: take the launch shape (field: lanes per
                                   point and coil chunks; pairs: primary-node evaluations per lane and work item)
                                   from the context's tuning cache; the first call of a size not in the cache times
                                   every candidate shape on the device (the call runs once per candidate) and caches
                                   the fastest. A cached shape is replayed as is, so results are bitwise
                                   reproducible for a given cache. */
+    CCOIL_DIRECT = 16,         /* FP32C fields: evaluate every (coil, point) pair with the rule. By default a pair
+                                  beyond the coil's far-field distance (at least twice the radius of the sphere that
+                                  holds the winding, more for coils of few angular increments) takes the coil's
+                                  exterior zonal-harmonic series in FP64 instead, with moments summed over the rule's
+                                  own thickness nodes: it departs from the FP64 rule by less than 1e-11 of the coil's
+                                  field there (the angular rule's error bound) and from the float rule it replaces
+                                  by the float rule's own rounding. No counterpart in the reference. */
+    CCOIL_FAR_FIELD = 32       /* CCOIL_FP64 fields: take the far-field series as CCOIL_FP32C does, with the
+                                  threshold at 1e-14 of the coil's field (the FP64 rule's own rounding level). No
+                                  counterpart in the reference; without it CCOIL_FP64 evaluates every pair with
+                                  the rule. */
 };
 
 /* Pair quantity mask for ccoil_pairs. CCOIL_PAIR_DM: gradient of M with respect to the secondary's position,

@@ -40,11 +40,47 @@ def rel_err_vec(x, ref, floor=0.0):
     return out
 
 
-def fp64_pair_count(params, pts, width_floor=1e-3):
+def far_threshold(n_ang, q, tol=1e-11):
+    """Distance in units of r_max beyond which CCOIL_FP32C takes a coil's far-field series (0: never), restated from
+    the bound the library documents: the least y in [2, 100] with 1000 y^2 rho_B^(-2 n_ang) <= tol, where rho_B is
+    the Bernstein-ellipse parameter of x_s = -1 + 2i eta / pi and cosh eta = 1 + (y - 1)^2 q / (2 y),
+    q = r_max / (R + a)."""
+    def bound(y):
+        d = y - 1.0
+        eta = np.arccosh(1.0 + d * d * q / (2.0 * y))
+        xs = complex(-1.0, 2.0 * eta / np.pi)
+        sq = np.sqrt(xs * xs - 1.0)
+        return 1000.0 * y * y * max(abs(xs + sq), abs(xs - sq)) ** (-2.0 * n_ang)
+    if n_ang < 5:
+        return 0.0
+    if bound(2.0) <= tol:
+        return 2.0
+    if bound(100.0) > tol:
+        return 0.0
+    lo, hi = 2.0, 100.0
+    for _ in range(40):
+        mid = 0.5 * (lo + hi)
+        if bound(mid) <= tol:
+            hi = mid
+        else:
+            lo = mid
+    return hi
+
+
+def far_radius(row):
+    """(r_max, far-field distance) of a coil row as CCOIL_FP32C sets them (distance 0: the coil keeps its rule)."""
+    R, a, b = row["R"], row["a"], row["b"]
+    rmax = float(np.hypot(R + a, 0.5 * b if row["use_fast"] else 0.0))
+    y = far_threshold(row["n_ang"], rmax / (R + a))
+    return rmax, y * rmax
+
+
+def fp64_pair_count(params, pts, width_floor=1e-3, rows=None):
     """(coil, point) pairs that CCOIL_FP32C evaluates in FP64 when the gradient is requested: the near-winding shell
-    of near_mask, and the cone and cylinder around each coil's axis, r < 0.1 sqrt(r^2 + z^2 + R^2)."""
+    of near_mask, and the cone and cylinder around each coil's axis, r < 0.1 sqrt(r^2 + z^2 + R^2); with rows (the
+    coils' GPU rows) less the pairs beyond the coil's far-field distance, which take the series instead."""
     n = 0
-    for p in params:
+    for k, p in enumerate(params):
         c = oracle_coil(p)
         T = np.array(c.Tinv).reshape(3, 3)
         l = (pts - np.array(p["pos"])) @ T.T
@@ -53,7 +89,12 @@ def fp64_pair_count(params, pts, width_floor=1e-3):
         dr = np.maximum(np.maximum(c.R - r, r - (c.R + c.a)), 0.0)
         dz = np.maximum(np.abs(z) - 0.5 * c.b, 0.0)
         w = max(c.a, c.b, width_floor * c.R)
-        n += int(np.sum((dr * dr + dz * dz < w * w) | (r * r < 0.01 * (r * r + z * z + c.R * c.R))))
+        sel = (dr * dr + dz * dz < w * w) | (r * r < 0.01 * (r * r + z * z + c.R * c.R))
+        if rows is not None:
+            _, dfar = far_radius(rows[k])
+            if dfar > 0:
+                sel &= ~(r * r + z * z > dfar * dfar)
+        n += int(np.sum(sel))
     return n
 
 

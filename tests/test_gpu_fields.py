@@ -106,9 +106,11 @@ def test_single_rect_fp32c_and_fp32(ctx):
     eg = H.rel_err_vec(out["G"], g.reshape(len(pts), 9))
     print(f"\nFP32C rect: B far max {np.nanmax(eb[~near]):.3g} p99 {np.nanpercentile(eb[~near], 99):.3g}; "
           f"G far max {np.nanmax(eg[~near]):.3g}; all-point B max {np.nanmax(eb):.3g}; near pairs {rep.near_pairs}")
-    assert rep.near_pairs == H.fp64_pair_count(params, pts)
+    assert rep.near_pairs == H.fp64_pair_count(params, pts, rows=rows)
     _, rep_b = _run(ctx, rows, pts, abi.B, abi.FP32C)   # without the gradient only the winding shell goes to FP64
     assert rep_b.near_pairs == int(near.sum())
+    _, rep_d = _run(ctx, rows, pts, abi.B | abi.G, abi.FP32C, abi.DIRECT)   # no far-field series: the axis cone too
+    assert rep_d.near_pairs == H.fp64_pair_count(params, pts)
     assert np.nanmax(eb[~near]) < 1e-6
     assert np.nanmax(eg[~near]) < 1e-6
     assert np.nanmax(eb) < 1e-9 or np.nanmax(eb[near]) < 1e-9
@@ -261,7 +263,7 @@ def test_fp32c_small_call_near_and_far_points(ctx):
         assert 0 < near.sum() < len(pts)
         o64, _ = _run(ctx, rows, pts, abi.B | abi.G, abi.FP64)
         o32, rep = _run(ctx, rows, pts, abi.B | abi.G, abi.FP32C)
-        assert rep.split == 16 and rep.near_pairs == H.fp64_pair_count(params, pts)
+        assert rep.split == 16 and rep.near_pairs == H.fp64_pair_count(params, pts, rows=rows)
         eb = H.rel_err_vec(o32["B"], o64["B"])
         eg = H.rel_err_vec(o32["G"], o64["G"])
         assert np.nanmax(eb[near]) < 1e-13 and np.nanmax(eg[near]) < 1e-12, typ

@@ -25,8 +25,9 @@ pytestmark = pytest.mark.gpu
 TERMS = abi.PAIR_GRAD_TERMS
 
 
-def _pairs(n, seed, incs=None, sep=2.5):
-    """n pairs, each coil of config[1]'s distributions, the secondary at sep times the radii from the primary."""
+def _pairs(n, seed, incs=None, sep=2.5, max_inc=80):
+    """n pairs of the mixed-coil distributions, the secondary at sep times the radii from the primary; increments
+    incs, or drawn from {8, 12, ..., max_inc} per active dimension."""
     rng = np.random.default_rng(seed)
     prim = W.random_coil_params(rng, n)
     sec = W.random_coil_params(rng, n)
@@ -35,7 +36,7 @@ def _pairs(n, seed, incs=None, sep=2.5):
         d = rng.standard_normal(3)
         s["pos"] = np.asarray(p["pos"]) + sep * (p["R"] + p["a"] + s["R"] + s["a"] + 0.5 * (p["b"] + s["b"])) * d / np.linalg.norm(d)
         cp, cs = W.make_coil(p), W.make_coil(s)
-        pick = (lambda: int(incs)) if incs else (lambda: int(4 * rng.integers(2, 21)))
+        pick = (lambda: int(incs)) if incs else (lambda: int(4 * rng.integers(2, max_inc // 4 + 1)))
         pa = (pick(), pick() if p["type"] in (W.FLAT, W.RECTANGULAR) else 1, 1)
         sa = (pick(), pick() if s["type"] in (W.FLAT, W.RECTANGULAR) else 1,
               pick() if s["type"] in (W.THIN, W.RECTANGULAR) else 1)
@@ -68,7 +69,8 @@ def _free(p, name):
 
 
 def test_gradient_matches_oracle_finite_differences(ctx):
-    prim, sec, prow, srow = _pairs(8, seed=61)
+    # increments up to 16: the oracle's rule costs (primary nodes) x (secondary nodes) per M, 64 M per pair here
+    prim, sec, prow, srow = _pairs(8, seed=61, max_inc=16)
     P, S = abi.CoilSet.from_rows(prow), abi.CoilSet.from_rows(srow)
     M, grad, _ = ctx.pairs_grad(P, S, len(prow), abi.FP64)
     names = ("x", "y", "z", "alpha", "beta", "R", "a", "b")
