@@ -4,23 +4,27 @@
 Timed workload (one step = one pass of the hot path over the whole batch, inputs resident in HBM):
   B and the gradient tensor of a 128-coil mixed CoilGroup (FILAMENT/THIN/FLAT/RECTANGULAR, R ~ U[0.02, 0.5] m,
   a, b ~ U[0.005, 0.2] m, N ~ U{1..2000}, I ~ U[-10, 10] A, centres ~ U[-1, 1]^3 m, random orientation) at
-  2^20 points ~ U[-1.5, 1.5]^3 m, PrecisionFactor 5, through one ccoil_fields call with device pointers, in
-  CCOIL_FP64 (the cancellation-free integrands in double with Newton-refined hardware reciprocals: the mode that
-  meets the 1e-6 relative-error bar; it agrees with IEEE double to ~1e-14).
+  2^20 points ~ U[-1.5, 1.5]^3 m, PrecisionFactor 5, through one ccoil_fields call with device pointers, by default
+  in CCOIL_FP64 with CCOIL_FAR_FIELD: the quadrature rule in double (cancellation-free integrands, Newton-refined
+  hardware reciprocals) for the (coil, point) pairs inside each coil's far-field distance, and beyond it the coil's
+  exterior zonal-harmonic series in double, which reproduces the rule to ~1e-14 there (the fastest mode that meets
+  the 1e-6 relative-error bar; --precision picks another).
 value = coil·point evaluations of B + gradient per second (coils x points / seconds per step).
 
 Reported beside it in the same line:
-  side         the same pass in CCOIL_FP32C (FP32 nodes, FP64 row sums, FP64 near-winding pairs), CCOIL_FP64_IEEE
-               (same integrands, IEEE division and sqrt) and CCOIL_FP64_REFERENCE (the reference's integrands in
-               double: the plain-FP64 comparator): speed and roofline fraction;
-  accuracy     FP32C and FP64_IEEE against CCOIL_FP64 at every point (max / p99 / p99.9, points inside the
+  side         the same pass in CCOIL_FP64 (the rule at every pair), CCOIL_FP32C (float rule inside the far-field
+               distance, FP64 series beyond, FP64 near-winding pairs), CCOIL_FP32C + CCOIL_DIRECT (the float rule at
+               every pair), CCOIL_FP64_IEEE and CCOIL_FP64_REFERENCE (the reference's integrands in double: the
+               plain-FP64 comparator): speed and roofline fraction;
+  accuracy     every other mode against CCOIL_FP64 at every point (max / p99 / p99.9, points inside the
                near-winding shell of any coil and the rest), relative to the field and relative to the sum of the
                coils' magnitudes at the point (the superposition's conditioning);
   pairs        mutual inductance + force + torque of 65,536 random coil pairs with per-dimension increments
                ~ U{4, 8, ..., 80} (one ccoil_pairs call) in CCOIL_FP64 and CCOIL_FP32C: pairs/s, node-pairs/s and the
                FP32C error;
-  roofline     field_kernel's algorithmic flops per launch (frozen per-node model, roofline_model.py) over its
-               hipEvent-timed duration on the library's stream, against the MI355X vector peak of its type;
+  roofline     field_kernel's algorithmic flops per launch (frozen per-node model of the rule and per-term model of
+               the series, roofline_model.py, with the kernel's own split of the pairs) over its hipEvent-timed
+               duration on the library's stream, against the MI355X vector peak of each part's type;
   adjoint      the coil-design workload (64 THIN/RECTANGULAR coils near the axis, 4,096 points in a small
                cylinder): forward B (ccoil_fields) and the adjoint of a field loss (ccoil_field_gradient, all 9
                parameters of every coil), FP64 and FP32C, and the adjoint / forward time ratio;
@@ -46,7 +50,9 @@ if PKG not in sys.path:
     sys.path.insert(0, PKG)
 
 METRIC = "coil·point B+∇B evals/s and M-pairs/s, FP32-comp vs FP64 at ≤1e-6 rel err"
-PMC_FILE = os.path.join(ROOT, "profiles", "field_kernel_fp64_group128_points1M_pmc.md")
+# committed rocprofv3 --pmc summaries of field_kernel per mode (HBM bytes per launch)
+PMC_FILES = {"fp64": os.path.join(ROOT, "profiles", "field_kernel_fp64_group128_points1M_pmc.md"),
+             "fp64_far_field": os.path.join(ROOT, "profiles", "field_kernel_fp64_far_field_group128_points1M_pmc.md")}
 
 
 def log(*a):
@@ -69,11 +75,12 @@ def stats(e):
             "n": int(e.size)}
 
 
-def pmc_traffic():
-    """HBM bytes per launch of the field kernel from the committed rocprofv3 --pmc summary, if present."""
-    if not os.path.exists(PMC_FILE):
+def pmc_traffic(mode):
+    """HBM bytes per launch of the field kernel of a mode from its committed rocprofv3 --pmc summary, if present."""
+    path = PMC_FILES.get(mode)
+    if path is None or not os.path.exists(path):
         return None
-    with open(PMC_FILE) as f:
+    with open(path) as f:
         for line in f:
             if line.startswith("traffic_bytes_per_launch"):
                 return float(line.split("=")[1].split()[0])
@@ -131,7 +138,8 @@ def main():
     ap.add_argument("--pairs", type=int, default=65536)
     ap.add_argument("--side-steps", type=int, default=3, help="timed steps of each secondary field mode")
     ap.add_argument("--cpu-points", type=int, default=1 << 19)
-    ap.add_argument("--precision", choices=("fp64", "fp32c"), default="fp64", help="mode of the timed steps")
+    ap.add_argument("--precision", choices=("fp64_far_field", "fp64", "fp32c", "fp32c_direct"),
+                    default="fp64_far_field", help="mode of the timed steps")
     ap.add_argument("--no-side", action="store_true", help="skip the secondary field modes and the accuracy report")
     ap.add_argument("--no-pairs", action="store_true", help="skip the pair leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -193,77 +201,105 @@ def main():
         all_ms = float(np.mean([r.kernel_ms for r in reps]))
         return dt, main_ms, all_ms, reps[-1]
 
-    def roofline(main_ms, peak):
-        achieved = flops / (main_ms * 1e-3) / 1e12
-        return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                "kernel": "field_kernel", "kernel_ms": main_ms, "algorithmic_flops_per_launch": flops,
-                "transcendental": {"achieved_ops": trans / (main_ms * 1e-3), "peak_ops": RM.PEAK_TRANS_OPS,
-                                   "frac": trans / (main_ms * 1e-3) / RM.PEAK_TRANS_OPS},
-                "hbm_algorithmic_bytes_per_launch": n * (24 + 24 + 72)}
+    # field modes: (precision, flags, label); the series modes take each coil's far-field series beyond its
+    # distance (ccoil_far_field_radius) and the rule inside it
+    modes = {"fp64_far_field": (abi.FP64, abi.FAR_FIELD, "CCOIL_FP64 + CCOIL_FAR_FIELD"),
+             "fp64": (abi.FP64, 0, "CCOIL_FP64"),
+             "fp32c": (abi.FP32C, 0, "CCOIL_FP32C (far-field series)"),
+             "fp32c_direct": (abi.FP32C, abi.DIRECT, "CCOIL_FP32C + CCOIL_DIRECT"),
+             "fp64_ieee": (abi.FP64_IEEE, 0, "CCOIL_FP64_IEEE"),
+             "fp64_reference": (abi.FP64_REFERENCE, 0, "CCOIL_FP64_REFERENCE")}
+    works = {}
 
-    fp64_head = args.precision == "fp64"
-    head_prec = abi.FP64 if fp64_head else abi.FP32C
-    head_flags = 0
+    def mode_work(name):
+        """(fp32 flops, fp64 flops, special ops) of one field_kernel launch of the mode."""
+        if name not in works:
+            prec, flags, _ = modes[name]
+            series = (prec == abi.FP32C and not flags & abi.DIRECT) or (prec == abi.FP64 and flags & abi.FAR_FIELD)
+            if series:
+                rmax, dist = abi.far_field_radius(cs, prec)
+                w = RM.series_field_work(rows, pts, dist, rmax, RM.LOG_TOL_FP32C if prec == abi.FP32C
+                                         else RM.LOG_TOL_FP64, with_gradient=True, near_pass=prec == abi.FP32C)
+                r32 = w["rule_flops"] if prec == abi.FP32C else 0.0
+                works[name] = (r32, w["series_flops"] + w["rule_flops"] - r32, w["rule_special"] + w["series_special"],
+                               w)
+            else:
+                f32 = prec in (abi.FP32C, abi.FP32)
+                works[name] = (flops if f32 else 0.0, 0.0 if f32 else flops, trans, None)
+        return works[name]
+
+    def roofline(name, main_ms):
+        f32, f64, sp, w = mode_work(name)
+        t = main_ms * 1e-3
+        at_peak = f32 / (RM.PEAK_FP32_TFLOPS * 1e12) + f64 / (RM.PEAK_FP64_TFLOPS * 1e12)
+        achieved = (f32 + f64) / t / 1e12
+        peak = (f32 + f64) / at_peak / 1e12
+        rl = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+              "kernel": "field_kernel", "kernel_ms": main_ms, "algorithmic_flops_per_launch": f32 + f64,
+              "fp32_flops": f32, "fp64_flops": f64,
+              "transcendental": {"achieved_ops": sp / t, "peak_ops": RM.PEAK_TRANS_OPS,
+                                 "frac": sp / t / RM.PEAK_TRANS_OPS},
+              "hbm_algorithmic_bytes_per_launch": n * (24 + 24 + 72)}
+        if w is not None:
+            rl["pairs"] = {"rule": w["rule_pairs"], "series": w["series_pairs"], "near_pass": w["near_pairs"]}
+        return rl
+
+    head_name = args.precision
+    head_prec, head_flags, head_label = modes[head_name]
     head = new_out()
     dt, main_ms, all_ms, rep = timed(head_prec, args.steps, args.warmup, head, head_flags)
     value = len(rows) * n / dt
-    log(f"{args.precision}: {dt * 1e3:.2f} ms/step, field kernel {main_ms:.2f} ms, all kernels {all_ms:.2f} ms")
-    rl = roofline(main_ms, RM.PEAK_FP64_TFLOPS if fp64_head else RM.PEAK_FP32_TFLOPS)
-    rl["traffic"] = pmc_traffic() if fp64_head else None
+    log(f"{head_name}: {dt * 1e3:.2f} ms/step, field kernel {main_ms:.2f} ms, all kernels {all_ms:.2f} ms")
+    rl = roofline(head_name, main_ms)
+    rl["traffic"] = pmc_traffic(head_name)
     result = {"metric": METRIC, "value": value, "unit": "coil·point evals/s", "n_gpus": 1,
               "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
-              "scaling": "weak", "vs_baseline": None, "dtype": "f64" if fp64_head else "f32", "data": "synthetic",
+              "scaling": "weak", "vs_baseline": None, "dtype": "f32" if head_prec == abi.FP32C else "f64",
+              "data": "synthetic",
               "config": {"workload": f"B+gradient of a {len(rows)}-coil mixed CoilGroup at {n} points",
                          "coils": len(rows), "points": n, "precision_factor": 5.0, "quantities": "B+G",
-                         "precision": "CCOIL_FP64" if fp64_head else "CCOIL_FP32C", "seed": 0},
+                         "precision": head_label, "seed": 0},
               "roofline": rl, "kernel_ms": {"field_kernel": main_ms, "all": all_ms}}
 
     if not args.no_side:
-        side, outs = {}, {}
-        # modes that evaluate every pair with the rule carry the rule's roofline; the far-field series modes (FP32C
-        # by default, FP64 with CCOIL_FAR_FIELD) skip most of the rule's nodes, so the node model does not apply
-        for name, prec, flags, peak in (("fp32c", abi.FP32C, 0, None),
-                                        ("fp32c_direct", abi.FP32C, abi.DIRECT, RM.PEAK_FP32_TFLOPS),
-                                        ("fp64_far_field", abi.FP64, abi.FAR_FIELD, None),
-                                        ("fp64_ieee", abi.FP64_IEEE, 0, RM.PEAK_FP64_TFLOPS),
-                                        ("fp64_reference", abi.FP64_REFERENCE, 0, RM.PEAK_FP64_TFLOPS)):
-            if prec == head_prec and flags == head_flags:
+        side, outs = {}, {head_name: head}
+        for name, (prec, flags, label) in modes.items():
+            if name == head_name:
                 continue
             outs[name] = new_out()
             sdt, sms, sall, srep = timed(prec, args.side_steps, 1, outs[name], flags)
-            side[name] = {"value": len(rows) * n / sdt, "ms_per_step": sdt * 1e3, "kernel_ms": sms, "all_ms": sall,
-                          "roofline_frac": roofline(sms, peak)["frac"] if peak else None,
+            side[name] = {"mode": label, "value": len(rows) * n / sdt, "ms_per_step": sdt * 1e3, "kernel_ms": sms,
+                          "all_ms": sall, "roofline_frac": roofline(name, sms)["frac"],
                           "near_pairs": int(srep.near_pairs), "value_over_headline": (len(rows) * n / sdt) / value}
             log(f"{name}: {sdt * 1e3:.1f} ms/step, field kernel {sms:.2f} ms, all kernels {sall:.2f} ms")
         result["side"] = side
-        if fp64_head:
-            b64, g64 = head["B"].cpu().numpy(), head["G"].cpu().numpy()
-            near = W.near_mask(rows, pts)
-            acc = {"against": "CCOIL_FP64 on the same points", "near_points": int(near.sum())}
-            # the superposition's own conditioning: sum over coils of |B_k| and |G_k| (Frobenius) per point, from one
-            # CCOIL_FP64 launch per coil
-            mb = torch.zeros(n, dtype=torch.float64, device=dev)
-            mg = torch.zeros(n, dtype=torch.float64, device=dev)
-            one = new_out()
-            for r in rows:
-                ctx.fields(abi.CoilSet.from_rows([r]), d_pts, q, abi.FP64, abi.DEVICE_POINTERS, out=one)
-                mb += torch.linalg.vector_norm(one["B"], dim=1)
-                mg += torch.linalg.vector_norm(one["G"], dim=1)
-            mb, mg = mb.cpu().numpy(), mg.cpu().numpy()
-            acc["conditioned"] = ("error over the sum of the coils' magnitudes at the point, sum_k |B_k| "
-                                  "(sum_k |G_k| for the gradient)")
-            for name in ("fp32c", "fp32c_direct", "fp64_far_field", "fp64_ieee"):
-                ob, og = outs[name]["B"].cpu().numpy(), outs[name]["G"].cpu().numpy()
-                eb, okb = rel_err(ob, b64)
-                eg, okg = rel_err(og, g64)
-                cb = np.linalg.norm(ob - b64, axis=1) / mb
-                cg = np.linalg.norm(og - g64, axis=1) / mg
-                acc[name] = {"B": {"far": stats(eb[~near[okb]]), "near": stats(eb[near[okb]]),
-                                   "far_conditioned": stats(cb[~near]), "near_conditioned": stats(cb[near])},
-                             "G": {"far": stats(eg[~near[okg]]), "near": stats(eg[near[okg]]),
-                                   "far_conditioned": stats(cg[~near]), "near_conditioned": stats(cg[near])}}
-            result["accuracy"] = acc
-            log(f"accuracy: {json.dumps(acc)}")
+        b64, g64 = outs["fp64"]["B"].cpu().numpy(), outs["fp64"]["G"].cpu().numpy()
+        near = W.near_mask(rows, pts)
+        acc = {"against": "CCOIL_FP64 (the rule in double) on the same points", "near_points": int(near.sum())}
+        # the superposition's own conditioning: sum over coils of |B_k| and |G_k| (Frobenius) per point, from one
+        # CCOIL_FP64 launch per coil
+        mb = torch.zeros(n, dtype=torch.float64, device=dev)
+        mg = torch.zeros(n, dtype=torch.float64, device=dev)
+        one = new_out()
+        for r in rows:
+            ctx.fields(abi.CoilSet.from_rows([r]), d_pts, q, abi.FP64, abi.DEVICE_POINTERS, out=one)
+            mb += torch.linalg.vector_norm(one["B"], dim=1)
+            mg += torch.linalg.vector_norm(one["G"], dim=1)
+        mb, mg = mb.cpu().numpy(), mg.cpu().numpy()
+        acc["conditioned"] = ("error over the sum of the coils' magnitudes at the point, sum_k |B_k| "
+                              "(sum_k |G_k| for the gradient)")
+        for name in ("fp64_far_field", "fp32c", "fp32c_direct", "fp64_ieee"):
+            ob, og = outs[name]["B"].cpu().numpy(), outs[name]["G"].cpu().numpy()
+            eb, okb = rel_err(ob, b64)
+            eg, okg = rel_err(og, g64)
+            cb = np.linalg.norm(ob - b64, axis=1) / mb
+            cg = np.linalg.norm(og - g64, axis=1) / mg
+            acc[name] = {"B": {"far": stats(eb[~near[okb]]), "near": stats(eb[near[okb]]),
+                               "far_conditioned": stats(cb[~near]), "near_conditioned": stats(cb[near])},
+                         "G": {"far": stats(eg[~near[okg]]), "near": stats(eg[near[okg]]),
+                               "far_conditioned": stats(cg[~near]), "near_conditioned": stats(cg[near])}}
+        result["accuracy"] = acc
+        log(f"accuracy: {json.dumps(acc)}")
         if "fp64_reference" in side:
             result["speedup_vs_plain_fp64"] = value / side["fp64_reference"]["value"]
 

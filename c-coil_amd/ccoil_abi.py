@@ -23,7 +23,7 @@ EXPORTED = ["ccoil_create", "ccoil_destroy", "ccoil_last_error", "ccoil_version"
             "ccoil_synchronize", "ccoil_gauss_legendre", "ccoil_fields", "ccoil_pairs", "ccoil_field_gradient",
             "ccoil_pairs_zaxis", "ccoil_self_inductance", "ccoil_get_stream", "ccoil_fields_noalloc",
             "ccoil_set_tuning_file", "ccoil_fields_plan", "ccoil_pairs_plan", "ccoil_plan_launch",
-            "ccoil_plan_synchronize", "ccoil_plan_destroy", "ccoil_pairs_grad"]
+            "ccoil_plan_synchronize", "ccoil_plan_destroy", "ccoil_pairs_grad", "ccoil_far_field_radius"]
 GRAD_TERMS = ("current", "R", "a", "b", "px", "py", "pz", "alpha", "beta")
 PAIR_GRAD_TERMS = ("sec_x", "sec_y", "sec_z", "sec_alpha", "sec_beta", "sec_R", "sec_a", "sec_b",
                    "prim_x", "prim_y", "prim_z", "prim_alpha", "prim_beta", "prim_R", "prim_a", "prim_b")
@@ -77,6 +77,7 @@ def lib():
                                        ctypes.POINTER(vp)]
         L.ccoil_pairs_grad.argtypes = [vp, ctypes.POINTER(Coils), ctypes.POINTER(Coils), vp, vp, ctypes.c_int64, vp,
                                        vp, ctypes.c_int32, ctypes.c_uint32, vp, vp, ctypes.POINTER(Report)]
+        L.ccoil_far_field_radius.argtypes = [ctypes.POINTER(Coils), ctypes.c_int32, dp, dp]
         L.ccoil_plan_launch.argtypes = [vp, vp]
         L.ccoil_plan_synchronize.argtypes = [vp, ctypes.POINTER(Report)]
         L.ccoil_plan_destroy.argtypes = [vp]
@@ -91,7 +92,7 @@ def lib():
         for f in ("ccoil_create", "ccoil_destroy", "ccoil_set_stream", "ccoil_synchronize", "ccoil_gauss_legendre",
                   "ccoil_fields", "ccoil_fields_noalloc", "ccoil_set_tuning_file", "ccoil_fields_plan",
                   "ccoil_pairs_plan", "ccoil_plan_launch", "ccoil_plan_synchronize", "ccoil_plan_destroy", "ccoil_pairs_grad", "ccoil_pairs", "ccoil_field_gradient", "ccoil_pairs_zaxis",
-                  "ccoil_self_inductance"):
+                  "ccoil_self_inductance", "ccoil_far_field_radius"):
             getattr(L, f).restype = ctypes.c_int32
         _lib = L
     return _lib
@@ -99,6 +100,19 @@ def lib():
 
 def version():
     return lib().ccoil_version().decode()
+
+
+def far_field_radius(coils, precision):
+    """(r_max, far-field distance) per coil for field calls of the given precision (host only; distance 0: the coil
+    keeps its rule)."""
+    n = coils.n
+    rmax, dist = np.zeros(n), np.zeros(n)
+    dp = ctypes.POINTER(ctypes.c_double)
+    st = lib().ccoil_far_field_radius(ctypes.byref(coils.struct), precision, rmax.ctypes.data_as(dp),
+                                      dist.ctypes.data_as(dp))
+    if st != OK:
+        raise CCoilError(st, lib().ccoil_last_error(None).decode())
+    return rmax, dist
 
 
 def _ptr(a):

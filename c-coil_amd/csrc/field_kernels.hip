@@ -137,8 +137,9 @@ void field_kernel(FieldParams p)
             if (__ballot(far)) {
                 // the loop runs to the wave's largest term count (uniform bounds and coefficient loads); each lane
                 // stops adding at its own count, so its value does not depend on the other points of its wave
-                const int own = far ? multipole_terms(r * r + z * z, c.rmax) : 0;
-                const int terms = wave_max_int(own);
+                const int own = far ? multipole_terms(r * r + z * z, c.rmax,
+                                                      PREC == kFP64 ? kMpLogTolFP64 : kMpLogTolFP32C) : 0;
+                const int terms = wave_max_small(own);
                 if (far && lane == 0)
                     eval_multipole<WA, WB, WG>(p.mp + kMpHeader + (size_t)ci * kMpCoil, p.mp, c.rmax, z, r, terms,
                                                own, L);

@@ -394,6 +394,24 @@ ccoil_status fields_call(ccoil_ctx ctx, const ccoil_coils *coils, const double *
 
 }  // namespace ccoil
 
+extern "C" ccoil_status ccoil_far_field_radius(const ccoil_coils *coils, int32_t precision, double *rmax,
+                                               double *distance)
+{
+    std::vector<CoilDev> hc;
+    const std::string msg = pack_coils(coils, false, false, hc);
+    if (!msg.empty()) return fail(nullptr, CCOIL_ERR_ARGUMENT, "ccoil_far_field_radius: " + msg);
+    if (!hc.empty() && (!rmax || !distance))
+        return fail(nullptr, CCOIL_ERR_ARGUMENT, "ccoil_far_field_radius: an output is NULL");
+    const bool far = precision == CCOIL_FP32C || precision == CCOIL_FP64;
+    const double tol = precision == CCOIL_FP64 ? kFarToleranceFP64 : kFarTolerance;
+    for (size_t i = 0; i < hc.size(); ++i) {
+        const CoilDev &c = hc[i];
+        rmax[i] = c.rmax;
+        distance[i] = far ? far_threshold(c.n_ang, c.rmax / (c.R + c.a), tol) * c.rmax : 0.0;
+    }
+    return CCOIL_OK;
+}
+
 extern "C" ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, const double *points, int64_t n_points,
                                      uint32_t quantities, int32_t precision, uint32_t flags, double *A, double *B,
                                      double *E, double *G, ccoil_report *report)

@@ -35,20 +35,25 @@ constexpr int kMpHeader = 2 * 64;   // recurrence coefficients (2l + 5) / (l + 3
 constexpr int kMpLenNodes = 40;     // Gauss-Legendre nodes over the length of a fast-rule coil
 
 // Number of terms for a point at distance D from a coil whose winding lies within r_max: the truncation falls like
-// (r_max / D)^L with a constant below 2, so L = ln(2e16) / ln(D / r_max), between 4 and kMpTerms.
-__device__ inline int multipole_terms(double D2, double rmax)
+// (r_max / D)^L with a constant below 2, so L = ln(2 / tol) / ln(D / r_max), between 4 and kMpTerms; log_tol =
+// ln(2 / tol): kMpLogTolFP64 (tol = 1e-16) or kMpLogTolFP32C (1e-13, far below the float rule's own rounding).
+constexpr float kMpLogTolFP64 = 37.5348f;
+constexpr float kMpLogTolFP32C = 30.6267f;
+__device__ inline int multipole_terms(double D2, double rmax, float log_tol)
 {
     const float lg = __logf((float)(D2 / (rmax * rmax)));   // 2 ln(D / r_max) > 2 ln 2
-    const int L = (int)(2.0f * 37.5348f / lg) + 2;   // 2 ln(2e16) / lg
+    const int L = (int)(2.0f * log_tol / lg) + 2;
     return L < 4 ? 4 : (L > kMpTerms ? kMpTerms : L);
 }
 
-// Largest v over the wave (every lane active).
-__device__ inline int wave_max_int(int v)
+// Largest v over the wave for 0 <= v < 64 (every lane active): its bits from the top, one ballot each.
+__device__ inline int wave_max_small(int v)
 {
+    int m = 0;
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m, 64));
-    return v;
+    for (int b = 32; b >= 1; b >>= 1)
+        if (__ballot(v >= (m | b))) m |= b;
+    return m;
 }
 
 // Local (A_phi, B_rho, B_z, B_rho / r, dB_rho/drho, dB_rho/dz, dB_z/dz) of one coil at local (z, r) from its series

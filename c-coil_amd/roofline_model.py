@@ -20,6 +20,12 @@ reciprocal), FMA counted as 2 flops, per-point work (the frame transform, cos ph
 The device kernels evaluate rearranged, cancellation-free integrands that issue more instructions per node than
 this; the extra instructions (and any compensation) are not algorithmic work and are not counted.
 Pairs: one node-pair of M costs one A evaluation; one node-pair of F/T costs one B evaluation.
+
+The far-field series (a field call's (coil, point) pairs beyond the coil's far-field distance, multipole.hpp) is
+counted per pair from its FP64 loop: per term 13 flops for B (the next power of r_max / D, two sums with their
+coefficient products, the Legendre recurrence of P and P') and 18 with the gradient (two more sums), and per pair
+20 (B) or 30 (B + gradient) flops and 2 special ops (1 / sqrt(D^2), the log of the term count) for the set-up and
+the outputs. The term count per pair is the kernel's: int(2 ln(2 / tol) / ln(D^2 / r_max^2)) + 2 in [4, 60].
 """
 
 FAST_B = (3, 19)
@@ -48,6 +54,53 @@ def field_work(rows, n_points, with_gradient=True):
         flops += nodes * f
         trans += nodes * t
     return float(flops) * n_points, float(trans) * n_points
+
+
+SERIES_B = (2, 20, 13)    # special ops and flops per pair, flops per term
+SERIES_BG = (2, 30, 18)
+LOG_TOL_FP64 = 37.5348    # ln(2 / 1e-16)
+LOG_TOL_FP32C = 30.6267   # ln(2 / 1e-13)
+
+
+def series_field_work(rows, pts, dist, rmax, log_tol, with_gradient=True, near_pass=False):
+    """Work of one field_kernel launch that takes the far-field series beyond each coil's distance dist (0: never)
+    and the rule elsewhere; near_pass: the FP32C near-winding (and, with the gradient, axis-cone) pairs are left to
+    the near pass. Returns dict(rule_flops, rule_special, series_flops, series_special, rule_pairs, series_pairs,
+    near_pairs)."""
+    import numpy as np
+    from workloads import rotation
+    out = dict(rule_flops=0.0, rule_special=0.0, series_flops=0.0, series_special=0.0, rule_pairs=0,
+               series_pairs=0, near_pairs=0)
+    sp, sf, st = SERIES_BG if with_gradient else SERIES_B
+    for r, d, rm in zip(rows, dist, rmax):
+        x = pts - np.asarray(r["pos"])
+        D2 = np.einsum("ij,ij->i", x, x)
+        far = (D2 > d * d) if d > 0 else np.zeros(len(pts), dtype=bool)
+        near = np.zeros(len(pts), dtype=bool)
+        if near_pass:
+            l = x @ rotation(r["alpha"], r["beta"])
+            rr, z = np.hypot(l[:, 0], l[:, 1]), l[:, 2]
+            dr = np.maximum(np.maximum(r["R"] - rr, rr - (r["R"] + r["a"])), 0.0)
+            dz = np.maximum(np.abs(z) - 0.5 * r["b"], 0.0)
+            w = max(r["a"], r["b"], 1e-3 * r["R"])
+            near = dr * dr + dz * dz < w * w
+            if with_gradient:
+                near |= rr * rr < 0.01 * (rr * rr + z * z + r["R"] ** 2)
+            near &= ~far
+        rule = ~far & ~near
+        nodes = int(r["n_ang"]) * int(r["n_thk"])
+        t, f = (FAST_BG if with_gradient else FAST_B) if r["use_fast"] else (SLOW_BG if with_gradient else SLOW_B)
+        nr = int(rule.sum())
+        out["rule_flops"] += float(nr) * nodes * f
+        out["rule_special"] += float(nr) * nodes * t
+        if far.any():
+            L = np.clip((2.0 * log_tol / np.log(D2[far] / (rm * rm))).astype(np.int64) + 2, 4, 60)
+            out["series_flops"] += float(far.sum()) * sf + float(L.sum()) * st
+            out["series_special"] += float(far.sum()) * sp
+        out["rule_pairs"] += nr
+        out["series_pairs"] += int(far.sum())
+        out["near_pairs"] += int(near.sum())
+    return out
 
 
 def pair_work(prim_rows, sec_rows, with_m=True, with_ft=True):

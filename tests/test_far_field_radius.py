@@ -1,0 +1,59 @@
+"""ccoil_far_field_radius (host only) against the restatement of the far-field threshold's bound in helpers.py, over
+angular orders 1-100, thin and thick windings, fast and slow rules, both precisions that use the series; and the
+bound itself pinned on cases fixed by hand."""
+import numpy as np
+import pytest
+
+import ccoil_abi as abi
+import helpers as H
+import workloads as W
+
+
+def _rows(rng, n):
+    rows = []
+    for k in range(n):
+        typ = ("RECTANGULAR", "THIN", "FLAT", "FILAMENT")[k % 4]
+        p = dict(type=typ, R=rng.uniform(0.01, 0.5), a=rng.uniform(0.005, 0.3) if typ in ("RECTANGULAR", "FLAT") else 0.0,
+                 b=rng.uniform(0.005, 0.3) if typ in ("RECTANGULAR", "THIN") else 0.0, N=100, I=1.0,
+                 pos=rng.uniform(-1, 1, 3), alpha=rng.uniform(0, 3), beta=rng.uniform(0, 6))
+        row = W.coil_row(W.make_coil(p))
+        row.update(n_ang=int(1 + k % 100), n_thk=int(1 + k % 7), blocks=(1, 1, 1))
+        rows.append(row)
+    return rows
+
+
+@pytest.mark.parametrize("prec,tol", [(abi.FP32C, 1e-11), (abi.FP64, 1e-14)])
+def test_library_threshold_matches_the_bound(prec, tol):
+    rows = _rows(np.random.default_rng(1), 400)
+    rmax, dist = abi.far_field_radius(abi.CoilSet.from_rows(rows), prec)
+    for row, rm, d in zip(rows, rmax, dist):
+        R, a, b = row["R"], row["a"], row["b"]
+        assert rm == pytest.approx(np.hypot(R + a, 0.5 * b if row["use_fast"] else 0.0), rel=1e-15)
+        y = H.far_threshold(row["n_ang"], rm / (R + a), tol)
+        assert d == pytest.approx(y * rm, rel=1e-12, abs=0.0), (row["n_ang"], y, d / rm)
+
+
+def test_other_precisions_never_use_the_series():
+    rows = _rows(np.random.default_rng(2), 20)
+    for prec in (abi.FP32, abi.FP64_REFERENCE, abi.FP64_IEEE):
+        _, dist = abi.far_field_radius(abi.CoilSet.from_rows(rows), prec)
+        assert np.all(dist == 0.0)
+
+
+def test_bound_pinned_by_hand():
+    # y = 2, q = 1: cosh(eta) = 1 + 1 / 4 = 1.25, eta = ln 2 (cosh ln 2 = (2 + 1/2) / 2); x_s = -1 + 2i ln2 / pi =
+    # -1 + 0.441271i; rho_B = |x_s - sqrt(x_s^2 - 1)| = 1.98580 (by hand: x_s^2 - 1 = -0.194720 - 0.882542i, whose
+    # root of positive real part is 0.595419 - 0.741110i, so x_s - root = -1.595419 + 1.182381i). With n = 40:
+    # 1000 * 4 * rho^-80 = 5.8e-21 <= 1e-14, so y = 2 is the threshold; with n = 12: 4000 * rho^-24 = 2.8e-4 > 1e-11,
+    # so it lies beyond 2.
+    assert H.far_threshold(40, 1.0, 1e-14) == 2.0
+    assert H.far_threshold(12, 1.0, 1e-11) > 2.0
+    xs = complex(-1.0, 2.0 * np.log(2.0) / np.pi)
+    sq = np.sqrt(xs * xs - 1)
+    assert max(abs(xs + sq), abs(xs - sq)) == pytest.approx(1.98580, abs=1e-5)
+    # fewer than 5 angular increments never use the series; a bound still above tol at y = 100 neither
+    assert H.far_threshold(4, 1.0, 1.0) == 0.0
+    assert H.far_threshold(8, 1.0, 1e-14) == 0.0
+    # the threshold grows as the tolerance tightens and falls as the angular order grows
+    assert H.far_threshold(16, 1.0, 1e-14) > H.far_threshold(16, 1.0, 1e-11) >= 2.0
+    assert H.far_threshold(14, 1.0, 1e-11) > H.far_threshold(18, 1.0, 1e-11)
