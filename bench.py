@@ -309,17 +309,22 @@ def main():
         what = abi.PAIR_M | abi.PAIR_FT
         pf, ptrans, pnodes = RM.pair_work(prow, srow)
         pres = {}
-        for name, prec in (("fp64", abi.FP64), ("fp32c", abi.FP32C)):
-            ctx.pairs(P, S, len(prow), what, prec)
+        # the rule at every node (fp64, fp32c_direct: node-pair rates and algorithmic flops are the rule's), and the
+        # primaries' series beyond their far-field distance (fp64_far_field, fp32c: pairs/s)
+        for name, prec, flags in (("fp64", abi.FP64, 0), ("fp64_far_field", abi.FP64, abi.FAR_FIELD),
+                                  ("fp32c", abi.FP32C, 0), ("fp32c_direct", abi.FP32C, abi.DIRECT)):
+            ctx.pairs(P, S, len(prow), what, prec, flags)
             t0 = time.perf_counter()
-            o, prep = ctx.pairs(P, S, len(prow), what, prec)
+            o, prep = ctx.pairs(P, S, len(prow), what, prec, flags)
             dt_p = time.perf_counter() - t0
             pres[name] = {"pairs_per_s": len(prow) / dt_p, "ms_per_step": dt_p * 1e3,
-                          "kernel_ms": prep.main_kernel_ms,
-                          "node_pairs_per_s": pnodes / (prep.main_kernel_ms * 1e-3),
-                          "tflops_algorithmic": pf / (prep.main_kernel_ms * 1e-3) / 1e12, "out": o}
+                          "kernel_ms": prep.main_kernel_ms, "out": o}
+            if name in ("fp64", "fp32c_direct"):
+                pres[name]["node_pairs_per_s"] = pnodes / (prep.main_kernel_ms * 1e-3)
+                pres[name]["tflops_algorithmic"] = pf / (prep.main_kernel_ms * 1e-3) / 1e12
             log(f"pairs {name}: {dt_p * 1e3:.1f} ms/step, kernel {prep.main_kernel_ms:.1f} ms")
         o64, o32 = pres["fp64"].pop("out"), pres["fp32c"].pop("out")
+        ofar, o32d = pres["fp64_far_field"].pop("out"), pres["fp32c_direct"].pop("out")
         em = np.abs(o32["M"] - o64["M"]) / np.abs(o64["M"])
         ef, _ = rel_err(o32["F"], o64["F"])
         et, _ = rel_err(o32["T"], o64["T"])
@@ -330,6 +335,11 @@ def main():
         mscale = np.abs(o64["M"]) + np.linalg.norm(o64["F"], axis=1) / np.abs(cur) * dist
         emc = np.abs(o32["M"] - o64["M"]) / mscale
         pres["accuracy_fp32c_vs_fp64"] = {"M": stats(em), "M_conditioned": stats(emc), "F": stats(ef), "T": stats(et)}
+        for name, o in (("fp64_far_field", ofar), ("fp32c_direct", o32d)):
+            pres[f"accuracy_{name}_vs_fp64"] = {
+                "M": stats(np.abs(o["M"] - o64["M"]) / np.abs(o64["M"])),
+                "M_conditioned": stats(np.abs(o["M"] - o64["M"]) / mscale),
+                "F": stats(rel_err(o["F"], o64["F"])[0]), "T": stats(rel_err(o["T"], o64["T"])[0])}
         pres["workload"] = (f"{len(prow)} random mixed pairs, increments ~ U{{4,...,80}} per active dimension, "
                             f"{pnodes:.3g} node-pairs, M+F+T")
         result["pairs"] = pres

@@ -154,6 +154,7 @@ struct PairBuild {
     int64_t node_pairs = 0;
     int64_t target = 0;        // primary-node evaluations per lane and work item
     int max_row_groups = 1;
+    bool far = false;          // the primaries' far-field series is on (hp[].far_d2 set)
 };
 // Validates a pair call and builds its work list; target_override > 0 replaces the lane-target rule.
 ccoil_status build_pairs(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coils *sec, const int64_t *prim_index,

@@ -102,4 +102,41 @@ __device__ inline void eval_multipole(const double *mp, const double *rec, doubl
     out.RR = -(out.RP + out.ZZ);
 }
 
+// The pair kernel's view of the series at a secondary node: A_phi (WA), B_rho and B_z (WB), and A_phi / r = u sum
+// m_l q_l P'_l / l (also on the axis, where it is the limit), for dA_phi/dr = B_z - A_phi / r. Loop as in
+// eval_multipole: L terms, the lane's own count own.
+template <bool WA, bool WB>
+__device__ inline void eval_multipole_pair(const double *mp, const double *rec, double rmax, double z, double r, int L,
+                                           int own, Local &out, double &a_over_r)
+{
+    const double D2 = r * r + z * z;
+    const double u = 1.0 / sqrt(D2);
+    const double c = z * u, s = r * u, qr = rmax * u;
+    const double *mA = mp, *mB = mp + 64, *mZ = mp + 128;
+    double P1 = 0.5 * (3.0 * c * c - 1.0), P2 = 0.5 * c * (5.0 * c * c - 3.0);
+    double d0 = 1.0, d1 = 3.0 * c, d2 = 1.5 * (5.0 * c * c - 1.0);
+    double q = qr;
+    double SA = 0.0, SZ = 0.0, SB = 0.0;
+    for (int l = 1; l <= L; ++l) {
+        q *= qr;
+        const double ql = l <= own ? q : 0.0;
+        if (WA) SA = fma(mA[l] * ql, d0, SA);
+        if (WB) {
+            SZ = fma(mZ[l] * ql, P1, SZ);
+            SB = fma(mB[l] * ql, d1, SB);
+        }
+        const double P3 = fma(rec[l] * c, P2, -rec[64 + l] * P1);
+        const double d3 = fma((double)(2 * l + 5), P2, d1);
+        P1 = P2;
+        P2 = P3;
+        d0 = d1;
+        d1 = d2;
+        d2 = d3;
+    }
+    out.A = s * SA;
+    out.Bz = u * SZ;
+    out.Bh = s * u * SB;
+    a_over_r = u * SA;
+}
+
 }  // namespace ccoil

@@ -10,6 +10,8 @@
 // thread per (configuration, secondary node) and FP32 atomicAdds per configuration (MInductance.cu:157,
 // Force.cu:172-178).
 #include "integrands.hpp"
+#include "kernels.hpp"
+#include "multipole.hpp"
 #include "pairs.hpp"
 
 namespace ccoil {
@@ -232,6 +234,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
                 d2 += n.w * gd[2];
             }
         };
+        // Secondary nodes beyond the primary's far-field distance take the primary's series (FP64): once per node,
+        // in row group 0 (the other groups of a pair skip them). The loop runs to the wave's largest term count and
+        // each lane stops adding at its own, so a node's value does not depend on its wave. Not in the gradient
+        // instance (the series carries no derivatives with respect to the primary's R, a, b).
+        auto series = [&](const Node &n, bool far) {
+            if constexpr (!GR) {
+                if (!__ballot(far)) return;
+                const int own = far ? multipole_terms(n.z * n.z + n.r * n.r, pc.rmax,
+                                                      PREC == 1 ? kMpLogTolFP32C : kMpLogTolFP64) : 0;
+                const int terms = wave_max_small(own);
+                if (far && rg == 0) {
+                    Local L = {0, 0, 0, 0, 0, 0, 0};
+                    double aor = 0.0;
+                    eval_multipole_pair<WM || WD, WF || WD>(p.mp + kMpHeader + (size_t)pd.prim * kMpCoil, p.mp,
+                                                            pc.rmax, n.z, n.r, terms, own, L, aor);
+                    AdjA q = {0.0, 0.0, 0.0, L.Bz - aor};   // dA_phi/dr = B_z - A_phi / r
+                    accumulate(n, L, q);
+                }
+            }
+        };
         if constexpr (PREC == 1) {
             // FP32C: far nodes in FP32 as they come; nodes in the primary's near-winding shell are queued (ring of
             // 128 in LDS, wave-uniform head and count) and evaluated in FP64 64 at a time, so the FP64 evaluation
@@ -241,11 +263,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
             for (int64_t base = n0; base < n1; base += 64) {
                 const int64_t k = base + lane;
                 Node n;
-                bool near = false;
+                bool near = false, far = false;
                 if (k < n1) {
                     node(k, n);
-                    near = is_near(pc, n.z, n.r, WD || GR);   // the position gradient divides A by r
-                    if (!near) {
+                    far = !GR && p.mp && is_far(pc, n.z, n.r);
+                    near = !far && is_near(pc, n.z, n.r, WD || GR);   // the position gradient divides A by r
+                    if (!near && !far) {
                         Local L = {0, 0, 0, 0, 0, 0, 0};
                         if constexpr (GR) {
                             AdjA q;
@@ -260,6 +283,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
                         }
                     }
                 }
+                series(n, far);
                 const unsigned long long mask = __ballot(near);
                 if (mask) {
                     if (near) queue[(qh + qn + __popcll(mask & ((1ull << lane) - 1ull))) & 127] = k;
@@ -305,11 +329,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
             for (int64_t k = n0 + lane; k < n1; k += 64) {
                 Node n;
                 node(k, n);
-                Local L = {0, 0, 0, 0, 0, 0, 0};
-                AdjA q;
-                eval_prim<PREC, WM || WD, WF || WD, GR || WD>(pc, p.tab, n.z, n.r, rg, pd.row_groups, L, &q);
-                if constexpr (GR) accumulate_grad(n, L, q);
-                else accumulate(n, L, q);
+                const bool far = !GR && p.mp && is_far(pc, n.z, n.r);
+                if (!far) {
+                    Local L = {0, 0, 0, 0, 0, 0, 0};
+                    AdjA q;
+                    eval_prim<PREC, WM || WD, WF || WD, GR || WD>(pc, p.tab, n.z, n.r, rg, pd.row_groups, L, &q);
+                    if constexpr (GR) accumulate_grad(n, L, q);
+                    else accumulate(n, L, q);
+                }
+                series(n, far);
             }
         }
         if constexpr (GR) {
@@ -406,6 +434,7 @@ static void launch_pairs_p(hipStream_t s, const PairParams &p)
 void launch_pairs(hipStream_t s, const PairParams &p)
 {
     if (p.n_items <= 0) return;
+    if (p.mp) launch_multipole_moments(s, p.prim, p.n_prim, p.tab, p.mp);
     if (p.precision == 0) launch_pairs_p<0>(s, p);
     else if (p.precision == 4) launch_pairs_p<4>(s, p);
     else if (p.precision == 2) launch_pairs_p<2>(s, p);

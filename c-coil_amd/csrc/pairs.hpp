@@ -32,6 +32,10 @@ struct PairParams {
     Tables tab;
     double *M, *F, *T, *dM;          // outputs (device), indexed by PairDev::out
     double *grad;                    // what & 8: kPairGradOut doubles per pair
+    // The primaries' far-field series (multipole.hpp; moments computed by launch_pairs before the pair kernel):
+    // a secondary node beyond the primary's far-field distance takes the series instead of the rule. NULL: none.
+    double *mp;
+    int32_t n_prim;
 };
 constexpr int kPairTerms = 10;
 // The gradient of M (what & 8): per item M and 13 derivative sums, dM/d(secondary x, y, z, alpha, beta, R, a, b,

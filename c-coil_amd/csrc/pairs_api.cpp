@@ -1,5 +1,6 @@
 // C ABI: ccoil_pairs (mutual inductance, force and torque of many coil pairs, include/ccoil_hip.h).
 #include "context.hpp"
+#include "multipole.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -38,6 +39,20 @@ void decompose(int64_t target, int n_ang, int n_thk, int &nodes_per_item, int &r
     }
 }
 
+// The cost of the primary's series at one secondary node, in primary-rule nodes (about 30 terms of 13 flops
+// against 20-40 flops per rule node), for the work decomposition of pairs that take the series at every node.
+constexpr int kSeriesNodes = 16;
+
+// Every secondary node lies beyond the primary's far-field distance: the secondary's nodes lie within
+// |(R + a, b / 2)| of its centre.
+bool series_only(const CoilDev &p, const CoilDev &s)
+{
+    if (!(p.far_d2 > 0.0)) return false;
+    const double d[3] = {s.pos[0] - p.pos[0], s.pos[1] - p.pos[1], s.pos[2] - p.pos[2]};
+    const double gap = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) - std::hypot(s.R + s.a, 0.5 * s.b);
+    return gap > 0.0 && gap * gap > p.far_d2;
+}
+
 }  // namespace
 
 namespace ccoil {
@@ -70,6 +85,15 @@ ccoil_status build_pairs(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coi
     if (!prim->current || !prim->turns) return fail(ctx, CCOIL_ERR_ARGUMENT, "primary coils need current and turns");
     if (n_pairs == 0) return CCOIL_OK;
     if (hp.empty() || hs.empty()) return fail(ctx, CCOIL_ERR_ARGUMENT, "pairs given but a coil array is empty");
+    // the primaries' far-field series: FP32C by default, FP64 with CCOIL_FAR_FIELD; never for the gradient of M
+    out.far = !(what & kPairGrad) && ((precision == CCOIL_FP32C && !(flags & CCOIL_DIRECT)) ||
+                                      (precision == CCOIL_FP64 && (flags & CCOIL_FAR_FIELD)));
+    if (out.far)
+        for (auto &c : hp) {
+            const double y = far_threshold(c.n_ang, c.rmax / (c.R + c.a),
+                                           precision == CCOIL_FP64 ? kFarToleranceFP64 : kFarTolerance);
+            c.far_d2 = y > 0.0 ? (y * c.rmax) * (y * c.rmax) : 0.0;
+        }
 
     // Pair table, longest work item first (LPT order of the persistent queue).
     std::vector<PairDev> pd((size_t)n_pairs);
@@ -94,7 +118,8 @@ ccoil_status build_pairs(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coi
         d.sec = (int32_t)si;
         d.out = (int32_t)k;
         d.n_nodes = (int64_t)s.blk_ang * s.n_ang * s.blk_thk * s.n_thk * s.blk_len * s.n_len;
-        decompose(target, p.n_ang, p.n_thk, d.nodes_per_item, d.row_groups);
+        if (series_only(p, s)) decompose(target, kSeriesNodes, 1, d.nodes_per_item, d.row_groups);
+        else decompose(target, p.n_ang, p.n_thk, d.nodes_per_item, d.row_groups);
         max_row_groups = std::max(max_row_groups, d.row_groups);
         d.ring_offset = ring_offset ? ring_offset[k] : 0.0;
         d.half_ring = half_ring ? (half_ring[k] ? 1 : 0) : 0;
@@ -111,7 +136,9 @@ ccoil_status build_pairs(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coi
         const PairDev &d = pd[(size_t)k];
         const CoilDev &p = hp[(size_t)d.prim];
         const int64_t rows = (p.n_thk + d.row_groups - 1) / d.row_groups;
-        cost[(size_t)k] = std::min<int64_t>(d.nodes_per_item, d.n_nodes) * p.n_ang * rows * (p.use_fast ? 2 : 1);
+        cost[(size_t)k] = series_only(p, hs[(size_t)d.sec])
+                              ? std::min<int64_t>(d.nodes_per_item, d.n_nodes) * kSeriesNodes
+                              : std::min<int64_t>(d.nodes_per_item, d.n_nodes) * p.n_ang * rows * (p.use_fast ? 2 : 1);
     }
     std::vector<int32_t> order((size_t)n_pairs);
     std::iota(order.begin(), order.end(), 0);
@@ -145,7 +172,10 @@ ccoil_status pairs_setup(ccoil_ctx ctx, const PairBuild &pb, uint32_t what, int3
     CC_HIP(ctx, hipSetDevice(ctx->device));
     const std::vector<CoilDev> &hp = pb.hp, &hs = pb.hs;
     const std::vector<PairDev> &sorted = pb.sorted;
-    CC_HIP(ctx, ensure(*bufs[0], sizeof(CoilDev) * hp.size()));
+    // primary records, then (series) the moments: kMpHeader + kMpCoil per primary
+    const size_t prim_bytes = (sizeof(CoilDev) * hp.size() + 255) & ~(size_t)255;
+    const size_t mp_bytes = pb.far ? sizeof(double) * ((size_t)kMpHeader + (size_t)kMpCoil * hp.size()) : 0;
+    CC_HIP(ctx, ensure(*bufs[0], prim_bytes + mp_bytes));
     CC_HIP(ctx, ensure(*bufs[1], sizeof(CoilDev) * hs.size()));
     CC_HIP(ctx, ensure(*bufs[2], sizeof(PairDev) * sorted.size()));
     const int terms = (what & kPairGrad) ? kPairGradTerms : kPairTerms;
@@ -156,6 +186,8 @@ ccoil_status pairs_setup(ccoil_ctx ctx, const PairBuild &pb, uint32_t what, int3
                                ctx->stream));
     std::memset(&pp, 0, sizeof pp);
     pp.prim = (const CoilDev *)bufs[0]->p;
+    pp.mp = pb.far && !(what & kPairGrad) ? (double *)((char *)bufs[0]->p + prim_bytes) : nullptr;
+    pp.n_prim = (int32_t)hp.size();
     pp.sec = (const CoilDev *)bufs[1]->p;
     pp.pairs = (const PairDev *)bufs[2]->p;
     pp.n_pairs = (int32_t)sorted.size();
@@ -279,8 +311,10 @@ extern "C" ccoil_status ccoil_pairs_grad(ccoil_ctx ctx, const ccoil_coils *prim,
         return fail(ctx, CCOIL_ERR_ARGUMENT, "ccoil_pairs_grad supports CCOIL_FP64, CCOIL_FP64_IEEE and CCOIL_FP32C");
     if (!grad) return fail(ctx, CCOIL_ERR_ARGUMENT, "grad is NULL");
     PairBuild pb;
+    // the rule at every node: the series carries no derivatives with respect to the primary's R, a and b
+    const uint32_t rule_flags = (flags & ~(uint32_t)CCOIL_FAR_FIELD) | (uint32_t)CCOIL_DIRECT;
     ccoil_status st = build_pairs(ctx, prim, sec, prim_index, sec_index, n_pairs, ring_offset, half_ring,
-                                  CCOIL_PAIR_M, precision, flags, M ? M : grad, nullptr, nullptr, nullptr, 0, pb);
+                                  CCOIL_PAIR_M, precision, rule_flags, M ? M : grad, nullptr, nullptr, nullptr, 0, pb);
     if (st != CCOIL_OK || n_pairs == 0) return st;
     PairParams pp;
     DevBuf *const bufs[4] = {&ctx->pprim, &ctx->psec, &ctx->ppairs, &ctx->ppartial};

@@ -551,6 +551,8 @@ std::vector<std::pair<vec3::Vector3, vec3::Vector3>> Coil::computeAllForceTorque
 
 void Coil::setGpuPrecision(int p) { ccoil::bridge::setPrecision(p); }
 int Coil::getGpuPrecision() { return ccoil::bridge::precision(); }
+void Coil::setGpuFarField(int mode) { ccoil::bridge::setFarField(mode); }
+int Coil::getGpuFarField() { return ccoil::bridge::farField(); }
 
 Coil::operator std::string() const
 {

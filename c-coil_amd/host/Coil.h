@@ -253,6 +253,10 @@ class Coil {
     // The reference's TYPE build option (CMakeLists.txt:18) as a runtime setting; default FP32C.
     static void setGpuPrecision(int precision);
     static int getGpuPrecision();
+    // The far-field series of GPU field calls for this thread (no counterpart in the reference): -1 the precision's
+    // default (on in FP32C, off in FP64), 0 off (every pair by the rule), 1 on.
+    static void setGpuFarField(int mode);
+    static int getGpuFarField();
 
     static bool isZAxisCase(const Coil &primary, const Coil &secondary);
     static std::pair<bool, double> improvedPrecisionCase(const Coil &primary, const Coil &secondary);

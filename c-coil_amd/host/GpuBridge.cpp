@@ -24,6 +24,7 @@ struct CtxHolder {
 thread_local CtxHolder t_ctx;
 thread_local int t_device = 0;
 thread_local int t_precision = CCOIL_FP32C;
+thread_local int t_far_field = -1;
 
 }  // namespace
 
@@ -76,6 +77,22 @@ void setPrecision(int p)
         throw std::invalid_argument("GPU precision must be CCOIL_FP64, CCOIL_FP32C, CCOIL_FP32, "
                                     "CCOIL_FP64_REFERENCE or CCOIL_FP64_IEEE");
     t_precision = p;
+}
+
+int farField() { return t_far_field; }
+
+void setFarField(int mode)
+{
+    if (mode < -1 || mode > 1) throw std::invalid_argument("far-field mode must be -1 (default), 0 (off) or 1 (on)");
+    t_far_field = mode;
+}
+
+uint32_t fieldFlags()
+{
+    if (t_far_field < 0) return 0u;
+    if (t_precision == CCOIL_FP32C) return t_far_field ? 0u : (uint32_t)CCOIL_DIRECT;
+    if (t_precision == CCOIL_FP64) return t_far_field ? (uint32_t)CCOIL_FAR_FIELD : 0u;
+    return 0u;
 }
 
 void CoilSoA::add(const Coil &c, const PrecisionArguments &args)
@@ -139,7 +156,8 @@ std::vector<std::pair<vec3::Vector3, vec3::Vector3>> dipoleForceTorque(const Coi
     std::vector<double> B(3 * n), J(9 * n);
     const ccoil_coils v = coils.view();
     ccoil_ctx ctx = context();
-    check(ccoil_fields(ctx, &v, points.data(), (int64_t)n, CCOIL_B | CCOIL_G_JACOBIAN, precision(), 0, nullptr,
+    check(ccoil_fields(ctx, &v, points.data(), (int64_t)n, CCOIL_B | CCOIL_G_JACOBIAN, precision(), fieldFlags(),
+                       nullptr,
                        B.data(), nullptr, J.data(), nullptr),
           ctx, "ccoil_fields");
     std::vector<std::pair<vec3::Vector3, vec3::Vector3>> out(n);
@@ -161,7 +179,8 @@ void fieldsInto(const CoilSoA &coils, const double *points, size_t n, double *A,
     if (!q) throw std::invalid_argument("no output requested: pass at least one of A, B, E, G");
     const ccoil_coils v = coils.view();
     ccoil_ctx ctx = context();
-    check(ccoil_fields(ctx, &v, points, (int64_t)n, q, precision(), devicePointers ? CCOIL_DEVICE_POINTERS : 0u, A, B,
+    check(ccoil_fields(ctx, &v, points, (int64_t)n, q, precision(),
+                       fieldFlags() | (devicePointers ? (uint32_t)CCOIL_DEVICE_POINTERS : 0u), A, B,
                        E, G, nullptr),
           ctx, "ccoil_fields");
 }
@@ -216,7 +235,8 @@ double zAxisSlow(const Coil &prim, const Coil &sec, const CoilPairArguments &arg
     const ccoil_coils v = p.view();
     std::vector<double> out(3 * pts.size());
     ccoil_ctx ctx = context();
-    check(ccoil_fields(ctx, &v, pts.data(), (int64_t)pts.size(), q, precision(), 0, q == CCOIL_A ? out.data() : nullptr,
+    check(ccoil_fields(ctx, &v, pts.data(), (int64_t)pts.size(), q, precision(), fieldFlags(),
+                       q == CCOIL_A ? out.data() : nullptr,
                        q == CCOIL_B ? out.data() : nullptr, nullptr, nullptr, nullptr),
           ctx, "ccoil_fields");
     double sum = 0.0;

@@ -26,6 +26,11 @@ void *stream();
 
 int precision();
 void setPrecision(int precision);
+// The far-field series for this thread's field calls: -1 the precision's default (on in CCOIL_FP32C, off in
+// CCOIL_FP64), 0 off, 1 on; fieldFlags() is the matching ccoil_fields flag for the current precision.
+int farField();
+void setFarField(int mode);
+uint32_t fieldFlags();
 
 struct CoilSoA {
     std::vector<int32_t> use_fast, n_ang, n_thk, n_len, blocks;

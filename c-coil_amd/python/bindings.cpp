@@ -533,6 +533,8 @@ static void init_coil(py::module_ &m)
                     py::arg("precision_factor") = PrecisionFactor(), py::arg("compute_method") = CPU_ST)
         .def_static("set_gpu_precision", &Coil::setGpuPrecision, py::arg("precision"))
         .def_static("get_gpu_precision", &Coil::getGpuPrecision)
+        .def_static("set_gpu_far_field", &Coil::setGpuFarField, py::arg("mode"))
+        .def_static("get_gpu_far_field", &Coil::getGpuFarField)
         .def("__repr__", &Coil::operator std::string);
     zc::add_methods<decltype(coil), Coil>(coil, "Coil");
 

@@ -134,3 +134,74 @@ def test_fp64_series_matches_the_rule(ctx, typ, R, a, b):
             assert err <= 1e-12, (typ, k, part, err)
     inner = slice(0, 256)   # y = 1.5: inside twice r_max, the rule itself
     assert np.array_equal(got["B"][inner], ref["B"][inner])
+
+
+def test_module_far_field_switch():
+    """c_coil.coil.Coil.set_gpu_far_field: per thread, -1 the precision's default, 0 every pair by the rule, 1 the
+    series beyond each coil's distance. FP64 on: within 1e-12 of the rule at far points; FP32C off: bitwise the
+    CCOIL_DIRECT call of the C ABI."""
+    import c_coil
+    C, V = c_coil.coil, c_coil.tensor.Vector3
+    coil = C.Coil(0.1, 0.05, 0.08, 100, 2.0, 60.0, C.PrecisionFactor(5.0), 8, V(0.1, -0.05, 0.02), 0.4, 1.1)
+    rng = np.random.default_rng(91)
+    d = rng.standard_normal((500, 3))
+    pts = np.array([0.1, -0.05, 0.02]) + 1.5 * d / np.linalg.norm(d, axis=1, keepdims=True)   # about 9 r_max away
+    try:
+        C.Coil.set_gpu_precision(C.GPU_FP64)
+        assert C.Coil.get_gpu_far_field() == -1
+        rule = coil.compute_all_B_field_vectors(pts, C.GPU)
+        C.Coil.set_gpu_far_field(1)
+        series = coil.compute_all_B_field_vectors(pts, C.GPU)
+        err = np.linalg.norm(series - rule, axis=1) / np.linalg.norm(rule, axis=1)
+        assert err.max() <= 1e-12 and not np.array_equal(series, rule)
+        C.Coil.set_gpu_precision(C.GPU_FP32C)
+        C.Coil.set_gpu_far_field(0)
+        direct = coil.compute_all_B_field_vectors(pts, C.GPU)
+        C.Coil.set_gpu_far_field(-1)
+        default = coil.compute_all_B_field_vectors(pts, C.GPU)
+        e32 = np.linalg.norm(default - rule, axis=1) / np.linalg.norm(rule, axis=1)
+        assert e32.max() <= 1e-12 and not np.array_equal(direct, default)
+        with pytest.raises(ValueError):
+            C.Coil.set_gpu_far_field(2)
+    finally:
+        C.Coil.set_gpu_far_field(-1)
+        C.Coil.set_gpu_precision(C.GPU_FP32C)
+
+
+def test_pairs_series_against_the_rule(ctx):
+    """Pairs: secondary nodes beyond the primary's far-field distance take the primary's series (CCOIL_FP64 with
+    CCOIL_FAR_FIELD, CCOIL_FP32C by default). On 4,096 random pairs of the mixed distributions, M, F, T and dM of the
+    FP64 series against the FP64 rule: M within 1e-10 of |M| + |F/(I1 I2)| d (d the centre distance: M's size over
+    the pose's neighbourhood), F, T, dM within 1e-10 of their norms; FP32C with the series against FP64 no worse than
+    without it; permuting the batch changes nothing, bitwise."""
+    prow, srow = W.pair_workload(4096, seed=3)
+    P, S = abi.CoilSet.from_rows(prow), abi.CoilSet.from_rows(srow)
+    n = len(prow)
+    what = abi.PAIR_M | abi.PAIR_FT | abi.PAIR_DM
+    ref, r64 = ctx.pairs(P, S, n, what, abi.FP64)
+    ser, s64 = ctx.pairs(P, S, n, what, abi.FP64, abi.FAR_FIELD)
+    cur = np.array([p["current"] * s["current"] for p, s in zip(prow, srow)])
+    dist = np.linalg.norm(np.array([s["pos"] for s in srow]) - np.array([p["pos"] for p in prow]), axis=1)
+    mscale = np.abs(ref["M"]) + np.linalg.norm(ref["F"], axis=1) / np.abs(cur) * dist
+    em = np.abs(ser["M"] - ref["M"]) / mscale
+    ef = np.linalg.norm(ser["F"] - ref["F"], axis=1) / np.linalg.norm(ref["F"], axis=1)
+    et = np.linalg.norm(ser["T"] - ref["T"], axis=1) / np.linalg.norm(ref["T"], axis=1)
+    ed = np.linalg.norm(ser["dM"] - ref["dM"], axis=1) / np.linalg.norm(ref["dM"], axis=1)
+    changed = np.mean(ser["M"] != ref["M"])
+    print(f"\npairs, FP64 series vs rule ({changed:.0%} of the pairs changed): M {em.max():.3g}, F {ef.max():.3g}, "
+          f"T {et.max():.3g}, dM {ed.max():.3g}; kernel {s64.main_kernel_ms:.1f} ms against {r64.main_kernel_ms:.1f}")
+    assert changed > 0.1
+    assert em.max() <= 1e-10 and ef.max() <= 1e-10 and et.max() <= 1e-10 and ed.max() <= 1e-10
+    f32, _ = ctx.pairs(P, S, n, what, abi.FP32C)
+    d32, _ = ctx.pairs(P, S, n, what, abi.FP32C, abi.DIRECT)
+    e32 = np.abs(f32["M"] - ref["M"]) / mscale
+    ed32 = np.abs(d32["M"] - ref["M"]) / mscale
+    f32f = np.linalg.norm(f32["F"] - ref["F"], axis=1) / np.linalg.norm(ref["F"], axis=1)
+    print(f"FP32C M vs FP64 (relative to M's size): with the series max {e32.max():.3g} p99 "
+          f"{np.percentile(e32, 99):.3g}; the rule only max {ed32.max():.3g} p99 {np.percentile(ed32, 99):.3g}; "
+          f"F max {f32f.max():.3g}")
+    assert e32.max() <= ed32.max() and np.percentile(e32, 99) <= np.percentile(ed32, 99)
+    perm = np.random.default_rng(4).permutation(n)
+    pp, _ = ctx.pairs(P, S, n, what, abi.FP64, abi.FAR_FIELD, prim_index=perm, sec_index=perm)
+    for k in ("M", "F", "T", "dM"):
+        assert np.array_equal(pp[k], ser[k][perm]), k
