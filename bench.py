@@ -298,6 +298,26 @@ def main():
                                "far_conditioned": stats(cb[~near]), "near_conditioned": stats(cb[near])},
                          "G": {"far": stats(eg[~near[okg]]), "near": stats(eg[near[okg]]),
                                "far_conditioned": stats(cg[~near]), "near_conditioned": stats(cg[near])}}
+        # A and E (E = 2 pi f A per coil) on the first 2^18 points, every other mode against CCOIL_FP64
+        ns = min(n, 1 << 18)
+        qae = abi.A | abi.E
+        ae = {}
+        for name, (prec, flags, label) in modes.items():
+            if name == "fp64_reference":
+                continue
+            o = {"A": torch.empty((ns, 3), dtype=torch.float64, device=dev),
+                 "E": torch.empty((ns, 3), dtype=torch.float64, device=dev)}
+            ctx.fields(cs, d_pts[:ns], qae, prec, abi.DEVICE_POINTERS | flags, out=o)
+            ae[name] = {k: o[k].cpu().numpy() for k in ("A", "E")}
+        nm = near[:ns]
+        acc["A_E"] = {"points": ns}
+        for name in ae:
+            if name == "fp64":
+                continue
+            acc["A_E"][name] = {}
+            for k in ("A", "E"):
+                e, ok = rel_err(ae[name][k], ae["fp64"][k])
+                acc["A_E"][name][k] = {"far": stats(e[~nm[ok]]), "near": stats(e[nm[ok]])}
         result["accuracy"] = acc
         log(f"accuracy: {json.dumps(acc)}")
         if "fp64_reference" in side:
@@ -353,6 +373,7 @@ def main():
         agrad = torch.empty((len(arows), len(abi.GRAD_TERMS)), dtype=torch.float64, device=dev)
         gB = torch.from_numpy(np.random.default_rng(1).standard_normal(apts.shape)).to(dev)
         adj = {"workload": "64 THIN/RECTANGULAR coils x 4,096 points, B and the 9-parameter adjoint of a field loss"}
+        grads = {}
         for name, prec in (("fp64", abi.FP64), ("fp32c", abi.FP32C)):
             fw, bw = [], []
             for it in range(4):
@@ -361,8 +382,14 @@ def main():
                 if it:
                     fw.append(r1.kernel_ms)
                     bw.append(r2.kernel_ms)
+            grads[name] = agrad.cpu().numpy().copy()
             adj[name] = {"forward_ms": float(np.mean(fw)), "adjoint_ms": float(np.mean(bw)),
                          "adjoint_over_forward": float(np.mean(bw) / np.mean(fw))}
+        # FP32C adjoint against FP64, per parameter kind, relative to that parameter's largest |gradient| over coils
+        g64, g32 = grads["fp64"], grads["fp32c"]
+        scale = np.max(np.abs(g64), axis=0)
+        adj["accuracy_fp32c_vs_fp64"] = {t: stats(np.abs(g32[:, k] - g64[:, k]) / scale[k])
+                                         for k, t in enumerate(abi.GRAD_TERMS)}
         result["adjoint"] = adj
         log(f"adjoint: {json.dumps(adj)}")
 

@@ -75,8 +75,10 @@ def test_adjoint_matches_finite_differences(ctx, typ, R, a, b):
         assert abs(got - v) <= 1e-6 * max(abs(v), 1e-3 * scale[k]), (k, got, v)
 
 
-def test_adjoint_fp32c_against_fp64(ctx):
-    params, pts = W.adjoint_workload(16, 1024, seed=3)
+@pytest.mark.parametrize("n_coils,n_points,seed", [(16, 1024, 3), (64, 4096, 0)])
+def test_adjoint_fp32c_against_fp64(ctx, n_coils, n_points, seed):
+    """On a small set and on the design workload itself (64 coils, 4,096 points, the bench's adjoint leg)."""
+    params, pts = W.adjoint_workload(n_coils, n_points, seed=seed)
     rows = W.group_rows(params)
     rng = np.random.default_rng(4)
     g = rng.standard_normal(pts.shape)

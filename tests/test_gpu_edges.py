@@ -4,7 +4,9 @@
     end planes z = +-b/2 and mirrored in z (the fast rule is evaluated at |z| with odd terms flipped);
   - points far away (10^3 R), where the angular sums extract a small first harmonic;
   - empty inputs: no points, no pairs, no coils (a CoilGroup without members has zero field).
-Tolerances as in test_gpu_fields.py: CCOIL_FP64 A, B <= 1e-11; gradient <= 1e-7.
+Tolerances as in test_gpu_fields.py: CCOIL_FP64 A, B <= 1e-11; gradient <= 1e-7. CCOIL_FP32C on the same points
+(its FP64 pass and series where they apply, the float rule elsewhere): 5e-6, and 1e-11 / 1e-9 on the axis and far
+points.
 """
 import numpy as np
 import pytest
@@ -54,6 +56,29 @@ def test_axis_end_planes_mirror_and_far_points(ctx, prec):
         assert np.max(ea) < 1e-11, (p["type"], ea)
         assert np.nanmax(H.rel_err_vec(out["B"], b)) < 1e-11, p["type"]
         assert np.nanmax(H.rel_err_vec(out["G"], g.reshape(-1, 9))) < 1e-7, p["type"]
+
+
+def test_fp32c_axis_end_planes_mirror_and_far_points(ctx):
+    """CCOIL_FP32C on the same points: the axis points go to the FP64 pass (the gradient's axis cone) or take the
+    FP64 series, the points at 10^3 R take the series, the end-plane points beside the winding are in its near shell;
+    the rest are float-rule points. A, B, gradient within 5e-6 everywhere; the axis points and the far points within
+    1e-11 (A, B) and 1e-9 (gradient)."""
+    rng = np.random.default_rng(91)
+    for p in _coils():
+        rows, args = H.gpu_args_rows([p])
+        pts = _local_points(p, rng)
+        out, rep = ctx.fields(abi.CoilSet.from_rows(rows), pts, abi.A | abi.B | abi.G, abi.FP32C)
+        a, b, g = O.coil_fields(H.oracle_coil(p), args[0], pts, True, True, True, axis_ratio=O.AXIS_RATIO_GPU,
+                                extended=True)
+        scale_a = np.max(np.linalg.norm(a, axis=1))
+        ea = np.linalg.norm(out["A"] - a, axis=1) / np.maximum(np.linalg.norm(a, axis=1), 1e-4 * scale_a)
+        eb = H.rel_err_vec(out["B"], b)
+        eg = H.rel_err_vec(out["G"], g.reshape(-1, 9))
+        exact = np.r_[np.arange(7), np.arange(len(pts) - 8, len(pts))]   # axis points, points at 10^3 R
+        print(f"\nFP32C {p['type']}: A {ea.max():.3g} B {np.nanmax(eb):.3g} G {np.nanmax(eg):.3g}; axis and far "
+              f"points B {np.nanmax(eb[exact]):.3g} G {np.nanmax(eg[exact]):.3g}; near pairs {rep.near_pairs}")
+        assert ea.max() < 5e-6 and np.nanmax(eb) < 5e-6 and np.nanmax(eg) < 5e-6, p["type"]
+        assert ea[exact].max() < 1e-11 and np.nanmax(eb[exact]) < 1e-11 and np.nanmax(eg[exact]) < 1e-9, p["type"]
 
 
 def test_empty_inputs(ctx):
