@@ -28,6 +28,7 @@ struct Arena {
     char *base = nullptr;
     size_t cap = 0;
     size_t used = 0;
+    bool fixed = false;   // a fixed-size scratch (the no-allocation entry point): no rule-record pass
     void *take(size_t bytes)
     {
         const size_t off = (used + 255) & ~(size_t)255;
@@ -105,6 +106,12 @@ struct FieldLaunch {
 constexpr double kFarTolerance = 1e-11;
 constexpr double kFarToleranceFP64 = 1e-14;
 constexpr int64_t kOrderMinPoints = 16384;
+// Rule records: calls of the series modes with kOrderMinPoints to kRecordMaxPoints points evaluate the rule's pairs
+// in the record pass (waves of one coil) instead of beside the series; larger calls are split into pieces of
+// kRecordMaxPoints points by ccoil_fields.
+constexpr int64_t kRecordMaxPoints = (int64_t)1 << 20;
+// Whether a field call of this precision, flags and size has a record pass, and whether it has rule records.
+bool field_record_pass(int32_t precision, uint32_t flags, int64_t n_points, bool fixed_arena, bool *rule_records);
 double far_threshold(int n_ang, double q, double tol);
 // Enqueues the field kernel, the chunk reduction and (FP32C) the whole near pass on s, with no host
 // synchronisation and kernel launches and copies only (so it can be captured in a graph); evm0/evm1 (may be NULL)

@@ -125,15 +125,15 @@ void field_kernel(FieldParams p)
             eval_plain<float, WA, WB, WG>(c, p.tab, z, r, lane, SPLIT, L);
         } else {
             // FP64 and FP32C: beyond the coil's far-field distance (set only when the call uses the series) the FP64
-            // series; FP32C: inside the near shell (or axis cone) nothing here (the near pass adds those pairs in
-            // FP64); elsewhere the rule
+            // series; pairs that become records (FP32C's near shell and axis cone; with rule records every other
+            // pair) nothing here: the record pass adds them; elsewhere the rule
             const bool far = is_far(c, z, r);
-            const bool skip = PREC == kFP32C && !far && p.near_fp64 && is_near(c, z, r, WG);
-            if (!far && !skip) {
+            const int kind = record_kind(p, c, z, r, WG);
+            if (!far && kind < 0) {
                 if constexpr (PREC == kFP64) eval_stable<f64s, WA, WB, WG>(c, p.tab, z, r, lane, SPLIT, L);
                 else eval_stable<float, WA, WB, WG>(c, p.tab, z, r, lane, SPLIT, L);
             }
-            if (skip) ++n_near;
+            if (kind >= 0) ++n_near;
             if (__ballot(far)) {
                 // the loop runs to the wave's largest term count (uniform bounds and coefficient loads); each lane
                 // stops adding at its own count, so its value does not depend on the other points of its wave
@@ -149,7 +149,7 @@ void field_kernel(FieldParams p)
         accumulate_global(c, L, r, cph, sph, p.q, acc);
     }
     if (!active || lane != 0) return;
-    if (PREC == kFP32C && n_near > 0)   // integer sums: exact in any order
+    if ((PREC == kFP32C || PREC == kFP64) && n_near > 0)   // integer sums: exact in any order
         atomicAdd((unsigned long long *)&p.near_counts[pi], (unsigned long long)n_near);
     if (p.n_chunks > 1) {
         double *o = p.partial + ((int64_t)blockIdx.y * p.n_pts + pi) * p.partial_width;

@@ -81,6 +81,15 @@ double far_threshold(int n_ang, double q, double tol)
     return hi;
 }
 
+bool field_record_pass(int32_t precision, uint32_t flags, int64_t n_points, bool fixed_arena, bool *rule_records)
+{
+    const bool far = (precision == CCOIL_FP32C && !(flags & CCOIL_DIRECT)) ||
+                     (precision == CCOIL_FP64 && (flags & CCOIL_FAR_FIELD));
+    const bool rr = far && !fixed_arena && n_points >= kOrderMinPoints && n_points <= kRecordMaxPoints;
+    if (rule_records) *rule_records = rr;
+    return rr || (precision == CCOIL_FP32C && !(flags & CCOIL_NO_NEAR_FP64));
+}
+
 // Sizes the near pass's buffers of L for cap records and points L.p at them: the context's grow-only buffers, or
 // parts of arena (then cap is lowered to what the arena still holds).
 ccoil_status near_buffers(ccoil_ctx ctx, FieldLaunch &L, int64_t cap, int width, Arena *arena)
@@ -93,7 +102,7 @@ ccoil_status near_buffers(ccoil_ctx ctx, FieldLaunch &L, int64_t cap, int width,
     ccoil_status st;
     if ((st = call_buffer(ctx, ctx->ncounts, arena, sizeof(int64_t) * (size_t)n, &counts)) != CCOIL_OK) return st;
     if ((st = call_buffer(ctx, ctx->noffsets, arena, sizeof(int64_t) * (size_t)n, &offsets)) != CCOIL_OK) return st;
-    if ((st = call_buffer(ctx, ctx->nhist, arena, 2 * sizeof(uint32_t) * ((size_t)p.n_coils + 2), &hist)) != CCOIL_OK)
+    if ((st = call_buffer(ctx, ctx->nhist, arena, 2 * sizeof(uint32_t) * ((size_t)p.n_keys + 2), &hist)) != CCOIL_OK)
         return st;
     if ((st = call_buffer(ctx, ctx->nstate, arena, sizeof(NearState), &state)) != CCOIL_OK) return st;
     if ((st = call_buffer(ctx, ctx->ntemp, arena, tb > 0 ? tb : 1, &temp)) != CCOIL_OK) return st;
@@ -105,11 +114,10 @@ ccoil_status near_buffers(ccoil_ctx ctx, FieldLaunch &L, int64_t cap, int width,
     if ((st = call_buffer(ctx, ctx->nvals, arena, sizeof(double) * (size_t)width * (size_t)cap, &vals)) != CCOIL_OK)
         return st;
     if (!arena) ctx->near_cap = std::max(ctx->near_cap, cap);
-    p.near_fp64 = 1;
     p.near_counts = (int64_t *)counts;
     p.near_offsets = (const int64_t *)offsets;
     p.near_hist = (uint32_t *)hist;
-    p.near_cursor = p.near_hist + ((size_t)p.n_coils + 2);
+    p.near_cursor = p.near_hist + ((size_t)p.n_keys + 2);
     p.near_state = (NearState *)state;
     p.near_cap = cap;
     p.near_keys = (uint32_t *)keys;
@@ -148,7 +156,7 @@ void enqueue_fields(hipStream_t s, const FieldLaunch &L, hipEvent_t evm0, hipEve
                                  const_cast<uint32_t *>(p.order));
     if (L.near) {
         launch_fill_zero(s, p.near_counts, p.n_pts);
-        launch_fill_zero(s, p.near_hist, ((int64_t)p.n_coils + 2) / 2);
+        launch_fill_zero(s, p.near_hist, ((int64_t)p.n_keys + 2) / 2);
         launch_fill_zero(s, p.near_state, sizeof(NearState) / 8);
     }
     if (evm0) (void)hipEventRecord(evm0, s);
@@ -198,7 +206,11 @@ ccoil_status fields_prepare(ccoil_ctx ctx, const ccoil_coils *coils, const doubl
     p.n_pts = n;
     p.q = quantities;
     L.precision = precision;
-    L.near = precision == CCOIL_FP32C && !(flags & CCOIL_NO_NEAR_FP64);
+    bool rule_records = false;
+    L.near = field_record_pass(precision, flags, n_points, arena && arena->fixed, &rule_records);
+    p.near_fp64 = precision == CCOIL_FP32C && !(flags & CCOIL_NO_NEAR_FP64);
+    p.rule_records = rule_records;
+    p.float_records = rule_records && precision == CCOIL_FP32C;
     const bool far = (precision == CCOIL_FP32C && !(flags & CCOIL_DIRECT)) ||
                      (precision == CCOIL_FP64 && (flags & CCOIL_FAR_FIELD));
     const double far_tol = precision == CCOIL_FP64 ? kFarToleranceFP64 : kFarTolerance;
@@ -237,6 +249,7 @@ ccoil_status fields_prepare(ccoil_ctx ctx, const ccoil_coils *coils, const doubl
                                hipMemcpyHostToDevice, ctx->stream));
     p.coils = (const CoilDev *)coil_mem;
     p.n_coils = (int32_t)hc.size();
+    p.n_keys = p.n_coils * (p.float_records ? 2 : 1);
     p.near_coil = (const uint32_t *)((const char *)coil_mem + coil_bytes);
     p.tab = ctx_tables(ctx);
     if (!dev) {
@@ -353,18 +366,19 @@ ccoil_status fields_call(ccoil_ctx ctx, const ccoil_coils *coils, const double *
     st = copy_out();
     if (st != CCOIL_OK) return st;
     CC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    int64_t near = 0;
+    int64_t near = 0, total = 0;
     if (L.near) {
         st = near_outcome(ctx, *ctx->near_host);
         if (st != CCOIL_OK) return st;
-        near = (int64_t)ctx->near_host->total;
+        total = (int64_t)ctx->near_host->total;
+        near = (int64_t)ctx->near_host->fp64;
         if (ctx->near_host->overflow && arena)
-            return fail(ctx, CCOIL_ERR_ARGUMENT, "FP32C: " + std::to_string(near) + " near-winding (coil, point) "
+            return fail(ctx, CCOIL_ERR_ARGUMENT, "FP32C: " + std::to_string(total) + " near-winding (coil, point) "
                         "pairs exceed the " + std::to_string(L.p.near_cap) + " records the fixed scratch holds for "
                         "this call; the near contributions are missing; split the points");
         if (ctx->near_host->overflow) {
-            // more near pairs than the buffers hold: the near pass did nothing; grow and run it again
-            st = near_buffers(ctx, L, std::min<int64_t>(near + near / 8 + 4096, INT32_MAX), fp.width, nullptr);
+            // more records than the buffers hold: the pass did nothing; grow and run it again
+            st = near_buffers(ctx, L, std::min<int64_t>(total + total / 8 + 4096, INT32_MAX), fp.width, nullptr);
             if (st != CCOIL_OK) return st;
             enqueue_near_tail(ctx->stream, L, ctx->near_host);
             CC_HIP(ctx, hipGetLastError());
@@ -375,8 +389,10 @@ ccoil_status fields_call(ccoil_ctx ctx, const ccoil_coils *coils, const double *
             st = near_outcome(ctx, *ctx->near_host);
             if (st != CCOIL_OK) return st;
             if (ctx->near_host->overflow)
-                return fail(ctx, CCOIL_ERR_DEVICE, "FP32C near pass: capacity still exceeded after growing");
+                return fail(ctx, CCOIL_ERR_DEVICE, "record pass: capacity still exceeded after growing");
+            near = (int64_t)ctx->near_host->fp64;
         }
+        if (precision != CCOIL_FP32C) near = 0;   // FP64's records are its rule, not a re-evaluation
     }
     if (report) {
         float ms = 0.f, mms = 0.f;
@@ -424,6 +440,32 @@ extern "C" ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, co
         if (st != CCOIL_OK) return st;
         shape = &tuned;
     }
-    return fields_call(ctx, coils, points, n_points, quantities, precision, flags, A, B, E, G, report, nullptr,
-                       shape);
+    // A series-mode call too large for one record pass runs as pieces of points, each with its own pass (records
+    // grow with points x coils); a point's result does not depend on the piece that holds it
+    const bool far = (precision == CCOIL_FP32C && !(flags & CCOIL_DIRECT)) ||
+                     (precision == CCOIL_FP64 && (flags & CCOIL_FAR_FIELD));
+    const int64_t n_coils = coils ? coils->n : 0;
+    const int64_t piece = n_coils > 0 ? std::max<int64_t>(kOrderMinPoints, std::min<int64_t>(kRecordMaxPoints,
+                                            kRecordMaxPoints * 128 / n_coils)) : kRecordMaxPoints;
+    if (!ctx || !far || shape || n_points <= piece || !points)
+        return fields_call(ctx, coils, points, n_points, quantities, precision, flags, A, B, E, G, report, nullptr,
+                           shape);
+    if (report) std::memset(report, 0, sizeof *report);
+    for (int64_t s0 = 0; s0 < n_points; s0 += piece) {
+        const int64_t m = std::min<int64_t>(piece, n_points - s0);
+        ccoil_report r;
+        const ccoil_status st = fields_call(ctx, coils, points + 3 * s0, m, quantities, precision, flags,
+                                            A ? A + 3 * s0 : nullptr, B ? B + 3 * s0 : nullptr,
+                                            E ? E + 3 * s0 : nullptr, G ? G + 9 * s0 : nullptr, &r, nullptr, nullptr);
+        if (st != CCOIL_OK) return st;
+        if (report) {
+            report->node_evaluations += r.node_evaluations;
+            report->near_pairs += r.near_pairs;
+            report->kernel_ms += r.kernel_ms;
+            report->main_kernel_ms += r.main_kernel_ms;
+            report->block_size = r.block_size;
+            report->split = r.split;
+        }
+    }
+    return CCOIL_OK;
 }

@@ -11,9 +11,10 @@ constexpr int kFieldBlock = 256;
 
 // Device-side outcome of the FP32C near pass (read back by the host once the call's work is done).
 struct NearState {
-    unsigned long long total;   // near (coil, point) pairs counted by field_kernel
+    unsigned long long total;   // records (deferred (coil, point) pairs) counted by field_kernel
     unsigned int overflow;      // total > near_cap: the pass was skipped
     unsigned int mismatch;      // near_fill classified a pair differently from field_kernel
+    unsigned long long fp64;    // records of kind 0 (FP64 rule)
 };
 
 struct FieldParams {
@@ -25,6 +26,13 @@ struct FieldParams {
     uint32_t q;            // Q_* mask
     Tables tab;
     int32_t near_fp64;     // FP32C: skip near pairs here, re-evaluate in FP64
+    // Rule records (the series modes, large calls): field_kernel evaluates only the series and defers every other
+    // (coil, point) pair to the record pass below, which runs the rule on waves of one coil each. A record's kind is
+    // 0 for the FP64 rule (FP32C's near pairs; every rule pair in FP64) and 1 for the float rule (FP32C's others);
+    // its key is kind * n_coils + CoilDev::near_key, n_keys = n_coils * (1 + float_records).
+    int32_t rule_records;
+    int32_t float_records;
+    int32_t n_keys;
     // FP32C far field: the recurrence header and each coil's moments (kMpHeader + n_coils * kMpCoil doubles,
     // multipole.hpp); a coil with CoilDev::far_d2 > 0 takes its series beyond that distance. NULL: no coil does.
     const double *mp;
@@ -44,8 +52,8 @@ struct FieldParams {
     // the host grows the buffers and runs them again.
     int64_t *near_counts;
     const int64_t *near_offsets;
-    uint32_t *near_hist;      // [n_coils + 1] counts per near_key
-    uint32_t *near_cursor;    // [n_coils + 1] next free record of each key
+    uint32_t *near_hist;      // [n_keys + 1] counts per key
+    uint32_t *near_cursor;    // [n_keys + 1] next free record of each key
     NearState *near_state;
     int64_t near_cap;
     uint32_t *near_keys;
@@ -61,6 +69,17 @@ struct FieldParams {
     int32_t partial_width;
     double *partial;
 };
+
+// What becomes of a (coil, point) pair: -1 field_kernel evaluates it (the series beyond the far-field distance, or
+// the rule in the kernel), 0 an FP64-rule record, 1 a float-rule record. field_kernel, near_count and near_fill all
+// classify with this, bitwise alike (to_local and the tests round operation by operation).
+__device__ inline int record_kind(const FieldParams &p, const CoilDev &c, double z, double r, bool axis)
+{
+    if (is_far(c, z, r)) return -1;
+    const bool near = p.near_fp64 && is_near(c, z, r, axis);
+    if (p.rule_records) return (p.float_records && !near) ? 1 : 0;
+    return near ? 0 : -1;
+}
 
 // Adjoint of the summed B field: per-coil gradients of sum_x dL/dB(x) . B(x), 9 per coil in the order
 // current, R, a, b, px, py, pz, alpha, beta. Grid: (point blocks, coils); partial sums per (coil, block).

@@ -1,5 +1,7 @@
-// FP32C near-winding pass (gfx950): (coil, point) pairs inside a coil's near shell (and, for outputs that divide by
-// r, its axis cone) are re-evaluated in FP64.
+// Record pass (gfx950): the (coil, point) pairs field_kernel defers. In FP32C those inside a coil's near shell (and,
+// for outputs that divide by r, its axis cone) are re-evaluated in FP64; with rule records (the series modes, large
+// calls) every pair inside the far-field distance is, by the FP64 or the float rule (its record's kind), on waves
+// that share one coil instead of beside the series.
 //
 // field_kernel<kFP32C> skips those pairs and counts them per point; an exclusive scan of the point counts gives each
 // point's slots; near_count_kernel counts each coil's near points and near_key_scan_kernel turns those counts into
@@ -28,8 +30,12 @@ __global__ __launch_bounds__(256) void near_count_kernel(FieldParams p)
         const CoilDev &c = p.coils[ci];
         double z, r, cph, sph;
         to_local(c, px, py, pz, z, r, cph, sph);
-        const uint64_t m = __ballot(near_pass_pair(c, z, r, p.q & (Q_G | Q_GJ)));   // the field kernel's test, bitwise
-        if (m && lane == __builtin_ctzll(m)) atomicAdd(&p.near_hist[c.near_key], (unsigned)__popcll(m));
+        const int kind = record_kind(p, c, z, r, p.q & (Q_G | Q_GJ));   // the field kernel's test, bitwise
+        for (int k = 0; k <= p.float_records; ++k) {
+            const uint64_t m = __ballot(kind == k);
+            if (m && lane == __builtin_ctzll(m))
+                atomicAdd(&p.near_hist[k * p.n_coils + c.near_key], (unsigned)__popcll(m));
+        }
     }
 }
 
@@ -37,7 +43,7 @@ __global__ __launch_bounds__(256) void near_count_kernel(FieldParams p)
 __global__ __launch_bounds__(256) void near_key_scan_kernel(FieldParams p)
 {
     __shared__ uint32_t part[256];
-    const int t = threadIdx.x, nk = p.n_coils + 1;
+    const int t = threadIdx.x, nk = p.n_keys + 1;
     uint32_t carry = 0;
     for (int base = 0; base < nk; base += 256) {
         const uint32_t v = base + t < nk ? p.near_hist[base + t] : 0u;
@@ -58,6 +64,7 @@ __global__ __launch_bounds__(256) void near_key_scan_kernel(FieldParams p)
             (unsigned long long)(p.near_offsets[p.n_pts - 1] + p.near_counts[p.n_pts - 1]);
         p.near_state->total = total;
         p.near_state->overflow = total > (unsigned long long)p.near_cap ? 1u : 0u;
+        p.near_state->fp64 = p.near_cursor[p.n_coils];   // the kind-0 keys come first
     }
 }
 
@@ -80,35 +87,41 @@ __global__ __launch_bounds__(256) void near_fill_kernel(FieldParams p)
         const CoilDev &c = p.coils[ci];
         double z, r, cph, sph;
         to_local(c, px, py, pz, z, r, cph, sph);
-        const bool near = near_pass_pair(c, z, r, p.q & (Q_G | Q_GJ));   // the field kernel's test, bitwise
-        // the wave's records of this coil take consecutive places from one atomic on the coil's cursor
-        const uint64_t m = __ballot(near);
-        if (!m) continue;
-        const int leader = __builtin_ctzll(m);
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&p.near_cursor[c.near_key], (unsigned)__popcll(m));
-        base = __shfl(base, leader);
-        if (!near) continue;
-        const uint64_t pos = base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
-        if (slot >= end || pos >= total) {   // classified differently from field_kernel: report, write nothing
-            bad = true;
-            continue;
+        const int kind = record_kind(p, c, z, r, p.q & (Q_G | Q_GJ));   // the field kernel's test, bitwise
+        // the wave's records of this coil and kind take consecutive places from one atomic on the key's cursor
+        for (int k = 0; k <= p.float_records; ++k) {
+            const bool mine = kind == k;
+            const uint64_t m = __ballot(mine);
+            if (!m) continue;
+            const int leader = __builtin_ctzll(m);
+            const uint32_t key = (uint32_t)(k * p.n_coils + c.near_key);
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&p.near_cursor[key], (unsigned)__popcll(m));
+            base = __shfl(base, leader);
+            if (!mine) continue;
+            const uint64_t pos = base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
+            if (slot >= end || pos >= total) {   // classified differently from field_kernel: report, write nothing
+                bad = true;
+                continue;
+            }
+            p.near_keys[pos] = key;
+            p.near_recs[pos] = hi | (uint64_t)slot;
+            ++slot;
         }
-        p.near_keys[pos] = (uint32_t)c.near_key;
-        p.near_recs[pos] = hi | (uint64_t)slot;
-        ++slot;
     }
     if (bad || slot != end) atomicOr(&p.near_state->mismatch, 1u);
 }
 
 // Records sorted by coil: lane e takes the e-th record; its value goes to the record's slot.
-template <bool WA, bool WB, bool WG>
-__global__ __launch_bounds__(256) void near_eval_kernel(FieldParams p)
+// FLOAT: the kind-1 records [fp64, total) by the float rule, at 5 waves per SIMD as field_kernel's float instances;
+// else the kind-0 records [0, fp64) by the FP64 rule.
+template <bool WA, bool WB, bool WG, bool FLOAT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FLOAT ? 5 : 1))) void near_eval_kernel(FieldParams p)
 {
     if (p.near_state->overflow) return;
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t total = (int64_t)p.near_state->total;
-    if (e >= total) return;
+    const int64_t total = (int64_t)p.near_state->total, fp64 = (int64_t)p.near_state->fp64;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x + (FLOAT ? fp64 : 0);
+    if (e >= (FLOAT ? total : fp64)) return;
     const uint32_t key = p.near_keys[e];
     const uint64_t rec = p.near_recs[e];
     const int64_t pi = (int64_t)(rec >> 32), slot = (int64_t)(rec & 0xffffffffu);
@@ -130,12 +143,15 @@ __global__ __launch_bounds__(256) void near_eval_kernel(FieldParams p)
         const uint32_t k = __builtin_amdgcn_readfirstlane(__shfl(key, __builtin_ctzll(todo)));
         const bool mine = key == k;
         todo &= ~(uint64_t)__ballot(mine);
-        if (k >= (uint32_t)p.n_coils) continue;   // never a valid key; only a reported mismatch can leave one
-        const CoilDev &c = p.coils[p.near_coil[k]];
+        // only a reported mismatch can leave a key outside the range's kind
+        const uint32_t kind = k / (uint32_t)p.n_coils;
+        if (k >= (uint32_t)p.n_keys || kind != (FLOAT ? 1u : 0u)) continue;
+        const CoilDev &c = p.coils[p.near_coil[k - kind * (uint32_t)p.n_coils]];
         double z, r, cph, sph;
         to_local(c, px, py, pz, z, r, cph, sph);
         Local L;
-        eval_stable<f64s, WA, WB, WG>(c, p.tab, z, r, 0, 1, L);
+        if (FLOAT) eval_stable<float, WA, WB, WG>(c, p.tab, z, r, 0, 1, L);
+        else eval_stable<f64s, WA, WB, WG>(c, p.tab, z, r, 0, 1, L);
         if (mine) accumulate_global(c, L, r, cph, sph, p.q, acc);
     }
     if (p.A) { for (int j = 0; j < 3; ++j) o[j] = acc.A[j]; o += 3; }
@@ -274,22 +290,29 @@ void launch_near_fill(hipStream_t s, const FieldParams &p)
     hipLaunchKernelGGL(near_fill_kernel, dim3((unsigned)((p.n_pts + 255) / 256)), dim3(256), 0, s, p);
 }
 
-// The grid covers the capacity; threads beyond the call's total (read on the device) return at once.
-void launch_near_eval(hipStream_t s, const FieldParams &p)
+template <bool FLOAT>
+static void launch_near_eval_k(hipStream_t s, const FieldParams &p)
 {
-    if (p.near_cap <= 0) return;
     const unsigned blocks = (unsigned)((p.near_cap + 255) / 256);
     const bool wa = p.q & (Q_A | Q_E), wb = p.q & Q_B, wg = p.q & (Q_G | Q_GJ);
     const int code = (wa ? 4 : 0) | (wb ? 2 : 0) | (wg ? 1 : 0);
     switch (code) {
-        case 1: hipLaunchKernelGGL((near_eval_kernel<false, false, true>), dim3(blocks), dim3(256), 0, s, p); break;
-        case 2: hipLaunchKernelGGL((near_eval_kernel<false, true, false>), dim3(blocks), dim3(256), 0, s, p); break;
-        case 3: hipLaunchKernelGGL((near_eval_kernel<false, true, true>), dim3(blocks), dim3(256), 0, s, p); break;
-        case 4: hipLaunchKernelGGL((near_eval_kernel<true, false, false>), dim3(blocks), dim3(256), 0, s, p); break;
-        case 5: hipLaunchKernelGGL((near_eval_kernel<true, false, true>), dim3(blocks), dim3(256), 0, s, p); break;
-        case 6: hipLaunchKernelGGL((near_eval_kernel<true, true, false>), dim3(blocks), dim3(256), 0, s, p); break;
-        default: hipLaunchKernelGGL((near_eval_kernel<true, true, true>), dim3(blocks), dim3(256), 0, s, p); break;
+        case 1: hipLaunchKernelGGL((near_eval_kernel<false, false, true, FLOAT>), dim3(blocks), dim3(256), 0, s, p); break;
+        case 2: hipLaunchKernelGGL((near_eval_kernel<false, true, false, FLOAT>), dim3(blocks), dim3(256), 0, s, p); break;
+        case 3: hipLaunchKernelGGL((near_eval_kernel<false, true, true, FLOAT>), dim3(blocks), dim3(256), 0, s, p); break;
+        case 4: hipLaunchKernelGGL((near_eval_kernel<true, false, false, FLOAT>), dim3(blocks), dim3(256), 0, s, p); break;
+        case 5: hipLaunchKernelGGL((near_eval_kernel<true, false, true, FLOAT>), dim3(blocks), dim3(256), 0, s, p); break;
+        case 6: hipLaunchKernelGGL((near_eval_kernel<true, true, false, FLOAT>), dim3(blocks), dim3(256), 0, s, p); break;
+        default: hipLaunchKernelGGL((near_eval_kernel<true, true, true, FLOAT>), dim3(blocks), dim3(256), 0, s, p); break;
     }
+}
+
+// Each grid covers the capacity; threads beyond their kind's range (read on the device) return at once.
+void launch_near_eval(hipStream_t s, const FieldParams &p)
+{
+    if (p.near_cap <= 0) return;
+    launch_near_eval_k<false>(s, p);
+    if (p.float_records) launch_near_eval_k<true>(s, p);
 }
 
 void launch_near_add(hipStream_t s, const FieldParams &p)

@@ -55,6 +55,7 @@ extern "C" ccoil_status ccoil_fields_noalloc(ccoil_ctx ctx, const ccoil_coils *c
     Arena arena;
     arena.base = (char *)ctx->scratch;
     arena.cap = kScratchBytes;
+    arena.fixed = true;
     return fields_call(ctx, coils, points, n_points, quantities, precision, flags, A, B, E, G, report, &arena,
                        nullptr);
 }

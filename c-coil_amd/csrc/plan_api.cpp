@@ -18,8 +18,9 @@ struct ccoil_plan_s {
     bool launched = false;
     DevBuf mem;                         // field plans: every buffer of the call, handed out by an Arena
     DevBuf pbufs[4];                    // pair plans: primary coils, secondary coils, work list, partials
-    NearState *host_state = nullptr;    // FP32C field plans: the last launch's near-pass outcome (pinned)
+    NearState *host_state = nullptr;    // field plans with a record pass: the last launch's outcome (pinned)
     bool near = false;
+    bool fp32c = false;
     int64_t near_cap = 0;
     int64_t node_evaluations = 0;
     int32_t split = 1;
@@ -86,14 +87,16 @@ ccoil_status ccoil_fields_plan(ccoil_ctx ctx, const ccoil_coils *coils, const do
         if (st != CCOIL_OK) return st;
         sh = &shape;
     }
-    // FP32C: one eager call finds the near pairs of these points; the plan holds a quarter more
+    // a call with a record pass (FP32C's near pairs, the series modes' rule records): one eager call counts the
+    // records of these points; the plan holds a quarter more
     int64_t cap = 0;
-    if (precision == CCOIL_FP32C && !(flags & CCOIL_NO_NEAR_FP64) && n_points > 0) {
+    if (field_record_pass(precision, flags, n_points, false, nullptr) && n_points > 0) {
         ccoil_report rep;
         const ccoil_status st = fields_call(ctx, coils, points, n_points, quantities, precision, flags, A, B, E, G,
                                             &rep, nullptr, sh);
         if (st != CCOIL_OK) return st;
-        cap = rep.near_pairs + rep.near_pairs / 4 + 4096;
+        const int64_t total = (int64_t)ctx->near_host->total;
+        cap = total + total / 4 + 4096;
     }
     ccoil_plan plan;
     ccoil_status st = new_plan(ctx, out, plan);
@@ -115,10 +118,11 @@ ccoil_status ccoil_fields_plan(ccoil_ctx ctx, const ccoil_coils *coils, const do
     if (pp.n_chunks > 1) bytes += sizeof(double) * (size_t)pp.partial_width * (size_t)probe.n * (size_t)pp.n_chunks + 256;
     bytes += probe.L.far_bytes;
     if (probe.L.near) {
-        bytes += 2 * sizeof(int64_t) * (size_t)probe.n + 2 * sizeof(uint32_t) * (nc + 2) + sizeof(NearState) +
+        bytes += 2 * sizeof(int64_t) * (size_t)probe.n + 2 * sizeof(uint32_t) * ((size_t)pp.n_keys + 2) + sizeof(NearState) +
                  probe.L.scan_bytes + (size_t)pp.near_cap * (sizeof(uint32_t) + sizeof(uint64_t) +
                                                              sizeof(double) * (size_t)probe.width) + 8 * 256;
         plan->near = true;
+        plan->fp32c = precision == CCOIL_FP32C;
         plan->near_cap = pp.near_cap;
         if (hipHostMalloc((void **)&plan->host_state, sizeof(NearState), hipHostMallocDefault) != hipSuccess) {
             plan->host_state = nullptr;
@@ -233,15 +237,16 @@ ccoil_status ccoil_plan_synchronize(ccoil_plan plan, ccoil_report *report)
         const ccoil_status st = near_outcome(ctx, *plan->host_state);
         if (st != CCOIL_OK) return st;
         if (plan->host_state->overflow)
-            return fail(ctx, CCOIL_ERR_CAPACITY, "FP32C plan: this launch found " +
-                        std::to_string(plan->host_state->total) + " near-winding (coil, point) pairs, the plan holds " +
-                        std::to_string(plan->near_cap) + "; its outputs lack the near contributions; make a new plan");
+            return fail(ctx, CCOIL_ERR_CAPACITY, "field plan: this launch found " +
+                        std::to_string(plan->host_state->total) + " record-pass (coil, point) pairs (near-winding "
+                        "or rule records), the plan holds " + std::to_string(plan->near_cap) +
+                        "; its outputs lack those contributions; make a new plan");
     }
     if (report) {
         float ms = 0.f;
         CC_HIP(ctx, hipEventElapsedTime(&ms, plan->ev0, plan->ev1));
         report->node_evaluations = plan->node_evaluations;
-        report->near_pairs = plan->near ? (int64_t)plan->host_state->total : 0;
+        report->near_pairs = plan->near && plan->fp32c ? (int64_t)plan->host_state->fp64 : 0;
         report->kernel_ms = ms;
         report->main_kernel_ms = ms;
         report->block_size = plan->block;
