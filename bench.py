@@ -52,7 +52,7 @@ if PKG not in sys.path:
 METRIC = "coil·point B+∇B evals/s and M-pairs/s, FP32-comp vs FP64 at ≤1e-6 rel err"
 # committed rocprofv3 --pmc summaries of field_kernel per mode (HBM bytes per launch)
 PMC_FILES = {"fp64": os.path.join(ROOT, "profiles", "field_kernel_fp64_group128_points1M_pmc.md"),
-             "fp64_far_field": os.path.join(ROOT, "profiles", "field_kernel_fp64_far_field_group128_points1M_pmc.md")}
+             "fp64_far_field": os.path.join(ROOT, "profiles", "near_eval_kernel_fp64_far_field_group128_points1M_pmc.md")}
 
 
 def log(*a):
@@ -199,7 +199,8 @@ def main():
         dt = (time.perf_counter() - t0) / steps
         main_ms = float(np.mean([r.main_kernel_ms for r in reps]))
         all_ms = float(np.mean([r.kernel_ms for r in reps]))
-        return dt, main_ms, all_ms, reps[-1]
+        rec_ms = float(np.mean([r.record_ms for r in reps]))
+        return dt, main_ms, all_ms, reps[-1], rec_ms
 
     # field modes: (precision, flags, label); the series modes take each coil's far-field series beyond its
     # distance (ccoil_far_field_radius) and the rule inside it
@@ -212,7 +213,10 @@ def main():
     works = {}
 
     def mode_work(name):
-        """(fp32 flops, fp64 flops, special ops) of one field_kernel launch of the mode."""
+        """Algorithmic work of one call of the mode, by kernel: {part: (fp32 flops, fp64 flops, special ops)}. The
+        series modes at this size run field_kernel for the series alone ("series") and the record pass for every pair
+        inside the far-field distance ("records": the FP64 rule, or FP32C's float rule plus its FP64 near pairs);
+        the others run the rule in field_kernel ("field")."""
         if name not in works:
             prec, flags, _ = modes[name]
             series = (prec == abi.FP32C and not flags & abi.DIRECT) or (prec == abi.FP64 and flags & abi.FAR_FIELD)
@@ -220,37 +224,52 @@ def main():
                 rmax, dist = abi.far_field_radius(cs, prec)
                 w = RM.series_field_work(rows, pts, dist, rmax, RM.LOG_TOL_FP32C if prec == abi.FP32C
                                          else RM.LOG_TOL_FP64, with_gradient=True, near_pass=prec == abi.FP32C)
-                r32 = w["rule_flops"] if prec == abi.FP32C else 0.0
-                works[name] = (r32, w["series_flops"] + w["rule_flops"] - r32, w["rule_special"] + w["series_special"],
-                               w)
+                rec = (w["rule_flops"], w["near_flops"], w["rule_special"]) if prec == abi.FP32C else \
+                    (0.0, w["rule_flops"], w["rule_special"])
+                works[name] = ({"series": (0.0, w["series_flops"], w["series_special"]), "records": rec}, w)
             else:
                 f32 = prec in (abi.FP32C, abi.FP32)
-                works[name] = (flops if f32 else 0.0, 0.0 if f32 else flops, trans, None)
+                works[name] = ({"field": (flops if f32 else 0.0, 0.0 if f32 else flops, trans)}, None)
         return works[name]
 
-    def roofline(name, main_ms):
-        f32, f64, sp, w = mode_work(name)
-        t = main_ms * 1e-3
+    def part_roofline(kernel, work, ms):
+        f32, f64, sp = work
+        t = ms * 1e-3
         at_peak = f32 / (RM.PEAK_FP32_TFLOPS * 1e12) + f64 / (RM.PEAK_FP64_TFLOPS * 1e12)
         achieved = (f32 + f64) / t / 1e12
         peak = (f32 + f64) / at_peak / 1e12
-        rl = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-              "kernel": "field_kernel", "kernel_ms": main_ms, "algorithmic_flops_per_launch": f32 + f64,
-              "fp32_flops": f32, "fp64_flops": f64,
-              "transcendental": {"achieved_ops": sp / t, "peak_ops": RM.PEAK_TRANS_OPS,
-                                 "frac": sp / t / RM.PEAK_TRANS_OPS},
-              "hbm_algorithmic_bytes_per_launch": n * (24 + 24 + 72)}
-        if w is not None:
+        return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                "kernel": kernel, "kernel_ms": ms, "algorithmic_flops_per_launch": f32 + f64, "fp32_flops": f32,
+                "fp64_flops": f64, "transcendental": {"achieved_ops": sp / t, "peak_ops": RM.PEAK_TRANS_OPS,
+                                                      "frac": sp / t / RM.PEAK_TRANS_OPS}}
+
+    def roofline(name, main_ms, rec_ms, all_ms):
+        """The dominant kernel's roofline, with the other part and the whole call (all kernels) beside it."""
+        parts, w = mode_work(name)
+        if "field" in parts:
+            rl = part_roofline("field_kernel", parts["field"], main_ms)
+        else:
+            ser = part_roofline("field_kernel (the series)", parts["series"], main_ms)
+            rec = part_roofline("near_eval_kernel (rule records)", parts["records"], rec_ms)
+            rl = dict(rec if rec_ms >= main_ms else ser)
+            tot = tuple(a + b for a, b in zip(parts["series"], parts["records"]))
+            rl["parts"] = {"series": ser, "records": rec, "call": part_roofline("all kernels of the call", tot, all_ms)}
             rl["pairs"] = {"rule": w["rule_pairs"], "series": w["series_pairs"], "near_pass": w["near_pairs"]}
+        # field_kernel: the points in, B and the gradient out; near_eval_kernel: per record its key, point and slot
+        # index and its point in, its B and gradient out
+        records = (w["rule_pairs"] + w["near_pairs"]) if w is not None else 0
+        rl["hbm_algorithmic_bytes_per_launch"] = records * (4 + 8 + 24 + 96) if rl["kernel"].startswith("near_eval") \
+            else n * (24 + 24 + 72)
         return rl
 
     head_name = args.precision
     head_prec, head_flags, head_label = modes[head_name]
     head = new_out()
-    dt, main_ms, all_ms, rep = timed(head_prec, args.steps, args.warmup, head, head_flags)
+    dt, main_ms, all_ms, rep, rec_ms = timed(head_prec, args.steps, args.warmup, head, head_flags)
     value = len(rows) * n / dt
-    log(f"{head_name}: {dt * 1e3:.2f} ms/step, field kernel {main_ms:.2f} ms, all kernels {all_ms:.2f} ms")
-    rl = roofline(head_name, main_ms)
+    log(f"{head_name}: {dt * 1e3:.2f} ms/step, field kernel {main_ms:.2f} ms, record pass {rec_ms:.2f} ms, "
+        f"all kernels {all_ms:.2f} ms")
+    rl = roofline(head_name, main_ms, rec_ms, all_ms)
     rl["traffic"] = pmc_traffic(head_name)
     result = {"metric": METRIC, "value": value, "unit": "coil·point evals/s", "n_gpus": 1,
               "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
@@ -259,7 +278,7 @@ def main():
               "config": {"workload": f"B+gradient of a {len(rows)}-coil mixed CoilGroup at {n} points",
                          "coils": len(rows), "points": n, "precision_factor": 5.0, "quantities": "B+G",
                          "precision": head_label, "seed": 0},
-              "roofline": rl, "kernel_ms": {"field_kernel": main_ms, "all": all_ms}}
+              "roofline": rl, "kernel_ms": {"field_kernel": main_ms, "record_pass": rec_ms, "all": all_ms}}
 
     if not args.no_side:
         side, outs = {}, {head_name: head}
@@ -267,11 +286,16 @@ def main():
             if name == head_name:
                 continue
             outs[name] = new_out()
-            sdt, sms, sall, srep = timed(prec, args.side_steps, 1, outs[name], flags)
+            sdt, sms, sall, srep, srec = timed(prec, args.side_steps, 1, outs[name], flags)
+            srl = roofline(name, sms, srec, sall)
             side[name] = {"mode": label, "value": len(rows) * n / sdt, "ms_per_step": sdt * 1e3, "kernel_ms": sms,
-                          "all_ms": sall, "roofline_frac": roofline(name, sms)["frac"],
-                          "near_pairs": int(srep.near_pairs), "value_over_headline": (len(rows) * n / sdt) / value}
-            log(f"{name}: {sdt * 1e3:.1f} ms/step, field kernel {sms:.2f} ms, all kernels {sall:.2f} ms")
+                          "record_ms": srec, "all_ms": sall, "roofline_kernel": srl["kernel"],
+                          "roofline_frac": srl["frac"], "near_pairs": int(srep.near_pairs),
+                          "value_over_headline": (len(rows) * n / sdt) / value}
+            if "parts" in srl:
+                side[name]["call_frac"] = srl["parts"]["call"]["frac"]
+            log(f"{name}: {sdt * 1e3:.1f} ms/step, field kernel {sms:.2f} ms, record pass {srec:.2f} ms, all kernels "
+                f"{sall:.2f} ms")
         result["side"] = side
         b64, g64 = outs["fp64"]["B"].cpu().numpy(), outs["fp64"]["G"].cpu().numpy()
         near = W.near_mask(rows, pts)

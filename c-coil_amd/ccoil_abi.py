@@ -38,7 +38,8 @@ class Coils(ctypes.Structure):
 
 class Report(ctypes.Structure):
     _fields_ = [("node_evaluations", ctypes.c_int64), ("near_pairs", ctypes.c_int64), ("kernel_ms", ctypes.c_double),
-                ("block_size", ctypes.c_int32), ("split", ctypes.c_int32), ("main_kernel_ms", ctypes.c_double)]
+                ("block_size", ctypes.c_int32), ("split", ctypes.c_int32), ("main_kernel_ms", ctypes.c_double),
+                ("record_ms", ctypes.c_double)]
 
 
 class CCoilError(RuntimeError):

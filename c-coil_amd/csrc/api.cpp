@@ -225,6 +225,8 @@ ccoil_status ccoil_create(int device, ccoil_ctx *out)
     if ((e = hipEventCreate(&ctx->ev1)) != hipSuccess) return bail(e, "hipEventCreate");
     if ((e = hipEventCreate(&ctx->evm0)) != hipSuccess) return bail(e, "hipEventCreate");
     if ((e = hipEventCreate(&ctx->evm1)) != hipSuccess) return bail(e, "hipEventCreate");
+    if ((e = hipEventCreate(&ctx->evr0)) != hipSuccess) return bail(e, "hipEventCreate");
+    if ((e = hipEventCreate(&ctx->evr1)) != hipSuccess) return bail(e, "hipEventCreate");
     if ((e = hipHostMalloc((void **)&ctx->near_host, sizeof(NearState), hipHostMallocDefault)) != hipSuccess)
         return bail(e, "hipHostMalloc");
     if ((e = scratch_address(device, &ctx->scratch)) != hipSuccess) return bail(e, "hipGetSymbolAddress(scratch)");
@@ -271,6 +273,8 @@ ccoil_status ccoil_destroy(ccoil_ctx ctx)
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->evm0) (void)hipEventDestroy(ctx->evm0);
     if (ctx->evm1) (void)hipEventDestroy(ctx->evm1);
+    if (ctx->evr0) (void)hipEventDestroy(ctx->evr0);
+    if (ctx->evr1) (void)hipEventDestroy(ctx->evr1);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return CCOIL_OK;

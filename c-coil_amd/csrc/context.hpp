@@ -73,6 +73,7 @@ struct ccoil_ctx_s {
     ccoil::DevBuf zitems, zpartial, zout;               // coaxial rules
     hipEvent_t ev0 = nullptr, ev1 = nullptr;   // around all compute kernels of a call
     hipEvent_t evm0 = nullptr, evm1 = nullptr; // around the main kernel (field_kernel / pair_kernel)
+    hipEvent_t evr0 = nullptr, evr1 = nullptr; // around the record pass's evaluation kernels
     std::map<std::string, std::string> tuning;  // autotuner cache: call key -> chosen launch shape
     std::string tuning_file;                    // where new tuning entries are appended ("" = memory only)
 };
@@ -116,9 +117,12 @@ double far_threshold(int n_ang, double q, double tol);
 // Enqueues the field kernel, the chunk reduction and (FP32C) the whole near pass on s, with no host
 // synchronisation and kernel launches and copies only (so it can be captured in a graph); evm0/evm1 (may be NULL)
 // bracket the field kernel. The near pass's outcome is copied to host_state (pinned) when given.
-void enqueue_fields(hipStream_t s, const FieldLaunch &L, hipEvent_t evm0, hipEvent_t evm1, NearState *host_state);
-// The near pass from the per-coil cursors on (after a grow-on-overflow retry): key scan, fill, eval, add.
-void enqueue_near_tail(hipStream_t s, const FieldLaunch &L, NearState *host_state);
+void enqueue_fields(hipStream_t s, const FieldLaunch &L, hipEvent_t evm0, hipEvent_t evm1, NearState *host_state,
+                    hipEvent_t evr0 = nullptr, hipEvent_t evr1 = nullptr);
+// The near pass from the per-coil cursors on (after a grow-on-overflow retry): key scan, fill, eval, add; evr0 / evr1
+// (may be NULL) bracket the evaluation kernels.
+void enqueue_near_tail(hipStream_t s, const FieldLaunch &L, NearState *host_state, hipEvent_t evr0 = nullptr,
+                       hipEvent_t evr1 = nullptr);
 
 ccoil_status fail(ccoil_ctx ctx, ccoil_status code, const std::string &msg);
 Tables ctx_tables(ccoil_ctx ctx);

@@ -138,16 +138,19 @@ ccoil_status near_outcome(ccoil_ctx ctx, const NearState &st)
     return CCOIL_OK;
 }
 
-void enqueue_near_tail(hipStream_t s, const FieldLaunch &L, NearState *host_state)
+void enqueue_near_tail(hipStream_t s, const FieldLaunch &L, NearState *host_state, hipEvent_t evr0, hipEvent_t evr1)
 {
     launch_near_key_scan(s, L.p);
     launch_near_fill(s, L.p);
+    if (evr0) (void)hipEventRecord(evr0, s);
     launch_near_eval(s, L.p);
+    if (evr1) (void)hipEventRecord(evr1, s);
     launch_near_add(s, L.p);
     if (host_state) (void)hipMemcpyAsync(host_state, L.p.near_state, sizeof(NearState), hipMemcpyDeviceToHost, s);
 }
 
-void enqueue_fields(hipStream_t s, const FieldLaunch &L, hipEvent_t evm0, hipEvent_t evm1, NearState *host_state)
+void enqueue_fields(hipStream_t s, const FieldLaunch &L, hipEvent_t evm0, hipEvent_t evm1, NearState *host_state,
+                    hipEvent_t evr0, hipEvent_t evr1)
 {
     const FieldParams &p = L.p;
     if (p.mp) launch_multipole_moments(s, p.coils, p.n_coils, p.tab, const_cast<double *>(p.mp));
@@ -167,7 +170,7 @@ void enqueue_fields(hipStream_t s, const FieldLaunch &L, hipEvent_t evm0, hipEve
         size_t tb = L.scan_bytes;
         (void)near_scan(s, p.near_counts, const_cast<int64_t *>(p.near_offsets), p.n_pts, L.scan_temp, tb);
         launch_near_count(s, p);
-        enqueue_near_tail(s, L, host_state);
+        enqueue_near_tail(s, L, host_state, evr0, evr1);
     }
 }
 
@@ -352,7 +355,7 @@ ccoil_status fields_call(ccoil_ctx ctx, const ccoil_coils *coils, const double *
     FieldLaunch &L = fp.L;
     const FieldParams &p = L.p;
     CC_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    enqueue_fields(ctx->stream, L, ctx->evm0, ctx->evm1, L.near ? ctx->near_host : nullptr);
+    enqueue_fields(ctx->stream, L, ctx->evm0, ctx->evm1, L.near ? ctx->near_host : nullptr, ctx->evr0, ctx->evr1);
     CC_HIP(ctx, hipGetLastError());
     CC_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     auto copy_out = [&]() -> ccoil_status {
@@ -380,7 +383,7 @@ ccoil_status fields_call(ccoil_ctx ctx, const ccoil_coils *coils, const double *
             // more records than the buffers hold: the pass did nothing; grow and run it again
             st = near_buffers(ctx, L, std::min<int64_t>(total + total / 8 + 4096, INT32_MAX), fp.width, nullptr);
             if (st != CCOIL_OK) return st;
-            enqueue_near_tail(ctx->stream, L, ctx->near_host);
+            enqueue_near_tail(ctx->stream, L, ctx->near_host, ctx->evr0, ctx->evr1);
             CC_HIP(ctx, hipGetLastError());
             CC_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
             st = copy_out();
@@ -402,6 +405,11 @@ ccoil_status fields_call(ccoil_ctx ctx, const ccoil_coils *coils, const double *
         report->near_pairs = near;
         report->kernel_ms = ms;
         report->main_kernel_ms = mms;
+        if (L.near) {
+            float rms = 0.f;
+            CC_HIP(ctx, hipEventElapsedTime(&rms, ctx->evr0, ctx->evr1));
+            report->record_ms = rms;
+        }
         report->block_size = kFieldBlock;
         report->split = L.split;
     }
@@ -463,6 +471,7 @@ extern "C" ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, co
             report->near_pairs += r.near_pairs;
             report->kernel_ms += r.kernel_ms;
             report->main_kernel_ms += r.main_kernel_ms;
+            report->record_ms += r.record_ms;
             report->block_size = r.block_size;
             report->split = r.split;
         }

@@ -70,7 +70,7 @@ def series_field_work(rows, pts, dist, rmax, log_tol, with_gradient=True, near_p
     import numpy as np
     from workloads import rotation
     out = dict(rule_flops=0.0, rule_special=0.0, series_flops=0.0, series_special=0.0, rule_pairs=0,
-               series_pairs=0, near_pairs=0)
+               series_pairs=0, near_pairs=0, near_flops=0.0)
     sp, sf, st = SERIES_BG if with_gradient else SERIES_B
     for r, d, rm in zip(rows, dist, rmax):
         x = pts - np.asarray(r["pos"])
@@ -100,6 +100,7 @@ def series_field_work(rows, pts, dist, rmax, log_tol, with_gradient=True, near_p
         out["rule_pairs"] += nr
         out["series_pairs"] += int(far.sum())
         out["near_pairs"] += int(near.sum())
+        out["near_flops"] += float(near.sum()) * nodes * f
     return out
 
 

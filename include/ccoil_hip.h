@@ -127,6 +127,7 @@ typedef struct {
     int32_t split;             /* fields: lanes cooperating on one point; pairs: most work items one pair's
                                   primary thickness rows were split over */
     double main_kernel_ms;     /* device time of the main kernel alone (field_kernel or pair_kernel; hipEvent) */
+    double record_ms;          /* fields with a record pass: device time of its evaluation kernels (hipEvent) */
 } ccoil_report;
 
 /* Context lifecycle. ccoil_last_error(NULL) returns the last error of ccoil_create on this thread. */
