@@ -205,22 +205,9 @@ template <typename T> __device__ inline T frexp_(T x, int *e);
 template <> __device__ inline double frexp_<double>(double x, int *e) { return frexp(x, e); }
 template <> __device__ inline float frexp_<float>(float x, int *e) { return frexpf(x, e); }
 
-// log(1 + x) for x >= 0 in one branch-free instruction stream (the lanes of a wave take different x):
-// 1 + x = m 2^e with m in [sqrt(1/2), sqrt(2)), log(1 + x) = e ln 2 + 2 atanh(u), u = (m - 1) / (m + 1), |u| <= 3 - 2 sqrt 2.
-// 2 atanh(u) / u is a polynomial in t = u^2 on [0, (3 - 2 sqrt 2)^2]: a Chebyshev fit of degree 7 (double; relative
-// error 3e-18 with the rounded coefficients below) or 3 (float; 1.7e-9), in place of the Taylor series of atanh.
-// For e = 0 the numerator is x itself (exact), so small x keep full relative accuracy; for e != 0, m - 1 is exact
-// (Sterbenz) and log(1 + x) >= 0.34 bounds the effect of rounding 1 + x.
-template <typename P> __device__ inline real_t<P> log1p_pos(real_t<P> x)
+// 2 atanh(u) / u as a polynomial in t = u^2 on [0, (3 - 2 sqrt 2)^2] (the fit log1p_pos describes).
+template <typename T> __device__ inline T atanh2_poly(T t)
 {
-    using T = real_t<P>;
-    int e;
-    T m = frexp_<T>((T)1 + x, &e);                       // m in [1/2, 1)
-    const bool lo = m < (T)0.70710678118654752440;
-    m = lo ? m + m : m;
-    e = lo ? e - 1 : e;
-    const T num = e == 0 ? x : m - (T)1;
-    const T u = num * prcp<P>(m + (T)1), t = u * u;
     T s;
     if constexpr (sizeof(T) == 8) {
         s = (T)0.14809710360655276;
@@ -237,7 +224,38 @@ template <typename P> __device__ inline real_t<P> log1p_pos(real_t<P> x)
         s = fma_<T>(s, t, (T)0.6666682f);
         s = fma_<T>(s, t, (T)2.0f);
     }
-    return fma_<T>((T)e, (T)0.69314718055994530942, u * s);
+    return s;
+}
+
+// log(1 + x) for x >= 0 in one branch-free instruction stream (the lanes of a wave take different x):
+// 1 + x = m 2^e with m in [sqrt(1/2), sqrt(2)), log(1 + x) = e ln 2 + 2 atanh(u), u = (m - 1) / (m + 1), |u| <= 3 - 2 sqrt 2.
+// 2 atanh(u) / u is a polynomial in t = u^2 on [0, (3 - 2 sqrt 2)^2]: a Chebyshev fit of degree 7 (double; relative
+// error 3e-18 with the rounded coefficients below) or 3 (float; 1.7e-9), in place of the Taylor series of atanh.
+// For e = 0 the numerator is x itself (exact), so small x keep full relative accuracy; for e != 0, m - 1 is exact
+// (Sterbenz) and log(1 + x) >= 0.34 bounds the effect of rounding 1 + x.
+template <typename P> __device__ inline real_t<P> log1p_pos(real_t<P> x)
+{
+    using T = real_t<P>;
+    int e;
+    T m = frexp_<T>((T)1 + x, &e);                       // m in [1/2, 1)
+    const bool lo = m < (T)0.70710678118654752440;
+    m = lo ? m + m : m;
+    e = lo ? e - 1 : e;
+    const T num = e == 0 ? x : m - (T)1;
+    const T u = num * prcp<P>(m + (T)1);
+    return fma_<T>((T)e, (T)0.69314718055994530942, u * atanh2_poly<T>(u * u));
+}
+
+// log(1 + x) for x = N / D >= 0, given v = x / (2 + x) = N / (2 D + N) from a reciprocal the caller shares with other
+// terms: 2 atanh(v) = v atanh2_poly(v^2) while v <= 3 - 2 sqrt 2 (x <= sqrt 2 - 1: log1p_pos's e = 0 case, without its
+// range reduction and its own reciprocal); lanes beyond that (nodes close to the winding) take log1p_pos(N / D). A
+// lane's value depends only on its own arguments.
+template <typename P> __device__ inline real_t<P> log1p_ratio(real_t<P> N, real_t<P> D, real_t<P> v)
+{
+    using T = real_t<P>;
+    T r = v * atanh2_poly<T>(v * v);
+    if (v > (T)0.17157287525380990) r = log1p_pos<P>(N * prcp<P>(D));
+    return r;
 }
 
 template <typename T> __device__ inline const T *tab_wcos(const Tables &t);
@@ -319,139 +337,152 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
         // same ? x : 1 as one exact FMA, fma(sg, x, sg1): a 64-bit select takes two instructions
         const T sg = same ? (T)1 : (T)0, sg1 = (T)1 - sg;
         const T bf = (T)c.b;
-        for (int j = row0; j < nt; j += rstep) {
-            const double rho_d = c.R + 0.5 * c.a * (1.0 + xt[j]);
-            const double dd = rho_d - rd;
-            const T base = (T)(dd * dd);
-            const T q = (T)(2.0 * rho_d * rd);
-            const T dm = (T)dd;                 // rho - r
-            const T rf = (T)rd, rho = (T)rho_d;
-            const T base2 = (T)(dd * dd * dd * dd);
-            const T n1 = (T)(dd * dd * (2.0 * rho_d * rd - dd * dd));
-            const T n2 = (T)(-3.0 * rho_d * rd * dd * dd);
-            const T n3 = (T)(-2.0 * rho_d * rho_d * rd * rd);
-            T rBh = 0, rBz = 0, rRZ = 0, rZZ = 0, rA = 0;
-            T rRRs = 0, rRRc = 0;   // sum w cos s dL and sum w cos dL: the row's RR is rho rRRs - dm rRRc
-            T rBzs = 0;             // double without the gradient: sum w s dgM (rBz then holds sum w dgM)
-            T eR = 0, eZ = 0;   // adjoint: end rings, sum w cos (z1 K1^3 + z2 K2^3), sum w (rho - r cos)(K1^3 + K2^3)
-            T eA = 0;           // pair adjoint: end rings of A, sum w cos (K1 + K2)
-            T rAr = 0;          // pair adjoint: sum w cos (r - rho cos) dgM
-#pragma unroll kStableUnroll<T, WA>
-            for (int i = 0; i < na; ++i) {
-                const T s = st[i], w = wa[i], wc = wct[i];
-                const T M = fma_<T>(q, s, base);
-                const T J1 = M + z1s, J2 = M + z2s;
-                const T K1 = prsq<P>(J1), K2 = prsq<P>(J2);
-                // A: asinh(z1 / sqrt(M)) - asinh(z2 / sqrt(M)) = log(P1 / P2), P_k = z_k + sqrt(M + z_k^2),
-                // = log1p(x) with x = (P1 - P2) / P2 = (b (sJ1 + sJ2) + 2 b |z|) / ((sJ1 + sJ2) P2) >= 0;
-                // for z2 < 0, P2 = M / (sJ2 - z2). A is even in z, so |z| serves directly. x = numr * nsel / Da.
-                T numr = 0, nsel = 0, Da = 1;
-                if (WA) {
-                    const T sJa1 = J1 * K1, sJa2 = J2 * K2;
-                    const T sum = sJa1 + sJa2;
-                    numr = fma_<T>(bf, sum, tbz);
-                    nsel = fma_<T>(sg1, sJa2 - z2, sg);                    // same ? 1 : sJ2 - z2
-                    Da = sum * (same ? z2 + sJa2 : M);
-                }
-                T Ra = 0;   // 1 / Da
-                if (WB || WG) {
-                    const T K12 = K1 * K2;
-                    const T tP2 = tbz * (K12 * K12);
-                    const T g2 = z2 * K2;
-                    const T gs = fma_<T>(z1, K1, g2);                   // z1 K1 + z2 K2
-                    const T gsel = fma_<T>(sg, gs, sg1);                // same ? gs : 1
-                    const T Y = M * gsel;
-                    const T Ks = K1 + K2;
-                    const T KsY = Ks * Y;
-                    // one reciprocal for 1 / (K1 + K2), 1 / Y and, with A, 1 / Da
-                    const T R = prcp<P>(WA ? KsY * Da : KsY);
-                    const T Rb = WA ? Da * R : R;
-                    if (WA) Ra = KsY * R;
-                    const T S = Y * Rb, Rr = Ks * Rb;
-                    const T dK = -tP2 * S;                              // K1 - K2
-                    const T invM = gsel * Rr;
-                    const T dgM = (same ? tP2 * M : fma_<T>(z1, K1, -g2)) * Rr;   // (z1 K1 - z2 K2) / M
-                    rBh = fma_<T>(wc, -dK, rBh);       // B_rho, also the gradient's rho-phi term
-                    if constexpr (ADA) {
-                        eA = fma_<T>(wc, Ks, eA);
-                        rAr = fma_<T>(wc * fma_<T>(rho, s, -dm), dgM, rAr);   // r - rho cos = rho s - (rho - r)
+        // The rows, for the wave's lanes all beyond the span (kAll: the selects above resolve at compile time) or
+        // for any mix; the two instances give bitwise the same values (fma(0, x, 1) = 1 and fma(1, x, 0) = x).
+        auto rows = [&](auto all_same) {
+            constexpr bool kAll = decltype(all_same)::value;
+            const bool sm = kAll || same;
+            for (int j = row0; j < nt; j += rstep) {
+                const double rho_d = c.R + 0.5 * c.a * (1.0 + xt[j]);
+                const double dd = rho_d - rd;
+                const T base = (T)(dd * dd);
+                const T q = (T)(2.0 * rho_d * rd);
+                const T dm = (T)dd;                 // rho - r
+                const T rf = (T)rd, rho = (T)rho_d;
+                const T base2 = (T)(dd * dd * dd * dd);
+                const T n1 = (T)(dd * dd * (2.0 * rho_d * rd - dd * dd));
+                const T n2 = (T)(-3.0 * rho_d * rd * dd * dd);
+                const T n3 = (T)(-2.0 * rho_d * rho_d * rd * rd);
+                T rBh = 0, rBz = 0, rRZ = 0, rZZ = 0, rA = 0;
+                T rRRs = 0, rRRc = 0;   // sum w cos s dL and sum w cos dL: the row's RR is rho rRRs - dm rRRc
+                T rBzs = 0;             // double without the gradient: sum w s dgM (rBz then holds sum w dgM)
+                // adjoint: end rings, sum w cos (z1 K1^3 + z2 K2^3), sum w (rho - r cos)(K1^3 + K2^3)
+                T eR = 0, eZ = 0;
+                T eA = 0;           // pair adjoint: end rings of A, sum w cos (K1 + K2)
+                T rAr = 0;          // pair adjoint: sum w cos (r - rho cos) dgM
+    #pragma unroll kStableUnroll<T, WA>
+                for (int i = 0; i < na; ++i) {
+                    const T s = st[i], w = wa[i], wc = wct[i];
+                    const T M = fma_<T>(q, s, base);
+                    const T J1 = M + z1s, J2 = M + z2s;
+                    const T K1 = prsq<P>(J1), K2 = prsq<P>(J2);
+                    // A: asinh(z1 / sqrt(M)) - asinh(z2 / sqrt(M)) = log(P1 / P2), P_k = z_k + sqrt(M + z_k^2),
+                    // = log1p(x) with x = (P1 - P2) / P2 = (b (sJ1 + sJ2) + 2 b |z|) / ((sJ1 + sJ2) P2) >= 0;
+                    // for z2 < 0, P2 = M / (sJ2 - z2). A is even in z, so |z| serves directly. x = Nx / Da with
+                    // Nx = numr * nsel, and x / (2 + x) = Nx / Ea, Ea = 2 Da + Nx (log1p_ratio).
+                    T Nx = 0, Da = 1, Ea = 1;
+                    if (WA) {
+                        const T sJa1 = J1 * K1, sJa2 = J2 * K2;
+                        const T sum = sJa1 + sJa2;
+                        const T numr = fma_<T>(bf, sum, tbz);
+                        const T nsel = kAll ? (T)1 : fma_<T>(sg1, sJa2 - z2, sg);   // same ? 1 : sJ2 - z2
+                        Da = sum * (sm ? z2 + sJa2 : M);
+                        Nx = numr * nsel;
+                        Ea = fma_<T>(Da, (T)2, Nx);
                     }
-                    constexpr bool kBzMoments = sizeof(T) == 8 && !WG;
-                    T wr = 0, rmc = 0;
-                    if constexpr (kBzMoments) {
-                        // without the gradient, B_z as dm sum w dgM + r sum w s dgM: two FMAs with tabulated weights
-                        if (WB) {
-                            rBz = fma_<T>(w, dgM, rBz);
-                            rBzs = fma_<T>((T)wsd[i], dgM, rBzs);
+                    T Re = 0;   // 1 / Ea
+                    if (WB || WG) {
+                        const T K12 = K1 * K2;
+                        const T tP2 = tbz * (K12 * K12);
+                        const T g2 = z2 * K2;
+                        const T gs = fma_<T>(z1, K1, g2);                   // z1 K1 + z2 K2
+                        const T gsel = kAll ? gs : fma_<T>(sg, gs, sg1);    // same ? gs : 1
+                        const T Y = M * gsel;
+                        const T Ks = K1 + K2;
+                        const T KsY = Ks * Y;
+                        // one reciprocal for 1 / (K1 + K2), 1 / Y and, with A, 1 / Ea
+                        const T R = prcp<P>(WA ? KsY * Ea : KsY);
+                        const T Rb = WA ? Ea * R : R;
+                        if (WA) Re = KsY * R;
+                        const T S = Y * Rb, Rr = Ks * Rb;
+                        const T dK = -tP2 * S;                              // K1 - K2
+                        const T invM = gsel * Rr;
+                        const T dgM = (sm ? tP2 * M : fma_<T>(z1, K1, -g2)) * Rr;   // (z1 K1 - z2 K2) / M
+                        rBh = fma_<T>(wc, -dK, rBh);       // B_rho, also the gradient's rho-phi term
+                        if constexpr (ADA) {
+                            eA = fma_<T>(wc, Ks, eA);
+                            rAr = fma_<T>(wc * fma_<T>(rho, s, -dm), dgM, rAr);   // r - rho cos = rho s - (rho - r)
                         }
-                    } else {
-                        rmc = fma_<T>(rf, s, dm);                           // rho - r cos
-                        wr = w * rmc;                                       // shared by B_z and dB_z/dz
-                        if (WB) rBz = fma_<T>(wr, dgM, rBz);
-                    }
-                    if (WG) {
-                        const T K1s = K1 * K1;
-                        // K1^3 - K2^3 = (K1 - K2)(K1^2 + K1 K2 + K2^2), the second factor as (K1 + K2)^2 - K1 K2
-                        // (the subtraction removes at most a quarter: no cancellation)
-                        const T dL = dK * fma_<T>(Ks, Ks, -K12);
-                        // r - rho cos = rho s - dm: two moments, each one FMA with a tabulated weight; their
-                        // combination per row (in double) is bounded by the same sum of magnitudes as the per-node
-                        // terms
-                        rRRs = fma_<T>(wcst[i], dL, rRRs);
-                        rRRc = fma_<T>(wc, dL, rRRc);
-                        rZZ = fma_<T>(wr, dL, rZZ);
-                        // hM = (z1 K1^3 - z2 K2^3) / M = dgM K1^2 + z2 K2 (K1^2 - K2^2) / M, K1^2 - K2^2 = -tP2:
-                        // with z >= 0 both terms are positive when z2 < 0, and dgM carries no cancellation when
-                        // z2 > 0
-                        const T hM = fma_<T>(dgM, K1s, -(tP2 * g2) * invM);
-                        // cos (rho^2 + r^2) - 2 rho r = cos base - q s = (1 + cos) base - M
-                        const T O = fma_<T>(opt[i], base, -M);
-                        const T N = fma_<T>(s, fma_<T>(s, fma_<T>(s, n3, n2), n1), base2);
-                        // the rho-z term (O (z1 K1 - z2 K2) + N (z1 K1^3 - z2 K2^3)) / M^2
-                        const T u = fma_<T>(O, dgM, N * hM);
-                        rRZ = fma_<T>(w, u * invM, rRZ);
-                        if constexpr (AD) {
-                            const T K13 = K1s * K1, K23 = K2 * K2 * K2;
-                            eR = fma_<T>(wc, fma_<T>(z1, K13, z2 * K23), eR);
-                            eZ = fma_<T>(w, rmc * (K13 + K23), eZ);
+                        constexpr bool kBzMoments = sizeof(T) == 8 && !WG;
+                        T wr = 0, rmc = 0;
+                        if constexpr (kBzMoments) {
+                            // without the gradient, B_z as dm sum w dgM + r sum w s dgM: two FMAs with tabulated
+                            // weights
+                            if (WB) {
+                                rBz = fma_<T>(w, dgM, rBz);
+                                rBzs = fma_<T>((T)wsd[i], dgM, rBzs);
+                            }
+                        } else {
+                            rmc = fma_<T>(rf, s, dm);                           // rho - r cos
+                            wr = w * rmc;                                       // shared by B_z and dB_z/dz
+                            if (WB) rBz = fma_<T>(wr, dgM, rBz);
                         }
+                        if (WG) {
+                            const T K1s = K1 * K1;
+                            // K1^3 - K2^3 = (K1 - K2)(K1^2 + K1 K2 + K2^2), the second factor as (K1 + K2)^2 - K1 K2
+                            // (the subtraction removes at most a quarter: no cancellation)
+                            const T dL = dK * fma_<T>(Ks, Ks, -K12);
+                            // r - rho cos = rho s - dm: two moments, each one FMA with a tabulated weight; their
+                            // combination per row (in double) is bounded by the same sum of magnitudes as the per-node
+                            // terms
+                            rRRs = fma_<T>(wcst[i], dL, rRRs);
+                            rRRc = fma_<T>(wc, dL, rRRc);
+                            rZZ = fma_<T>(wr, dL, rZZ);
+                            // hM = (z1 K1^3 - z2 K2^3) / M = dgM K1^2 + z2 K2 (K1^2 - K2^2) / M, K1^2 - K2^2 = -tP2:
+                            // with z >= 0 both terms are positive when z2 < 0, and dgM carries no cancellation when
+                            // z2 > 0
+                            const T hM = fma_<T>(dgM, K1s, -(tP2 * g2) * invM);
+                            // cos (rho^2 + r^2) - 2 rho r = cos base - q s = (1 + cos) base - M
+                            const T O = fma_<T>(opt[i], base, -M);
+                            const T N = fma_<T>(s, fma_<T>(s, fma_<T>(s, n3, n2), n1), base2);
+                            // the rho-z term (O (z1 K1 - z2 K2) + N (z1 K1^3 - z2 K2^3)) / M^2
+                            const T u = fma_<T>(O, dgM, N * hM);
+                            rRZ = fma_<T>(w, u * invM, rRZ);
+                            if constexpr (AD) {
+                                const T K13 = K1s * K1, K23 = K2 * K2 * K2;
+                                eR = fma_<T>(wc, fma_<T>(z1, K13, z2 * K23), eR);
+                                eZ = fma_<T>(w, rmc * (K13 + K23), eZ);
+                            }
+                        }
+                    } else if (WA) {
+                        Re = prcp<P>(Ea);
                     }
-                } else if (WA) {
-                    Ra = prcp<P>(Da);
+                    if (WA) rA = fma_<T>(wc, log1p_ratio<P>(Nx, Da, Nx * Re), rA);
                 }
-                if (WA) rA = fma_<T>(wc, log1p_pos<P>(numr * nsel * Ra), rA);
+                const double rowRR = WG ? fma(rho_d, (double)rRRs, -dd * (double)rRRc) : 0.0;
+                const double rowBz = (sizeof(T) == 8 && !WG) ? fma(rd, (double)rBzs, dd * (double)rBz) : (double)rBz;
+                const double W = c.cf * wt[j] * rho_d;
+                aBh += W * (double)rBh;
+                aBz += W * rowBz;
+                aRR += W * rowRR;
+                aZZ += W * (double)rZZ;
+                aRZ += W * (double)rRZ;
+                aA += W * (double)rA;
+                if constexpr (ADA) {
+                    const double rowA = W * (double)rA, dAdb = W * 0.5 * (double)eA - rowA / c.b;
+                    const double dAdr = -W * (double)rAr;
+                    const double dAdrho = -(rd * dAdr - zm * W * (double)rBh + c.b * dAdb) / rho_d;
+                    qR += dAdrho;
+                    qa += 0.5 * (1.0 + xt[j]) * dAdrho;
+                    qb += dAdb;
+                    qAr += dAdr;
+                }
+                if constexpr (AD) {
+                    const double Wr = c.cf * wt[j], hx = 0.5 * (1.0 + xt[j]);
+                    const double e0 = 0.5 * (double)eR, e1 = 0.5 * (double)eZ;
+                    const double g0 = rd * rowRR + zm * (double)rRZ + c.b * e0;         // -d f_rho / d rho
+                    const double g1 = rd * (double)rRZ + zm * (double)rZZ + c.b * e1;   // -d f_z / d rho
+                    hR0 += Wr * g0;
+                    hR1 += Wr * g1;
+                    ha0 += Wr * hx * g0;
+                    ha1 += Wr * hx * g1;
+                    hb0 += W * e0;
+                    hb1 += W * e1;
+                }
             }
-            const double rowRR = WG ? fma(rho_d, (double)rRRs, -dd * (double)rRRc) : 0.0;
-            const double rowBz = (sizeof(T) == 8 && !WG) ? fma(rd, (double)rBzs, dd * (double)rBz) : (double)rBz;
-            const double W = c.cf * wt[j] * rho_d;
-            aBh += W * (double)rBh;
-            aBz += W * rowBz;
-            aRR += W * rowRR;
-            aZZ += W * (double)rZZ;
-            aRZ += W * (double)rRZ;
-            aA += W * (double)rA;
-            if constexpr (ADA) {
-                const double rowA = W * (double)rA, dAdb = W * 0.5 * (double)eA - rowA / c.b;
-                const double dAdr = -W * (double)rAr;
-                const double dAdrho = -(rd * dAdr - zm * W * (double)rBh + c.b * dAdb) / rho_d;
-                qR += dAdrho;
-                qa += 0.5 * (1.0 + xt[j]) * dAdrho;
-                qb += dAdb;
-                qAr += dAdr;
-            }
-            if constexpr (AD) {
-                const double Wr = c.cf * wt[j], hx = 0.5 * (1.0 + xt[j]);
-                const double e0 = 0.5 * (double)eR, e1 = 0.5 * (double)eZ;
-                const double g0 = rd * rowRR + zm * (double)rRZ + c.b * e0;         // -d f_rho / d rho
-                const double g1 = rd * (double)rRZ + zm * (double)rZZ + c.b * e1;   // -d f_z / d rho
-                hR0 += Wr * g0;
-                hR1 += Wr * g1;
-                ha0 += Wr * hx * g0;
-                ha1 += Wr * hx * g1;
-                hb0 += W * e0;
-                hb1 += W * e1;
-            }
-        }
+        };
+        if (__ballot(same) == __ballot(true)) rows(std::true_type{});
+        else rows(std::false_type{});
         const double sgn = neg ? -1.0 : 1.0;   // odd terms at z < 0
         if constexpr (ADA) {   // A is even in z
             ada->dR = qR;
