@@ -248,13 +248,14 @@ template <typename P> __device__ inline real_t<P> log1p_pos(real_t<P> x)
 
 // log(1 + x) for x = N / D >= 0, given v = x / (2 + x) = N / (2 D + N) from a reciprocal the caller shares with other
 // terms: 2 atanh(v) = v atanh2_poly(v^2) while v <= 3 - 2 sqrt 2 (x <= sqrt 2 - 1: log1p_pos's e = 0 case, without its
-// range reduction and its own reciprocal); lanes beyond that (nodes close to the winding) take log1p_pos(N / D). A
-// lane's value depends only on its own arguments.
-template <typename P> __device__ inline real_t<P> log1p_ratio(real_t<P> N, real_t<P> D, real_t<P> v)
+// range reduction and its own reciprocal); lanes beyond that (nodes close to the winding) take log1p_pos(N / D), with
+// D = den() computed only there. A lane's value depends only on its own arguments.
+constexpr double kLog1pRatioMax = 0.17157287525380990;   // 3 - 2 sqrt 2
+template <typename P, typename F> __device__ inline real_t<P> log1p_ratio(real_t<P> N, real_t<P> v, F den)
 {
     using T = real_t<P>;
     T r = v * atanh2_poly<T>(v * v);
-    if (v > (T)0.17157287525380990) r = log1p_pos<P>(N * prcp<P>(D));
+    if (v > (T)kLog1pRatioMax) r = log1p_pos<P>(N * prcp<P>(den()));
     return r;
 }
 
@@ -303,7 +304,7 @@ struct AdjA {
 // measured 2 % faster with 2 and 12 % slower with 8).
 template <typename T, bool WA> constexpr int kStableUnroll = (sizeof(T) == 4 && !WA) ? 2 : 4;
 
-template <typename P, bool WA, bool WB, bool WG, bool AD = false, bool ADA = false>
+template <typename P, bool WA, bool WB, bool WG, bool AD = false, bool ADA = false, bool LEAN = false>
 __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double zd, double rd, int row0, int rstep,
                                    Local &L, AdjLocal *ad = nullptr, AdjA *ada = nullptr)
 {
@@ -332,7 +333,7 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
         const double z1d = zm + 0.5 * c.b, z2d = zm - 0.5 * c.b;
         const T z1 = (T)z1d, z2 = (T)z2d;
         const T z1s = (T)(z1d * z1d), z2s = (T)(z2d * z2d);
-        const T tbz = (T)(2.0 * c.b * zm);
+        const T tbz = (T)(2.0 * c.b * zm), zs = (T)(2.0 * zm);
         const bool same = z2d > 0.0;
         // same ? x : 1 as one exact FMA, fma(sg, x, sg1): a 64-bit select takes two instructions
         const T sg = same ? (T)1 : (T)0, sg1 = (T)1 - sg;
@@ -341,6 +342,10 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
         // for any mix; the two instances give bitwise the same values (fma(0, x, 1) = 1 and fma(1, x, 0) = x).
         auto rows = [&](auto all_same) {
             constexpr bool kAll = decltype(all_same)::value;
+            // LEAN (the pair kernels, whose waves are fixed by the pair's schedule): with every lane beyond the span
+            // and no gradient terms, the node loop below takes fewer operations; its values may differ from the
+            // general loop's in the last bits, so a lane's value depends on its wave there.
+            constexpr bool kLean = LEAN && kAll && !WG && !AD && !ADA;
             const bool sm = kAll || same;
             for (int j = row0; j < nt; j += rstep) {
                 const double rho_d = c.R + 0.5 * c.a * (1.0 + xt[j]);
@@ -360,12 +365,47 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                 T eR = 0, eZ = 0;
                 T eA = 0;           // pair adjoint: end rings of A, sum w cos (K1 + K2)
                 T rAr = 0;          // pair adjoint: sum w cos (r - rho cos) dgM
-    #pragma unroll kStableUnroll<T, WA>
+#pragma unroll kStableUnroll<T, WA>
                 for (int i = 0; i < na; ++i) {
                     const T s = st[i], w = wa[i], wc = wct[i];
                     const T M = fma_<T>(q, s, base);
                     const T J1 = M + z1s, J2 = M + z2s;
                     const T K1 = prsq<P>(J1), K2 = prsq<P>(J2);
+                    if constexpr (kLean) {
+                        // z2 > 0: with gs = z1 K1 + z2 K2, (z1 K1 - z2 K2) / M = tbz K1^2 K2^2 / gs and
+                        // K2 - K1 = tbz K1^2 K2^2 / (K1 + K2), so the B sums are taken without their common factor
+                        // tbz (applied per row) and one reciprocal gives 1 / (K1 + K2), 1 / gs and 1 / Ea, where
+                        // Ea = 2 Da + numr = sum (2 sJ2 + 2 |z|) + tbz (all terms positive).
+                        const T K12 = K1 * K2, K12s = K12 * K12;
+                        T sJa2 = 0, sum = 0, numr = 0, Re = 0;
+                        if (WA) {
+                            sJa2 = J2 * K2;
+                            sum = fma_<T>(J1, K1, sJa2);
+                            numr = fma_<T>(bf, sum, tbz);
+                        }
+                        const T Ea = WA ? fma_<T>(sum, fma_<T>((T)2, sJa2, zs), tbz) : (T)1;
+                        if (WB) {
+                            const T Ks = K1 + K2;
+                            const T gs = fma_<T>(z1, K1, z2 * K2);
+                            const T KG = Ks * gs;
+                            const T R = prcp<P>(WA ? KG * Ea : KG);
+                            const T Rb = WA ? Ea * R : R;                       // 1 / (Ks gs)
+                            if (WA) Re = KG * R;
+                            rBh = fma_<T>(wc, K12s * (gs * Rb), rBh);          // B_rho / tbz
+                            const T dg = K12s * (Ks * Rb);                      // (z1 K1 - z2 K2) / (M tbz)
+                            if constexpr (sizeof(T) == 8) {
+                                rBz = fma_<T>(w, dg, rBz);
+                                rBzs = fma_<T>((T)wsd[i], dg, rBzs);
+                            } else {
+                                rBz = fma_<T>(w * fma_<T>(rf, s, dm), dg, rBz);
+                            }
+                        } else if (WA) {
+                            Re = prcp<P>(Ea);
+                        }
+                        if (WA)
+                            rA = fma_<T>(wc, log1p_ratio<P>(numr, numr * Re, [&] { return sum * (z2 + sJa2); }), rA);
+                        continue;
+                    }
                     // A: asinh(z1 / sqrt(M)) - asinh(z2 / sqrt(M)) = log(P1 / P2), P_k = z_k + sqrt(M + z_k^2),
                     // = log1p(x) with x = (P1 - P2) / P2 = (b (sJ1 + sJ2) + 2 b |z|) / ((sJ1 + sJ2) P2) >= 0;
                     // for z2 < 0, P2 = M / (sJ2 - z2). A is even in z, so |z| serves directly. x = Nx / Da with
@@ -447,13 +487,14 @@ __device__ inline void eval_stable(const CoilDev &c, const Tables &tab, double z
                     } else if (WA) {
                         Re = prcp<P>(Ea);
                     }
-                    if (WA) rA = fma_<T>(wc, log1p_ratio<P>(Nx, Da, Nx * Re), rA);
+                    if (WA) rA = fma_<T>(wc, log1p_ratio<P>(Nx, Nx * Re, [&] { return Da; }), rA);
                 }
                 const double rowRR = WG ? fma(rho_d, (double)rRRs, -dd * (double)rRRc) : 0.0;
                 const double rowBz = (sizeof(T) == 8 && !WG) ? fma(rd, (double)rBzs, dd * (double)rBz) : (double)rBz;
                 const double W = c.cf * wt[j] * rho_d;
-                aBh += W * (double)rBh;
-                aBz += W * rowBz;
+                const double Wb = kLean ? W * (2.0 * c.b * zm) : W;   // the lean loop's B sums lack tbz
+                aBh += Wb * (double)rBh;
+                aBz += Wb * rowBz;
                 aRR += W * rowRR;
                 aZZ += W * (double)rZZ;
                 aRZ += W * (double)rRZ;

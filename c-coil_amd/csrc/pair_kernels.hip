@@ -37,7 +37,7 @@ __device__ inline void eval_prim(const CoilDev &pc, const Tables &tab, double z,
         return;
     }
     if constexpr (PREC == 0) {
-        eval_stable<f64s, WM, WF, false>(pc, tab, z, r, row0, rstep, L);
+        eval_stable<f64s, WM, WF, false, false, false, true>(pc, tab, z, r, row0, rstep, L);
     } else if constexpr (PREC == 4) {
         eval_stable<double, WM, WF, false>(pc, tab, z, r, row0, rstep, L);
     } else if constexpr (PREC == 2) {
@@ -277,7 +277,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
                             accumulate_grad(n, L, q);
                         } else {
                             AdjA q;
-                            eval_stable<float, WM || WD, WF || WD, false, false, WD>(pc, p.tab, n.z, n.r, rg,
+                            eval_stable<float, WM || WD, WF || WD, false, false, WD, true>(pc, p.tab, n.z, n.r, rg,
                                                                                      pd.row_groups, L, nullptr, &q);
                             accumulate(n, L, q);
                         }
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
                             accumulate_grad(n, L, q);
                         } else {
                             AdjA q;
-                            eval_stable<f64s, WM || WD, WF || WD, false, false, WD>(pc, p.tab, n.z, n.r, rg,
+                            eval_stable<f64s, WM || WD, WF || WD, false, false, WD, true>(pc, p.tab, n.z, n.r, rg,
                                                                                     pd.row_groups, L, nullptr, &q);
                             accumulate(n, L, q);
                         }
@@ -320,8 +320,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
                     accumulate_grad(n, L, q);
                 } else {
                     AdjA q;
-                    eval_stable<f64s, WM || WD, WF || WD, false, false, WD>(pc, p.tab, n.z, n.r, rg, pd.row_groups,
-                                                                            L, nullptr, &q);
+                    eval_stable<f64s, WM || WD, WF || WD, false, false, WD, true>(pc, p.tab, n.z, n.r, rg,
+                                                                                  pd.row_groups, L, nullptr, &q);
                     accumulate(n, L, q);
                 }
             }
