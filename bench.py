@@ -27,9 +27,11 @@ Reported beside it in the same line:
                duration on the library's stream, against the MI355X vector peak of each part's type;
   adjoint      the coil-design workload (64 THIN/RECTANGULAR coils near the axis, 4,096 points in a small
                cylinder): forward B (ccoil_fields) and the adjoint of a field loss (ccoil_field_gradient, all 9
-               parameters of every coil), FP64 and FP32C, and the adjoint / forward time ratio;
+               parameters of every coil), FP64 and FP32C, the adjoint / forward time ratio, and the forward's
+               per-call wall time eager and replayed from a captured plan (HIP graph);
   coaxial      4,096 coaxial THIN/RECTANGULAR pairs through the z-axis fast rules (ccoil_pairs_zaxis, M + Fz);
-  io           small-call cost through the c_coil module: Vector3Array API vs numpy overload vs DLPack device tensors;
+  io           small-call cost through the c_coil module: Vector3Array API vs numpy overload vs DLPack device tensors,
+               and through the C ABI on device pointers, eager and replayed from a captured plan;
   cpu_baseline the FP64 CPU restatement of the reference's CPU_MT field path (-O3 -mavx2 -ffast-math, point chunks on
                a thread pool), timed on a bounded sample of the same workload on this host.
 With --large (not part of the default run): B+gradient of a 1,024-coil group at 2^22 points, and the full
@@ -366,6 +368,8 @@ def main():
             if name in ("fp64", "fp32c_direct"):
                 pres[name]["node_pairs_per_s"] = pnodes / (prep.main_kernel_ms * 1e-3)
                 pres[name]["tflops_algorithmic"] = pf / (prep.main_kernel_ms * 1e-3) / 1e12
+                peak = RM.PEAK_FP64_TFLOPS if prec == abi.FP64 else RM.PEAK_FP32_TFLOPS
+                pres[name]["frac"] = pres[name]["tflops_algorithmic"] / peak
             log(f"pairs {name}: {dt_p * 1e3:.1f} ms/step, kernel {prep.main_kernel_ms:.1f} ms")
         o64, o32 = pres["fp64"].pop("out"), pres["fp32c"].pop("out")
         ofar, o32d = pres["fp64_far_field"].pop("out"), pres["fp32c_direct"].pop("out")
@@ -409,6 +413,19 @@ def main():
             grads[name] = agrad.cpu().numpy().copy()
             adj[name] = {"forward_ms": float(np.mean(fw)), "adjoint_ms": float(np.mean(bw)),
                          "adjoint_over_forward": float(np.mean(bw) / np.mean(fw))}
+            # per-call wall time of the forward: eager library calls, and replays of a captured plan (HIP graph)
+            plan = ctx.fields_plan(acs, ad_pts, abi.B, prec, {"B": aB})
+            for key, fn in (("forward_eager_call_ms",
+                             lambda: ctx.fields(acs, ad_pts, abi.B, prec, abi.DEVICE_POINTERS, out={"B": aB})),
+                            ("forward_plan_replay_ms", plan.launch)):
+                fn()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(50):
+                    fn()
+                torch.cuda.synchronize()
+                adj[name][key] = (time.perf_counter() - t0) / 50 * 1e3
+            plan.close()
         # FP32C adjoint against FP64, per parameter kind, relative to that parameter's largest |gradient| over coils
         g64, g32 = grads["fp64"], grads["fp32c"]
         scale = np.max(np.abs(g64), axis=0)
@@ -462,6 +479,13 @@ def main():
               "vector3array_ms": per_call(v3),
               "numpy_ms": per_call(lambda: icoil.compute_all_B_field_vectors(ipts, C.GPU)),
               "dlpack_device_ms": per_call(lambda: icoil.compute_all_fields_into(dpts, B=dB))}
+        # the same call through the C ABI on device pointers: eager, and replayed from a captured plan (HIP graph)
+        ics = abi.CoilSet.from_rows([W.coil_row(icoil)])
+        io["abi_eager_ms"] = per_call(lambda: ctx.fields(ics, dpts, abi.B, abi.FP64, abi.DEVICE_POINTERS,
+                                                         out={"B": dB}))
+        iplan = ctx.fields_plan(ics, dpts, abi.B, abi.FP64, {"B": dB})
+        io["plan_replay_ms"] = per_call(iplan.launch)
+        iplan.close()
         C.Coil.set_gpu_precision(C.GPU_FP32C)
         result["io"] = io
         log(f"io: {json.dumps(io)}")
