@@ -50,8 +50,13 @@ __global__ __launch_bounds__(kGradBlock) void field_grad_kernel(GradParams p)
         } else if constexpr (PREC == kFP64IEEE) {
             eval_stable<double, false, true, true, true>(c, p.tab, z, r, 0, 1, L, &ad);
         } else {
-            if (is_near(c, z, r, true)) eval_stable<f64s, false, true, true, true>(c, p.tab, z, r, 0, 1, L, &ad);
-            else eval_stable<float, false, true, true, true>(c, p.tab, z, r, 0, 1, L, &ad);
+            // a wave holding a near-winding or near-axis pair evaluates all its lanes in FP64: a float pass beside
+            // the FP64 one would add its own issue time. A pair's value thus depends on its wave, which the
+            // point order and the grid fix, as they fix the order of the sums over points.
+            if (__ballot(is_near(c, z, r, true)))
+                eval_stable<f64s, false, true, true, true>(c, p.tab, z, r, 0, 1, L, &ad);
+            else
+                eval_stable<float, false, true, true, true>(c, p.tab, z, r, 0, 1, L, &ad);
         }
         // local Cartesian B, gradient (the same on-axis rule as the forward path) and position
         const double Bl[3] = {L.Bh * cph, L.Bh * sph, L.Bz};

@@ -16,6 +16,10 @@
 
 namespace ccoil {
 
+// Calls with fewer points than kNearSplitPoints split each near record's thickness rows over kNearSplit lanes.
+constexpr int64_t kNearSplitPoints = (int64_t)1 << 17;
+constexpr int kNearSplit = 8;
+
 // Per coil key, the near points (points with near pairs only; the coil loop is wave-uniform, so each coil's count
 // in a wave is one ballot and one atomic).
 __global__ __launch_bounds__(256) void near_count_kernel(FieldParams p)
@@ -112,52 +116,82 @@ __global__ __launch_bounds__(256) void near_fill_kernel(FieldParams p)
     if (bad || slot != end) atomicOr(&p.near_state->mismatch, 1u);
 }
 
-// Records sorted by coil: lane e takes the e-th record; its value goes to the record's slot.
-// FLOAT: the kind-1 records [fp64, total) by the float rule, at 5 waves per SIMD as field_kernel's float instances;
-// else the kind-0 records [0, fp64) by the FP64 rule.
-template <bool WA, bool WB, bool WG, bool FLOAT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FLOAT ? 5 : 1))) void near_eval_kernel(FieldParams p)
+// Records sorted by coil: S consecutive lanes take one record (thickness rows sub, sub + S, ..., combined by a
+// fixed xor butterfly, so every lane of the group holds the same sum); its value goes to the record's slot. S > 1 on
+// calls with few points, where the records are too few to fill the chip one lane each and one lane's serial chain
+// of nodes sets the kernel's duration; the lanes then stride over the records (a group's value does not depend on
+// the grid). FLOAT: the kind-1 records [fp64, total) by the float rule, at 5 waves per SIMD as field_kernel's float
+// instances; else the kind-0 records [0, fp64) by the FP64 rule.
+template <bool WA, bool WB, bool WG, bool FLOAT, int S>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FLOAT ? 5 : 1)))
+void near_eval_kernel(FieldParams p)
 {
     if (p.near_state->overflow) return;
     const int64_t total = (int64_t)p.near_state->total, fp64 = (int64_t)p.near_state->fp64;
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x + (FLOAT ? fp64 : 0);
-    if (e >= (FLOAT ? total : fp64)) return;
-    const uint32_t key = p.near_keys[e];
-    const uint64_t rec = p.near_recs[e];
-    const int64_t pi = (int64_t)(rec >> 32), slot = (int64_t)(rec & 0xffffffffu);
-    // only a near_fill that reported a mismatch leaves records unwritten; never follow one out of bounds
-    if (pi >= p.n_pts || slot >= total) return;
-    double *o = p.near_vals + slot * p.near_width;
-    // A wave holds one coil's records, or two or more at the boundaries of the sorted keys: it takes its keys one
-    // at a time (the key of the lowest lane still to do, read into a scalar register), and every lane evaluates
-    // that coil at its point, keeping the value only if the key is its own. The control flow stays uniform and no
-    // store precedes the loads, so the coil's fields and the quadrature tables are scalar loads, as in
-    // field_kernel; a divergent branch would take the same issue time. todo is wave-uniform and loses at least
-    // one lane on every pass.
-    const double px = p.pts[3 * pi], py = p.pts[3 * pi + 1], pz = p.pts[3 * pi + 2];
-    Cart acc;
-    for (int j = 0; j < 3; ++j) { acc.A[j] = 0.0; acc.E[j] = 0.0; acc.B[j] = 0.0; }
-    for (int j = 0; j < 9; ++j) acc.G[j] = 0.0;
-    uint64_t todo = __ballot(1);
-    while (todo) {
-        const uint32_t k = __builtin_amdgcn_readfirstlane(__shfl(key, __builtin_ctzll(todo)));
-        const bool mine = key == k;
-        todo &= ~(uint64_t)__ballot(mine);
-        // only a reported mismatch can leave a key outside the range's kind
-        const uint32_t kind = k / (uint32_t)p.n_coils;
-        if (k >= (uint32_t)p.n_keys || kind != (FLOAT ? 1u : 0u)) continue;
-        const CoilDev &c = p.coils[p.near_coil[k - kind * (uint32_t)p.n_coils]];
-        double z, r, cph, sph;
-        to_local(c, px, py, pz, z, r, cph, sph);
-        Local L;
-        if (FLOAT) eval_stable<float, WA, WB, WG>(c, p.tab, z, r, 0, 1, L);
-        else eval_stable<f64s, WA, WB, WG>(c, p.tab, z, r, 0, 1, L);
-        if (mine) accumulate_global(c, L, r, cph, sph, p.q, acc);
+    const int64_t lo = FLOAT ? fp64 : 0, hi = FLOAT ? total : fp64;
+    const int sub = (int)(threadIdx.x % S);
+    auto record = [&](int64_t e) {
+        const uint32_t key = p.near_keys[e];
+        const uint64_t rec = p.near_recs[e];
+        const int64_t pi = (int64_t)(rec >> 32), slot = (int64_t)(rec & 0xffffffffu);
+        // only a near_fill that reported a mismatch leaves records unwritten; never follow one out of bounds
+        if (pi >= p.n_pts || slot >= total) return;
+        double *o = p.near_vals + slot * p.near_width;
+        // A wave holds one coil's records, or two or more at the boundaries of the sorted keys: it takes its keys
+        // one at a time (the key of the lowest lane still to do, read into a scalar register), and every lane
+        // evaluates that coil at its point, keeping the value only if the key is its own. The control flow stays
+        // uniform and no store precedes the loads, so the coil's fields and the quadrature tables are scalar loads,
+        // as in field_kernel; a divergent branch would take the same issue time. todo is wave-uniform and loses at
+        // least one lane on every pass; a group's lanes share their key, so a group is always active as a whole.
+        const double px = p.pts[3 * pi], py = p.pts[3 * pi + 1], pz = p.pts[3 * pi + 2];
+        Cart acc;
+        for (int j = 0; j < 3; ++j) { acc.A[j] = 0.0; acc.E[j] = 0.0; acc.B[j] = 0.0; }
+        for (int j = 0; j < 9; ++j) acc.G[j] = 0.0;
+        uint64_t todo = __ballot(1);
+        while (todo) {
+            const uint32_t k = __builtin_amdgcn_readfirstlane(__shfl(key, __builtin_ctzll(todo)));
+            const bool mine = key == k;
+            todo &= ~(uint64_t)__ballot(mine);
+            // only a reported mismatch can leave a key outside the range's kind
+            const uint32_t kind = k / (uint32_t)p.n_coils;
+            if (k >= (uint32_t)p.n_keys || kind != (FLOAT ? 1u : 0u)) continue;
+            const CoilDev &c = p.coils[p.near_coil[k - kind * (uint32_t)p.n_coils]];
+            double z, r, cph, sph;
+            to_local(c, px, py, pz, z, r, cph, sph);
+            Local L;
+            if (FLOAT) eval_stable<float, WA, WB, WG>(c, p.tab, z, r, sub, S, L);
+            else eval_stable<f64s, WA, WB, WG>(c, p.tab, z, r, sub, S, L);
+            if constexpr (S > 1) {
+#pragma unroll
+                for (int m = 1; m < S; m <<= 1) {
+                    L.A += __shfl_xor(L.A, m, 64);
+                    L.Bh += __shfl_xor(L.Bh, m, 64);
+                    L.Bz += __shfl_xor(L.Bz, m, 64);
+                    L.RP += __shfl_xor(L.RP, m, 64);
+                    L.RR += __shfl_xor(L.RR, m, 64);
+                    L.RZ += __shfl_xor(L.RZ, m, 64);
+                    L.ZZ += __shfl_xor(L.ZZ, m, 64);
+                }
+            }
+            if (mine) accumulate_global(c, L, r, cph, sph, p.q, acc);
+        }
+        if (sub == 0) {
+            if (p.A) { for (int j = 0; j < 3; ++j) o[j] = acc.A[j]; o += 3; }
+            if (p.E) { for (int j = 0; j < 3; ++j) o[j] = acc.E[j]; o += 3; }
+            if (p.B) { for (int j = 0; j < 3; ++j) o[j] = acc.B[j]; o += 3; }
+            if (p.G) for (int j = 0; j < 9; ++j) o[j] = acc.G[j];
+        }
+    };
+    if constexpr (S == 1) {
+        const int64_t e = lo + (int64_t)blockIdx.x * 256 + threadIdx.x;
+        if (e < hi) record(e);
+    } else {
+        // a wave's groups hold consecutive records, so the loop's exit is wave-uniform once no lane has a record
+        // left, and a group is active or idle as a whole
+        const int64_t stride = (int64_t)gridDim.x * (256 / S);
+        for (int64_t e = lo + ((int64_t)blockIdx.x * 256 + threadIdx.x) / S; __ballot(e < hi); e += stride)
+            if (e < hi) record(e);
     }
-    if (p.A) { for (int j = 0; j < 3; ++j) o[j] = acc.A[j]; o += 3; }
-    if (p.E) { for (int j = 0; j < 3; ++j) o[j] = acc.E[j]; o += 3; }
-    if (p.B) { for (int j = 0; j < 3; ++j) o[j] = acc.B[j]; o += 3; }
-    if (p.G) for (int j = 0; j < 9; ++j) o[j] = acc.G[j];
 }
 
 __global__ __launch_bounds__(256) void near_add_kernel(FieldParams p)
@@ -290,29 +324,40 @@ void launch_near_fill(hipStream_t s, const FieldParams &p)
     hipLaunchKernelGGL(near_fill_kernel, dim3((unsigned)((p.n_pts + 255) / 256)), dim3(256), 0, s, p);
 }
 
-template <bool FLOAT>
+template <bool FLOAT, int S>
 static void launch_near_eval_k(hipStream_t s, const FieldParams &p)
 {
-    const unsigned blocks = (unsigned)((p.near_cap + 255) / 256);
+    // S = 1: one lane per record of the capacity; S > 1: a strided grid of at most 2^21 lanes
+    const int64_t cap_lanes = p.near_cap * S, max_lanes = (int64_t)1 << 21;
+    const int64_t lanes = (S == 1 || cap_lanes < max_lanes) ? cap_lanes : max_lanes;
+    const unsigned blocks = (unsigned)((lanes + 255) / 256);
     const bool wa = p.q & (Q_A | Q_E), wb = p.q & Q_B, wg = p.q & (Q_G | Q_GJ);
     const int code = (wa ? 4 : 0) | (wb ? 2 : 0) | (wg ? 1 : 0);
+    const dim3 g(blocks), b(256);
     switch (code) {
-        case 1: hipLaunchKernelGGL((near_eval_kernel<false, false, true, FLOAT>), dim3(blocks), dim3(256), 0, s, p); break;
-        case 2: hipLaunchKernelGGL((near_eval_kernel<false, true, false, FLOAT>), dim3(blocks), dim3(256), 0, s, p); break;
-        case 3: hipLaunchKernelGGL((near_eval_kernel<false, true, true, FLOAT>), dim3(blocks), dim3(256), 0, s, p); break;
-        case 4: hipLaunchKernelGGL((near_eval_kernel<true, false, false, FLOAT>), dim3(blocks), dim3(256), 0, s, p); break;
-        case 5: hipLaunchKernelGGL((near_eval_kernel<true, false, true, FLOAT>), dim3(blocks), dim3(256), 0, s, p); break;
-        case 6: hipLaunchKernelGGL((near_eval_kernel<true, true, false, FLOAT>), dim3(blocks), dim3(256), 0, s, p); break;
-        default: hipLaunchKernelGGL((near_eval_kernel<true, true, true, FLOAT>), dim3(blocks), dim3(256), 0, s, p); break;
+        case 1: hipLaunchKernelGGL((near_eval_kernel<false, false, true, FLOAT, S>), g, b, 0, s, p); break;
+        case 2: hipLaunchKernelGGL((near_eval_kernel<false, true, false, FLOAT, S>), g, b, 0, s, p); break;
+        case 3: hipLaunchKernelGGL((near_eval_kernel<false, true, true, FLOAT, S>), g, b, 0, s, p); break;
+        case 4: hipLaunchKernelGGL((near_eval_kernel<true, false, false, FLOAT, S>), g, b, 0, s, p); break;
+        case 5: hipLaunchKernelGGL((near_eval_kernel<true, false, true, FLOAT, S>), g, b, 0, s, p); break;
+        case 6: hipLaunchKernelGGL((near_eval_kernel<true, true, false, FLOAT, S>), g, b, 0, s, p); break;
+        default: hipLaunchKernelGGL((near_eval_kernel<true, true, true, FLOAT, S>), g, b, 0, s, p); break;
     }
 }
 
-// Each grid covers the capacity; threads beyond their kind's range (read on the device) return at once.
+// Each grid covers the capacity; threads beyond their kind's range (read on the device) return at once. Calls with
+// fewer than 2^17 points split each record's rows over 8 lanes (a choice made from the call's shape alone, so a
+// call and its captured plan run the same split).
 void launch_near_eval(hipStream_t s, const FieldParams &p)
 {
     if (p.near_cap <= 0) return;
-    launch_near_eval_k<false>(s, p);
-    if (p.float_records) launch_near_eval_k<true>(s, p);
+    if (p.n_pts < kNearSplitPoints) {
+        launch_near_eval_k<false, kNearSplit>(s, p);
+        if (p.float_records) launch_near_eval_k<true, kNearSplit>(s, p);
+    } else {
+        launch_near_eval_k<false, 1>(s, p);
+        if (p.float_records) launch_near_eval_k<true, 1>(s, p);
+    }
 }
 
 void launch_near_add(hipStream_t s, const FieldParams &p)
