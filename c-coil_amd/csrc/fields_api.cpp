@@ -449,7 +449,8 @@ extern "C" ccoil_status ccoil_fields(ccoil_ctx ctx, const ccoil_coils *coils, co
         shape = &tuned;
     }
     // A series-mode call too large for one record pass runs as pieces of points, each with its own pass (records
-    // grow with points x coils); a point's result does not depend on the piece that holds it
+    // grow with points x coils). Each piece is a call of its own: its launch shape follows its size, so a point in
+    // a short last piece can differ in the last bits from the same point in a longer one
     const bool far = (precision == CCOIL_FP32C && !(flags & CCOIL_DIRECT)) ||
                      (precision == CCOIL_FP64 && (flags & CCOIL_FAR_FIELD));
     const int64_t n_coils = coils ? coils->n : 0;
