@@ -86,7 +86,10 @@ struct FieldShape {
     int chunks = 1;
 };
 // The fixed rule (few points: split rows over lanes; few waves: chunk the coils; large: fill the last round).
-FieldShape field_shape_rule(ccoil_ctx ctx, int64_t n_points, int64_t n_coils, uint32_t q, int precision);
+// series_only: the call's record pass takes every pair inside the far-field distances, so the launch is the
+// series-only kernel (one lane per point).
+FieldShape field_shape_rule(ccoil_ctx ctx, int64_t n_points, int64_t n_coils, uint32_t q, int precision,
+                            bool series_only = false);
 
 // Everything a field call launches, with its buffers already sized.
 struct FieldLaunch {

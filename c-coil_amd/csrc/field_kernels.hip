@@ -139,10 +139,10 @@ void field_kernel(FieldParams p)
                 // stops adding at its own count, so its value does not depend on the other points of its wave
                 const int own = far ? multipole_terms(r * r + z * z, c.rmax,
                                                       PREC == kFP64 ? kMpLogTolFP64 : kMpLogTolFP32C) : 0;
-                const int terms = wave_max_small(own);
+                const int terms = wave_max_small(own), lo = wave_min_small(own, far);
                 if (far && lane == 0)
                     eval_multipole<WA, WB, WG>(p.mp + kMpHeader + (size_t)ci * kMpCoil, p.mp, c.rmax, z, r, terms,
-                                               own, L);
+                                               lo, own, L);
             }
         }
         if (SPLIT > 1) reduce_local<SPLIT>(L);
@@ -150,6 +150,61 @@ void field_kernel(FieldParams p)
     }
     if (!active || lane != 0) return;
     if ((PREC == kFP32C || PREC == kFP64) && n_near > 0)   // integer sums: exact in any order
+        atomicAdd((unsigned long long *)&p.near_counts[pi], (unsigned long long)n_near);
+    if (p.n_chunks > 1) {
+        double *o = p.partial + ((int64_t)blockIdx.y * p.n_pts + pi) * p.partial_width;
+        if (p.A) { for (int k = 0; k < 3; ++k) o[k] = acc.A[k]; o += 3; }
+        if (p.E) { for (int k = 0; k < 3; ++k) o[k] = acc.E[k]; o += 3; }
+        if (p.B) { for (int k = 0; k < 3; ++k) o[k] = acc.B[k]; o += 3; }
+        if (p.G) for (int k = 0; k < 9; ++k) o[k] = acc.G[k];
+        return;
+    }
+    if (p.A) for (int k = 0; k < 3; ++k) p.A[3 * pi + k] = acc.A[k];
+    if (p.E) for (int k = 0; k < 3; ++k) p.E[3 * pi + k] = acc.E[k];
+    if (p.B) for (int k = 0; k < 3; ++k) p.B[3 * pi + k] = acc.B[k];
+    if (p.G) for (int k = 0; k < 9; ++k) p.G[9 * pi + k] = acc.G[k];
+}
+
+// The series alone, for calls whose record pass takes every pair inside the coils' far-field distances (FP64 or FP32C
+// with rule records): one lane per point, the coils looped inside; a pair beyond the distance takes the series, a
+// pair inside it is counted for the record pass. Without the rule's node loops the kernel holds far fewer registers
+// than field_kernel, so more waves hide the latency of the coefficient loads. A point's value is its far coils'
+// series in coil order, the same sums as field_kernel's.
+template <int PREC, bool WA, bool WB, bool WG>
+__global__ __launch_bounds__(kFieldBlock) void field_series_kernel(FieldParams p)
+{
+    const int64_t vi = (int64_t)blockIdx.x * kFieldBlock + threadIdx.x;
+    const bool active = vi < p.n_pts;
+    const int64_t vc = active ? vi : (p.n_pts - 1);   // inactive lanes shadow a valid point
+    const int64_t pi = p.order ? (int64_t)p.order[vc] : vc;
+    const double px = p.pts[3 * pi], py = p.pts[3 * pi + 1], pz = p.pts[3 * pi + 2];
+
+    Cart acc;
+    for (int k = 0; k < 3; ++k) { acc.A[k] = 0.0; acc.E[k] = 0.0; acc.B[k] = 0.0; }
+    for (int k = 0; k < 9; ++k) acc.G[k] = 0.0;
+
+    int n_near = 0;
+    const int c0 = blockIdx.y * p.coils_per_chunk, c1 = min(c0 + p.coils_per_chunk, p.n_coils);
+    for (int ci = c0; ci < c1; ++ci) {
+        const CoilDev &c = p.coils[ci];
+        double z, r, cph, sph;
+        to_local(c, px, py, pz, z, r, cph, sph);
+        const bool far = is_far(c, z, r);
+        if (record_kind(p, c, z, r, WG) >= 0) ++n_near;
+        if (__ballot(far)) {
+            const int own = far ? multipole_terms(r * r + z * z, c.rmax,
+                                                  PREC == kFP64 ? kMpLogTolFP64 : kMpLogTolFP32C) : 0;
+            const int terms = wave_max_small(own), lo = wave_min_small(own, far);
+            if (far) {
+                Local L;
+                eval_multipole<WA, WB, WG>(p.mp + kMpHeader + (size_t)ci * kMpCoil, p.mp, c.rmax, z, r, terms, lo,
+                                           own, L);
+                accumulate_global(c, L, r, cph, sph, p.q, acc);
+            }
+        }
+    }
+    if (!active) return;
+    if (n_near > 0)   // integer sums: exact in any order
         atomicAdd((unsigned long long *)&p.near_counts[pi], (unsigned long long)n_near);
     if (p.n_chunks > 1) {
         double *o = p.partial + ((int64_t)blockIdx.y * p.n_pts + pi) * p.partial_width;
@@ -243,9 +298,37 @@ static void launch_field_q(hipStream_t s, const FieldParams &p, int split)
     else hipLaunchKernelGGL((field_kernel<PREC, WA, WB, WG, 16>), grid, dim3(kFieldBlock), 0, s, p);
 }
 
+template <int PREC, bool WA, bool WB, bool WG>
+static void launch_series_q(hipStream_t s, const FieldParams &p)
+{
+    const dim3 grid((unsigned)((p.n_pts + kFieldBlock - 1) / kFieldBlock), (unsigned)p.n_chunks);
+    hipLaunchKernelGGL((field_series_kernel<PREC, WA, WB, WG>), grid, dim3(kFieldBlock), 0, s, p);
+}
+
+template <int PREC>
+static void launch_series_p(hipStream_t s, const FieldParams &p)
+{
+    const bool wa = p.q & (Q_A | Q_E), wb = p.q & Q_B, wg = p.q & (Q_G | Q_GJ);
+    switch ((wa ? 4 : 0) | (wb ? 2 : 0) | (wg ? 1 : 0)) {
+        case 1: launch_series_q<PREC, false, false, true>(s, p); break;
+        case 2: launch_series_q<PREC, false, true, false>(s, p); break;
+        case 3: launch_series_q<PREC, false, true, true>(s, p); break;
+        case 4: launch_series_q<PREC, true, false, false>(s, p); break;
+        case 5: launch_series_q<PREC, true, false, true>(s, p); break;
+        case 6: launch_series_q<PREC, true, true, false>(s, p); break;
+        default: launch_series_q<PREC, true, true, true>(s, p); break;
+    }
+}
+
 template <int PREC>
 static void launch_field_p(hipStream_t s, const FieldParams &p, int split)
 {
+    if constexpr (PREC == kFP64 || PREC == kFP32C) {
+        if (p.rule_records) {   // the record pass takes every pair inside the far-field distances
+            launch_series_p<PREC>(s, p);
+            return;
+        }
+    }
     const bool wa = p.q & (Q_A | Q_E), wb = p.q & Q_B, wg = p.q & (Q_G | Q_GJ);
     const int code = (wa ? 4 : 0) | (wb ? 2 : 0) | (wg ? 1 : 0);
     switch (code) {
@@ -283,6 +366,35 @@ static int blocks_per_cu_p(uint32_t q, int split)
         case 6: return blocks_per_cu_q<PREC, true, true, false>(split);
         default: return blocks_per_cu_q<PREC, true, true, true>(split);
     }
+}
+
+template <int PREC, bool WA, bool WB, bool WG>
+static int series_blocks_q()
+{
+    int b = 0;
+    const hipError_t e =
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, field_series_kernel<PREC, WA, WB, WG>, kFieldBlock, 0);
+    return e == hipSuccess ? b : 0;
+}
+
+template <int PREC>
+static int series_blocks_per_cu_p(uint32_t q)
+{
+    const bool wa = q & (Q_A | Q_E), wb = q & Q_B, wg = q & (Q_G | Q_GJ);
+    switch ((wa ? 4 : 0) | (wb ? 2 : 0) | (wg ? 1 : 0)) {
+        case 1: return series_blocks_q<PREC, false, false, true>();
+        case 2: return series_blocks_q<PREC, false, true, false>();
+        case 3: return series_blocks_q<PREC, false, true, true>();
+        case 4: return series_blocks_q<PREC, true, false, false>();
+        case 5: return series_blocks_q<PREC, true, false, true>();
+        case 6: return series_blocks_q<PREC, true, true, false>();
+        default: return series_blocks_q<PREC, true, true, true>();
+    }
+}
+
+int field_series_blocks_per_cu(uint32_t q, int precision)
+{
+    return precision == kFP64 ? series_blocks_per_cu_p<kFP64>(q) : series_blocks_per_cu_p<kFP32C>(q);
 }
 
 int field_blocks_per_cu(uint32_t q, int precision, int split)

@@ -21,16 +21,18 @@ namespace ccoil {
 // the launch still has fewer than about 8 waves per SIMD. Large launches take 1-4 chunks, whichever makes the last
 // round of resident waves fullest: every wave does the same work, so with R rounds of resident waves the launch
 // runs ceil(R) rounds.
-FieldShape field_shape_rule(ccoil_ctx ctx, int64_t n, int64_t n_coils, uint32_t q, int precision)
+FieldShape field_shape_rule(ccoil_ctx ctx, int64_t n, int64_t n_coils, uint32_t q, int precision, bool series_only)
 {
     FieldShape sh;
     const int64_t target = (int64_t)256 * 4 * 8;
     sh.split = n < (int64_t)256 * 4 * 64 * 4 ? 4 : 1;
     if (sh.split == 4 && (n * 4 + 63) / 64 * n_coils < (int64_t)256 * 4) sh.split = 16;
+    if (series_only) sh.split = 1;   // no rows to split
     const int64_t waves = (n * sh.split + 63) / 64;
     int64_t chunks = waves >= target ? 1 : std::min<int64_t>(n_coils, (target + waves - 1) / waves);
     if (waves >= target && n * 18 < ((int64_t)1 << 31)) {   // the reduction grid stays below 2^31 threads
-        const int bpc = field_blocks_per_cu(q, precision, sh.split);
+        const int bpc = series_only ? field_series_blocks_per_cu(q, precision)
+                                    : field_blocks_per_cu(q, precision, sh.split);
         const double slots = (double)ctx->cu_count * bpc * (kFieldBlock / 64);
         if (slots > 0) {
             double best = 0.0;
@@ -265,9 +267,10 @@ ccoil_status fields_prepare(ccoil_ctx ctx, const ccoil_coils *coils, const doubl
         if (quantities & CCOIL_E) { CC_HIP(ctx, ensure(ctx->outE, s3)); p.E = (double *)ctx->outE.p; }
         if (quantities & (CCOIL_G | CCOIL_G_JACOBIAN)) { CC_HIP(ctx, ensure(ctx->outG, s9)); p.G = (double *)ctx->outG.p; }
     }
-    const FieldShape sh = shape ? *shape : field_shape_rule(ctx, n, (int64_t)hc.size(), quantities, precision);
+    const FieldShape sh = shape ? *shape : field_shape_rule(ctx, n, (int64_t)hc.size(), quantities, precision,
+                                                            rule_records);
     if (sh.split != 1 && sh.split != 4 && sh.split != 16) return fail(ctx, CCOIL_ERR_ARGUMENT, "split must be 1, 4 or 16");
-    L.split = sh.split;
+    L.split = rule_records ? 1 : sh.split;   // the series-only kernel runs one lane per point
     const int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(sh.chunks, (int64_t)hc.size()));
     p.coils_per_chunk = (int32_t)(((int64_t)hc.size() + chunks - 1) / chunks);
     p.n_chunks = (int32_t)(((int64_t)hc.size() + p.coils_per_chunk - 1) / p.coils_per_chunk);

@@ -111,6 +111,8 @@ void launch_field(hipStream_t s, const FieldParams &p, int precision, int split)
 void launch_field_reduce(hipStream_t s, const FieldParams &p);   // after launch_field when n_chunks > 1
 // Resident 256-thread blocks per CU of the field kernel instance (0 if the runtime cannot tell).
 int field_blocks_per_cu(uint32_t q, int precision, int split);
+// Resident 256-thread blocks per CU of the series-only kernel (the launch of calls with rule records).
+int field_series_blocks_per_cu(uint32_t q, int precision);
 // Exclusive prefix sum of n counts into offsets (kernel launches only); with temp == NULL only sets temp_bytes.
 hipError_t near_scan(hipStream_t s, const int64_t *counts, int64_t *offsets, int64_t n, void *temp,
                      size_t &temp_bytes);

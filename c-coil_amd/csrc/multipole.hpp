@@ -31,7 +31,7 @@ namespace ccoil {
 
 constexpr int kMpTerms = 60;        // highest order l
 constexpr int kMpCoil = 4 * 64;     // doubles per coil: m_l / l, m_l, (l + 1) m_l, (l + 1)(l + 2) m_l (index l)
-constexpr int kMpHeader = 2 * 64;   // recurrence coefficients (2l + 5) / (l + 3) and (l + 2) / (l + 3) (index l)
+constexpr int kMpHeader = 3 * 64;   // recurrence coefficients (2l + 5) / (l + 3), (l + 2) / (l + 3) and 2l + 5 (index l)
 constexpr int kMpLenNodes = 40;     // Gauss-Legendre nodes over the length of a fast-rule coil
 
 // Number of terms for a point at distance D from a coil whose winding lies within r_max: the truncation falls like
@@ -56,12 +56,19 @@ __device__ inline int wave_max_small(int v)
     return m;
 }
 
+// Smallest v over the lanes with active set, for 0 <= v < 64 (every lane of the wave executing); 63 if none is.
+__device__ inline int wave_min_small(int v, bool active)
+{
+    return 63 - wave_max_small(active ? 63 - v : 0);
+}
+
 // Local (A_phi, B_rho, B_z, B_rho / r, dB_rho/drho, dB_rho/dz, dB_z/dz) of one coil at local (z, r) from its series
 // (mp: the coil's kMpCoil doubles, rec: the shared recurrence coefficients, rmax: the coil's r_max): the loop runs L
-// terms (wave-uniform), and terms beyond own add exact zeros, so the value is that of own terms.
+// terms (wave-uniform), and terms beyond own add exact zeros, so the value is that of own terms. The first lo terms
+// (the least own count among the lanes that call; wave-uniform) need no such test, and run unrolled by two.
 template <bool WA, bool WB, bool WG>
 __device__ inline void eval_multipole(const double *mp, const double *rec, double rmax, double z, double r, int L,
-                                      int own, Local &out)
+                                      int lo, int own, Local &out)
 {
     const double D2 = r * r + z * z;
     const double u = 1.0 / sqrt(D2);
@@ -72,9 +79,7 @@ __device__ inline void eval_multipole(const double *mp, const double *rec, doubl
     double d0 = 1.0, d1 = 3.0 * c, d2 = 1.5 * (5.0 * c * c - 1.0);
     double q = qr;   // q_l = (r_max / D)^(l + 1)
     double SA = 0.0, SZ = 0.0, SB = 0.0, SZZ = 0.0, SRZ = 0.0;
-    for (int l = 1; l <= L; ++l) {
-        q *= qr;
-        const double ql = l <= own ? q : 0.0;
+    auto term = [&](int l, double ql) {
         if (WA) SA = fma(mA[l] * ql, d0, SA);
         if (WB || WG) {
             SZ = fma(mZ[l] * ql, P1, SZ);
@@ -86,12 +91,23 @@ __device__ inline void eval_multipole(const double *mp, const double *rec, doubl
         }
         // P_(l+3) = ((2l + 5) c P_(l+2) - (l + 2) P_(l+1)) / (l + 3), P'_(l+3) = P'_(l+1) + (2l + 5) P_(l+2)
         const double P3 = fma(rec[l] * c, P2, -rec[64 + l] * P1);
-        const double d3 = fma((double)(2 * l + 5), P2, d1);
+        const double d3 = fma(rec[128 + l], P2, d1);
         P1 = P2;
         P2 = P3;
         d0 = d1;
         d1 = d2;
         d2 = d3;
+    };
+    int l = 1;
+#pragma unroll 2
+    for (; l <= lo; ++l) {
+        q *= qr;
+        term(l, q);
+    }
+#pragma unroll 2
+    for (; l <= L; ++l) {
+        q *= qr;
+        term(l, l <= own ? q : 0.0);
     }
     out.A = s * SA;
     out.Bz = u * SZ;
@@ -104,10 +120,10 @@ __device__ inline void eval_multipole(const double *mp, const double *rec, doubl
 
 // The pair kernel's view of the series at a secondary node: A_phi (WA), B_rho and B_z (WB), and A_phi / r = u sum
 // m_l q_l P'_l / l (also on the axis, where it is the limit), for dA_phi/dr = B_z - A_phi / r. Loop as in
-// eval_multipole: L terms, the lane's own count own.
+// eval_multipole: L terms, the first lo without the test, the lane's own count own.
 template <bool WA, bool WB>
 __device__ inline void eval_multipole_pair(const double *mp, const double *rec, double rmax, double z, double r, int L,
-                                           int own, Local &out, double &a_over_r)
+                                           int lo, int own, Local &out, double &a_over_r)
 {
     const double D2 = r * r + z * z;
     const double u = 1.0 / sqrt(D2);
@@ -117,21 +133,30 @@ __device__ inline void eval_multipole_pair(const double *mp, const double *rec, 
     double d0 = 1.0, d1 = 3.0 * c, d2 = 1.5 * (5.0 * c * c - 1.0);
     double q = qr;
     double SA = 0.0, SZ = 0.0, SB = 0.0;
-    for (int l = 1; l <= L; ++l) {
-        q *= qr;
-        const double ql = l <= own ? q : 0.0;
+    auto term = [&](int l, double ql) {
         if (WA) SA = fma(mA[l] * ql, d0, SA);
         if (WB) {
             SZ = fma(mZ[l] * ql, P1, SZ);
             SB = fma(mB[l] * ql, d1, SB);
         }
         const double P3 = fma(rec[l] * c, P2, -rec[64 + l] * P1);
-        const double d3 = fma((double)(2 * l + 5), P2, d1);
+        const double d3 = fma(rec[128 + l], P2, d1);
         P1 = P2;
         P2 = P3;
         d0 = d1;
         d1 = d2;
         d2 = d3;
+    };
+    int l = 1;
+#pragma unroll 2
+    for (; l <= lo; ++l) {
+        q *= qr;
+        term(l, q);
+    }
+#pragma unroll 2
+    for (; l <= L; ++l) {
+        q *= qr;
+        term(l, l <= own ? q : 0.0);
     }
     out.A = s * SA;
     out.Bz = u * SZ;

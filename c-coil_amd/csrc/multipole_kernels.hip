@@ -27,6 +27,7 @@ __global__ __launch_bounds__(256) void multipole_moments_kernel(const CoilDev *c
     if (ci == 0 && t < 64) {   // recurrence coefficients of the Legendre polynomials
         mp[t] = (2.0 * t + 5.0) / (t + 3.0);
         mp[64 + t] = (t + 2.0) / (t + 3.0);
+        mp[128 + t] = 2.0 * t + 5.0;
     }
     const CoilDev &c = coils[ci];
     double *out = mp + kMpHeader + (size_t)ci * kMpCoil;

@@ -243,12 +243,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
                 if (!__ballot(far)) return;
                 const int own = far ? multipole_terms(n.z * n.z + n.r * n.r, pc.rmax,
                                                       PREC == 1 ? kMpLogTolFP32C : kMpLogTolFP64) : 0;
-                const int terms = wave_max_small(own);
+                const int terms = wave_max_small(own), lo = wave_min_small(own, far);
                 if (far && rg == 0) {
                     Local L = {0, 0, 0, 0, 0, 0, 0};
                     double aor = 0.0;
                     eval_multipole_pair<WM || WD, WF || WD>(p.mp + kMpHeader + (size_t)pd.prim * kMpCoil, p.mp,
-                                                            pc.rmax, n.z, n.r, terms, own, L, aor);
+                                                            pc.rmax, n.z, n.r, terms, lo, own, L, aor);
                     AdjA q = {0.0, 0.0, 0.0, L.Bz - aor};   // dA_phi/dr = B_z - A_phi / r
                     accumulate(n, L, q);
                 }

@@ -208,17 +208,19 @@ def test_argument_errors(ctx):
     assert out["B"].shape == (0, 3)
 
 
-@pytest.mark.parametrize("prec", [abi.FP64, abi.FP32C])
-def test_coil_chunking_matches_single_pass(ctx, prec):
+@pytest.mark.parametrize("prec,flags,splits", [(abi.FP64, 0, (1, 4, 16)), (abi.FP32C, abi.DIRECT, (1, 4, 16)),
+                                                (abi.FP32C, 0, (1, 1, 16))])
+def test_coil_chunking_matches_single_pass(ctx, prec, flags, splits):
     """Smaller calls split a point's thickness rows over 4 or 16 lanes and the coils over grid rows (summed in
-    order by field_reduce_kernel); a large call runs one lane and one pass per point. All give the same field to
-    rounding."""
+    order by field_reduce_kernel); a large call runs one lane and one pass per point. FP32C with the series runs the
+    series-only kernel (one lane per point) from 16,384 points on, where the record pass takes the rule's pairs. All
+    give the same field to rounding."""
     params, pts = W.group_workload(16, 600000, seed=13)
     rows, _ = H.gpu_args_rows(params)
-    big, rep_big = _run(ctx, rows, pts, abi.A | abi.B | abi.G, prec)
-    mid, rep_mid = _run(ctx, rows, pts[:40000], abi.A | abi.B | abi.G, prec)
-    small, rep_small = _run(ctx, rows, pts[:256], abi.A | abi.B | abi.G, prec)
-    assert rep_big.split == 1 and rep_mid.split == 4 and rep_small.split == 16
+    big, rep_big = _run(ctx, rows, pts, abi.A | abi.B | abi.G, prec, flags)
+    mid, rep_mid = _run(ctx, rows, pts[:40000], abi.A | abi.B | abi.G, prec, flags)
+    small, rep_small = _run(ctx, rows, pts[:256], abi.A | abi.B | abi.G, prec, flags)
+    assert (rep_big.split, rep_mid.split, rep_small.split) == splits
     for key in ("A", "B", "G"):
         for part in (small, mid):
             e = H.rel_err_vec(part[key][:256], big[key][:256])
