@@ -169,9 +169,11 @@ void field_kernel(FieldParams p)
 // with rule records): one lane per point, the coils looped inside; a pair beyond the distance takes the series, a
 // pair inside it is counted for the record pass. Without the rule's node loops the kernel holds far fewer registers
 // than field_kernel, so more waves hide the latency of the coefficient loads. A point's value is its far coils'
-// series in coil order, the same sums as field_kernel's.
+// series in coil order, the same sums as field_kernel's. It also counts the records of each coil key, which
+// near_count_kernel does for the other calls with a near pass.
 template <int PREC, bool WA, bool WB, bool WG>
-__global__ __launch_bounds__(kFieldBlock) void field_series_kernel(FieldParams p)
+__global__ __launch_bounds__(kFieldBlock) __attribute__((amdgpu_waves_per_eu(4)))
+void field_series_kernel(FieldParams p)
 {
     const int64_t vi = (int64_t)blockIdx.x * kFieldBlock + threadIdx.x;
     const bool active = vi < p.n_pts;
@@ -184,13 +186,29 @@ __global__ __launch_bounds__(kFieldBlock) void field_series_kernel(FieldParams p
     for (int k = 0; k < 9; ++k) acc.G[k] = 0.0;
 
     int n_near = 0;
+    const int lane = threadIdx.x & 63;
     const int c0 = blockIdx.y * p.coils_per_chunk, c1 = min(c0 + p.coils_per_chunk, p.n_coils);
+    // the record pass's per-key counts (its cursors come from their scan), per block in LDS: one ballot and one LDS
+    // atomic per wave, coil and kind in the loop, one global atomic per nonzero key after it. A global write inside
+    // the loop would keep the compiler from loading the coil records through the scalar cache.
+    extern __shared__ uint32_t key_counts[];   // [kind][coil of the chunk]
+    const bool count = series_counts_keys(p);
+    const int n_kinds = p.float_records ? 2 : 1, cpc = p.coils_per_chunk;
+    if (count)
+        for (int i = threadIdx.x; i < n_kinds * cpc; i += kFieldBlock) key_counts[i] = 0u;
+    __syncthreads();
     for (int ci = c0; ci < c1; ++ci) {
         const CoilDev &c = p.coils[ci];
         double z, r, cph, sph;
         to_local(c, px, py, pz, z, r, cph, sph);
         const bool far = is_far(c, z, r);
-        if (record_kind(p, c, z, r, WG) >= 0) ++n_near;
+        const int kind = active ? record_kind(p, c, z, r, WG) : -1;
+        if (kind >= 0) ++n_near;
+        if (count)
+            for (int k = 0; k < n_kinds; ++k) {
+                const uint64_t m = __ballot(kind == k);
+                if (m && lane == __builtin_ctzll(m)) atomicAdd(&key_counts[k * cpc + (ci - c0)], (unsigned)__popcll(m));
+            }
         if (__ballot(far)) {
             const int own = far ? multipole_terms(r * r + z * z, c.rmax,
                                                   PREC == kFP64 ? kMpLogTolFP64 : kMpLogTolFP32C) : 0;
@@ -203,6 +221,13 @@ __global__ __launch_bounds__(kFieldBlock) void field_series_kernel(FieldParams p
             }
         }
     }
+    __syncthreads();
+    if (count)
+        for (int i = threadIdx.x; i < n_kinds * cpc; i += kFieldBlock) {
+            const uint32_t v = key_counts[i];
+            const int k = i / cpc, ci = c0 + i % cpc;
+            if (v && ci < c1) atomicAdd(&p.near_hist[k * p.n_coils + p.coils[ci].near_key], v);
+        }
     if (!active) return;
     if (n_near > 0)   // integer sums: exact in any order
         atomicAdd((unsigned long long *)&p.near_counts[pi], (unsigned long long)n_near);
@@ -302,7 +327,8 @@ template <int PREC, bool WA, bool WB, bool WG>
 static void launch_series_q(hipStream_t s, const FieldParams &p)
 {
     const dim3 grid((unsigned)((p.n_pts + kFieldBlock - 1) / kFieldBlock), (unsigned)p.n_chunks);
-    hipLaunchKernelGGL((field_series_kernel<PREC, WA, WB, WG>), grid, dim3(kFieldBlock), 0, s, p);
+    const size_t lds = series_counts_keys(p) ? sizeof(uint32_t) * (p.float_records ? 2 : 1) * p.coils_per_chunk : 0;
+    hipLaunchKernelGGL((field_series_kernel<PREC, WA, WB, WG>), grid, dim3(kFieldBlock), lds, s, p);
 }
 
 template <int PREC>

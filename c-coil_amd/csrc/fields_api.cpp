@@ -171,7 +171,7 @@ void enqueue_fields(hipStream_t s, const FieldLaunch &L, hipEvent_t evm0, hipEve
     if (L.near) {
         size_t tb = L.scan_bytes;
         (void)near_scan(s, p.near_counts, const_cast<int64_t *>(p.near_offsets), p.n_pts, L.scan_temp, tb);
-        launch_near_count(s, p);
+        if (!series_counts_keys(p)) launch_near_count(s, p);   // else the series kernel counted the keys
         enqueue_near_tail(s, L, host_state, evr0, evr1);
     }
 }

@@ -111,6 +111,12 @@ void launch_field(hipStream_t s, const FieldParams &p, int precision, int split)
 void launch_field_reduce(hipStream_t s, const FieldParams &p);   // after launch_field when n_chunks > 1
 // Resident 256-thread blocks per CU of the field kernel instance (0 if the runtime cannot tell).
 int field_blocks_per_cu(uint32_t q, int precision, int split);
+// Whether the series-only kernel counts the record pass's keys (in LDS, 4 bytes per coil of a chunk and kind), so
+// that near_count_kernel is not launched: calls with rule records whose coil chunks fit.
+__host__ __device__ inline bool series_counts_keys(const FieldParams &p)
+{
+    return p.rule_records && p.coils_per_chunk <= 4096;
+}
 // Resident 256-thread blocks per CU of the series-only kernel (the launch of calls with rule records).
 int field_series_blocks_per_cu(uint32_t q, int precision);
 // Exclusive prefix sum of n counts into offsets (kernel launches only); with temp == NULL only sets temp_bytes.
