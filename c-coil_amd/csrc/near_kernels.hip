@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void near_fill_kernel(FieldParams p)
 // of nodes sets the kernel's duration; the lanes then stride over the records (a group's value does not depend on
 // the grid). FLOAT: the kind-1 records [fp64, total) by the float rule, at 5 waves per SIMD as field_kernel's float
 // instances; else the kind-0 records [0, fp64) by the FP64 rule.
-template <bool WA, bool WB, bool WG, bool FLOAT, int S>
+template <bool WA, bool WB, bool WG, bool FLOAT, int S, bool LOOP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FLOAT ? 5 : 1)))
 void near_eval_kernel(FieldParams p)
 {
@@ -182,14 +182,17 @@ void near_eval_kernel(FieldParams p)
             if (p.G) for (int j = 0; j < 9; ++j) o[j] = acc.G[j];
         }
     };
-    if constexpr (S == 1) {
-        const int64_t e = lo + (int64_t)blockIdx.x * 256 + threadIdx.x;
-        if (e < hi) record(e);
+    const int64_t e0 = lo + ((int64_t)blockIdx.x * 256 + threadIdx.x) / S;
+    if constexpr (!LOOP) {
+        // the grid covers the capacity, one record per group; with no store before the loads, the coil records
+        // and the tables are scalar loads
+        if (e0 < hi) record(e0);
     } else {
-        // a wave's groups hold consecutive records, so the loop's exit is wave-uniform once no lane has a record
+        // a bounded grid strides over a larger capacity (the stores in the loop make the loads vector loads). A
+        // wave's groups hold consecutive records, so the loop's exit is wave-uniform once no lane has a record
         // left, and a group is active or idle as a whole
         const int64_t stride = (int64_t)gridDim.x * (256 / S);
-        for (int64_t e = lo + ((int64_t)blockIdx.x * 256 + threadIdx.x) / S; __ballot(e < hi); e += stride)
+        for (int64_t e = e0; __ballot(e < hi); e += stride)
             if (e < hi) record(e);
     }
 }
@@ -324,24 +327,33 @@ void launch_near_fill(hipStream_t s, const FieldParams &p)
     hipLaunchKernelGGL(near_fill_kernel, dim3((unsigned)((p.n_pts + 255) / 256)), dim3(256), 0, s, p);
 }
 
+template <bool FLOAT, int S, bool LOOP>
+static void launch_near_eval_g(hipStream_t s, const FieldParams &p, unsigned blocks)
+{
+    const bool wa = p.q & (Q_A | Q_E), wb = p.q & Q_B, wg = p.q & (Q_G | Q_GJ);
+    const dim3 g(blocks), b(256);
+    switch ((wa ? 4 : 0) | (wb ? 2 : 0) | (wg ? 1 : 0)) {
+        case 1: hipLaunchKernelGGL((near_eval_kernel<false, false, true, FLOAT, S, LOOP>), g, b, 0, s, p); break;
+        case 2: hipLaunchKernelGGL((near_eval_kernel<false, true, false, FLOAT, S, LOOP>), g, b, 0, s, p); break;
+        case 3: hipLaunchKernelGGL((near_eval_kernel<false, true, true, FLOAT, S, LOOP>), g, b, 0, s, p); break;
+        case 4: hipLaunchKernelGGL((near_eval_kernel<true, false, false, FLOAT, S, LOOP>), g, b, 0, s, p); break;
+        case 5: hipLaunchKernelGGL((near_eval_kernel<true, false, true, FLOAT, S, LOOP>), g, b, 0, s, p); break;
+        case 6: hipLaunchKernelGGL((near_eval_kernel<true, true, false, FLOAT, S, LOOP>), g, b, 0, s, p); break;
+        default: hipLaunchKernelGGL((near_eval_kernel<true, true, true, FLOAT, S, LOOP>), g, b, 0, s, p); break;
+    }
+}
+
+// S = 1: one lane per record of the capacity. S > 1: S lanes per record of the capacity while that is at most 2^21
+// lanes, else a grid of 2^21 lanes striding over the records.
 template <bool FLOAT, int S>
 static void launch_near_eval_k(hipStream_t s, const FieldParams &p)
 {
-    // S = 1: one lane per record of the capacity; S > 1: a strided grid of at most 2^21 lanes
-    const int64_t cap_lanes = p.near_cap * S, max_lanes = (int64_t)1 << 21;
-    const int64_t lanes = (S == 1 || cap_lanes < max_lanes) ? cap_lanes : max_lanes;
-    const unsigned blocks = (unsigned)((lanes + 255) / 256);
-    const bool wa = p.q & (Q_A | Q_E), wb = p.q & Q_B, wg = p.q & (Q_G | Q_GJ);
-    const int code = (wa ? 4 : 0) | (wb ? 2 : 0) | (wg ? 1 : 0);
-    const dim3 g(blocks), b(256);
-    switch (code) {
-        case 1: hipLaunchKernelGGL((near_eval_kernel<false, false, true, FLOAT, S>), g, b, 0, s, p); break;
-        case 2: hipLaunchKernelGGL((near_eval_kernel<false, true, false, FLOAT, S>), g, b, 0, s, p); break;
-        case 3: hipLaunchKernelGGL((near_eval_kernel<false, true, true, FLOAT, S>), g, b, 0, s, p); break;
-        case 4: hipLaunchKernelGGL((near_eval_kernel<true, false, false, FLOAT, S>), g, b, 0, s, p); break;
-        case 5: hipLaunchKernelGGL((near_eval_kernel<true, false, true, FLOAT, S>), g, b, 0, s, p); break;
-        case 6: hipLaunchKernelGGL((near_eval_kernel<true, true, false, FLOAT, S>), g, b, 0, s, p); break;
-        default: hipLaunchKernelGGL((near_eval_kernel<true, true, true, FLOAT, S>), g, b, 0, s, p); break;
+    const int64_t lanes = p.near_cap * S, max_lanes = (int64_t)1 << 21;
+    if constexpr (S == 1) {
+        launch_near_eval_g<FLOAT, 1, false>(s, p, (unsigned)((lanes + 255) / 256));
+    } else {
+        if (lanes <= max_lanes) launch_near_eval_g<FLOAT, S, false>(s, p, (unsigned)((lanes + 255) / 256));
+        else launch_near_eval_g<FLOAT, S, true>(s, p, (unsigned)(max_lanes / 256));
     }
 }
 
