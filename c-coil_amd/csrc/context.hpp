@@ -116,7 +116,14 @@ constexpr int64_t kOrderMinPoints = 16384;
 constexpr int64_t kRecordMaxPoints = (int64_t)1 << 20;
 // Whether a field call of this precision, flags and size has a record pass, and whether it has rule records.
 bool field_record_pass(int32_t precision, uint32_t flags, int64_t n_points, bool fixed_arena, bool *rule_records);
-double far_threshold(int n_ang, double q, double tol);
+double far_threshold(int n_ang, double q, double tol, double ylo);
+// The nearest distance, in units of r_max, at which the series may replace a coil's rule on cost, in [kFarMinY, 2].
+constexpr double kFarMinY = 1.3;
+double far_cost_floor(int n_ang, int n_thk, bool fast, double log_tol);
+// Distance, in units of r_max, beyond which the coil's series replaces its rule (0: never): far_threshold at the
+// precision's tolerance (fp64: kFarToleranceFP64, else kFarTolerance) from the cost floor on (cost_floor: launches
+// with one lane per point or secondary node), else from 2.
+double far_distance(const CoilDev &c, bool fp64, bool cost_floor);
 // Enqueues the field kernel, the chunk reduction and (FP32C) the whole near pass on s, with no host
 // synchronisation and kernel launches and copies only (so it can be captured in a graph); evm0/evm1 (may be NULL)
 // bracket the field kernel. The near pass's outcome is copied to host_state (pinned) when given.

@@ -28,7 +28,7 @@ FieldShape field_shape_rule(ccoil_ctx ctx, int64_t n, int64_t n_coils, uint32_t 
     sh.split = n < (int64_t)256 * 4 * 64 * 4 ? 4 : 1;
     if (sh.split == 4 && (n * 4 + 63) / 64 * n_coils < (int64_t)256 * 4) sh.split = 16;
     if (series_only) sh.split = 1;   // no rows to split
-    const int64_t waves = (n * sh.split + 63) / 64;
+    const int64_t waves = std::max<int64_t>(1, (n * sh.split + 63) / 64);
     int64_t chunks = waves >= target ? 1 : std::min<int64_t>(n_coils, (target + waves - 1) / waves);
     if (waves >= target && n * 18 < ((int64_t)1 << 31)) {   // the reduction grid stays below 2^31 threads
         const int bpc = series_only ? field_series_blocks_per_cu(q, precision)
@@ -49,7 +49,8 @@ FieldShape field_shape_rule(ccoil_ctx ctx, int64_t n, int64_t n_coils, uint32_t 
     return sh;
 }
 
-// Distance, in units of the coil's r_max, beyond which the FP32C far-field series replaces the rule (0: never).
+// Distance, in units of the coil's r_max, beyond which the far-field series replaces the rule (0: never), at least
+// ylo (far_distance below).
 //
 // The series sums the rule's own rings exactly, so it departs from the rule by the angular rule's error and its
 // truncation (below 1e-16 from D = 2 r_max on). The angular integrand over phi in [0, pi] (Gauss-Legendre on
@@ -61,8 +62,8 @@ FieldShape field_shape_rule(ccoil_ctx ctx, int64_t n, int64_t n_coils, uint32_t 
 // rho_B^(-2 n_ang). Relative to the coil's field at D the error is at most 1000 y^2 rho_B^(-2 n_ang) (y = D / r_max;
 // the y^2 for the node terms that cancel down to the far field, the constant 10x above what a filament, a thin and a
 // thick coil measured in FP64 against the series at n_ang = 4..24 and y = 2..50). The threshold is the least y in
-// [2, 100] with that bound below tol: for n_ang >= 5 the bound falls monotonically there, so a bisection finds it.
-double far_threshold(int n_ang, double q, double tol)
+// [ylo, 100] with that bound below tol: for n_ang >= 5 the bound falls monotonically there, so a bisection finds it.
+double far_threshold(int n_ang, double q, double tol, double ylo)
 {
     auto bound = [&](double y) {
         const double d = y - 1.0;
@@ -73,14 +74,34 @@ double far_threshold(int n_ang, double q, double tol)
         return 1000.0 * y * y * std::pow(rho, -2.0 * n_ang);
     };
     if (n_ang < 5) return 0.0;
-    if (bound(2.0) <= tol) return 2.0;
+    if (bound(ylo) <= tol) return ylo;
     if (bound(100.0) > tol) return 0.0;
-    double lo = 2.0, hi = 100.0;
+    double lo = ylo, hi = 100.0;
     for (int it = 0; it < 40; ++it) {
         const double mid = 0.5 * (lo + hi);
         (bound(mid) <= tol ? hi : lo) = mid;
     }
     return hi;
+}
+
+// Nearest distance, in units of r_max, at which the series may take over on cost: a pair's series of L(y) terms
+// (multipole_terms: L = 2 ln(2 / tol) / ln(y^2) + 2) costs about 15 VALU operations per term plus 200 in the kernels
+// of this library, the coil's rule about 55 per fast-method node and 12 per slow-method node (n_ang x n_thk nodes).
+// Where the rule is dearer the series starts at the nearest distance with L(y) within budget, but not below kFarMinY
+// (where L reaches about kMpTerms) and not beyond 2 r_max, from where the series replaces every rule.
+double far_cost_floor(int n_ang, int n_thk, bool fast, double log_tol)
+{
+    const double rule = (double)n_ang * n_thk * (fast ? 55.0 : 12.0);
+    if (rule <= 230.0) return 2.0;
+    const double y = std::exp(15.0 * log_tol / (rule - 230.0));
+    return std::min(2.0, std::max(kFarMinY, y));
+}
+
+double far_distance(const CoilDev &c, bool fp64, bool cost_floor)
+{
+    const double log_tol = fp64 ? kMpLogTolFP64 : kMpLogTolFP32C;
+    return far_threshold(c.n_ang, c.rmax / (c.R + c.a), fp64 ? kFarToleranceFP64 : kFarTolerance,
+                         cost_floor ? far_cost_floor(c.n_ang, c.n_thk, c.use_fast != 0, log_tol) : 2.0);
 }
 
 bool field_record_pass(int32_t precision, uint32_t flags, int64_t n_points, bool fixed_arena, bool *rule_records)
@@ -218,7 +239,6 @@ ccoil_status fields_prepare(ccoil_ctx ctx, const ccoil_coils *coils, const doubl
     p.float_records = rule_records && precision == CCOIL_FP32C;
     const bool far = (precision == CCOIL_FP32C && !(flags & CCOIL_DIRECT)) ||
                      (precision == CCOIL_FP64 && (flags & CCOIL_FAR_FIELD));
-    const double far_tol = precision == CCOIL_FP64 ? kFarToleranceFP64 : kFarTolerance;
     if (dev) {
         p.pts = points;
         p.A = (quantities & CCOIL_A) ? A : nullptr;
@@ -229,14 +249,19 @@ ccoil_status fields_prepare(ccoil_ctx ctx, const ccoil_coils *coils, const doubl
     out.width = ((quantities & CCOIL_A) ? 3 : 0) + ((quantities & CCOIL_E) ? 3 : 0) +
                 ((quantities & CCOIL_B) ? 3 : 0) + ((quantities & (CCOIL_G | CCOIL_G_JACOBIAN)) ? 9 : 0);
     out.nodes = 0;
-    for (auto &c : hc) {
-        out.nodes += (int64_t)c.n_ang * c.n_thk;
-        if (far) {
-            const double y = far_threshold(c.n_ang, c.rmax / (c.R + c.a), far_tol);
+    for (auto &c : hc) out.nodes += (int64_t)c.n_ang * c.n_thk;
+    if (n == 0 || hc.empty()) return CCOIL_OK;   // nothing to evaluate, or every field zero
+    const FieldShape sh = shape ? *shape : field_shape_rule(ctx, n, (int64_t)hc.size(), quantities, precision,
+                                                            rule_records);
+    if (sh.split != 1 && sh.split != 4 && sh.split != 16) return fail(ctx, CCOIL_ERR_ARGUMENT, "split must be 1, 4 or 16");
+    L.split = rule_records ? 1 : sh.split;   // the series-only kernel runs one lane per point
+    if (far)
+        for (auto &c : hc) {
+            // the cost floor holds for one lane per point; a launch that splits the rows runs the series on one
+            // lane of the point's group, so it keeps the series from 2 r_max on
+            const double y = far_distance(c, precision == CCOIL_FP64, L.split == 1);
             c.far_d2 = y > 0.0 ? (y * c.rmax) * (y * c.rmax) : 0.0;
         }
-    }
-    if (n == 0 || hc.empty()) return CCOIL_OK;   // nothing to evaluate, or every field zero
 
     // The FP32C near pass sorts its (coil, point) records by near_key: coils with the most nodes first, so its
     // longest waves start first. near_coil (near_key -> coil) follows the coils in the same buffer.
@@ -267,10 +292,6 @@ ccoil_status fields_prepare(ccoil_ctx ctx, const ccoil_coils *coils, const doubl
         if (quantities & CCOIL_E) { CC_HIP(ctx, ensure(ctx->outE, s3)); p.E = (double *)ctx->outE.p; }
         if (quantities & (CCOIL_G | CCOIL_G_JACOBIAN)) { CC_HIP(ctx, ensure(ctx->outG, s9)); p.G = (double *)ctx->outG.p; }
     }
-    const FieldShape sh = shape ? *shape : field_shape_rule(ctx, n, (int64_t)hc.size(), quantities, precision,
-                                                            rule_records);
-    if (sh.split != 1 && sh.split != 4 && sh.split != 16) return fail(ctx, CCOIL_ERR_ARGUMENT, "split must be 1, 4 or 16");
-    L.split = rule_records ? 1 : sh.split;   // the series-only kernel runs one lane per point
     const int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(sh.chunks, (int64_t)hc.size()));
     p.coils_per_chunk = (int32_t)(((int64_t)hc.size() + chunks - 1) / chunks);
     p.n_chunks = (int32_t)(((int64_t)hc.size() + p.coils_per_chunk - 1) / p.coils_per_chunk);
@@ -430,11 +451,10 @@ extern "C" ccoil_status ccoil_far_field_radius(const ccoil_coils *coils, int32_t
     if (!hc.empty() && (!rmax || !distance))
         return fail(nullptr, CCOIL_ERR_ARGUMENT, "ccoil_far_field_radius: an output is NULL");
     const bool far = precision == CCOIL_FP32C || precision == CCOIL_FP64;
-    const double tol = precision == CCOIL_FP64 ? kFarToleranceFP64 : kFarTolerance;
     for (size_t i = 0; i < hc.size(); ++i) {
         const CoilDev &c = hc[i];
         rmax[i] = c.rmax;
-        distance[i] = far ? far_threshold(c.n_ang, c.rmax / (c.R + c.a), tol) * c.rmax : 0.0;
+        distance[i] = far ? far_distance(c, precision == CCOIL_FP64, true) * c.rmax : 0.0;
     }
     return CCOIL_OK;
 }

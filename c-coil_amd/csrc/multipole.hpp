@@ -12,7 +12,7 @@
 //
 // The rings are the field rule's own: its thickness nodes (the rule's Gauss-Legendre nodes over [R, R + a], so the
 // series reproduces the rule's thickness sum exactly, whatever its order), and over the length either the exact
-// average (the fast rule integrates the length in closed form; 40 Gauss-Legendre nodes are exact for l <= 60) or
+// average (the fast rule integrates the length in closed form; 76 Gauss-Legendre nodes are exact for l <= 150) or
 // the mid-plane alone (the slow rule's flat sheet). What the series leaves out is then the angular rule's own error
 // and the truncation. With s = sin theta, u = 1 / D and q_l = (r_max / D)^(l+1), and the moments normalised as
 // m_l = mu0 M_l / r_max^(l+1):
@@ -29,10 +29,12 @@
 
 namespace ccoil {
 
-constexpr int kMpTerms = 60;        // highest order l
-constexpr int kMpCoil = 4 * 64;     // doubles per coil: m_l / l, m_l, (l + 1) m_l, (l + 1)(l + 2) m_l (index l)
-constexpr int kMpHeader = 3 * 64;   // recurrence coefficients (2l + 5) / (l + 3), (l + 2) / (l + 3) and 2l + 5 (index l)
-constexpr int kMpLenNodes = 40;     // Gauss-Legendre nodes over the length of a fast-rule coil
+constexpr int kMpTerms = 150;               // highest order l
+constexpr int kMpStride = 160;              // doubles per coefficient array (index l)
+constexpr int kMpCoil = 4 * kMpStride;      // per coil: m_l / l, m_l, (l + 1) m_l, (l + 1)(l + 2) m_l
+constexpr int kMpHeader = 3 * kMpStride;    // recurrence coefficients (2l + 5) / (l + 3), (l + 2) / (l + 3), 2l + 5
+constexpr int kMpLenNodes = 76;             // Gauss-Legendre nodes over the length of a fast-rule coil: exact to
+                                            // degree 151, and V_l has degree l - 1 in the ring's height
 
 // Number of terms for a point at distance D from a coil whose winding lies within r_max: the truncation falls like
 // (r_max / D)^L with a constant below 2, so L = ln(2 / tol) / ln(D / r_max), between 4 and kMpTerms; log_tol =
@@ -46,20 +48,20 @@ __device__ inline int multipole_terms(double D2, double rmax, float log_tol)
     return L < 4 ? 4 : (L > kMpTerms ? kMpTerms : L);
 }
 
-// Largest v over the wave for 0 <= v < 64 (every lane active): its bits from the top, one ballot each.
+// Largest v over the wave for 0 <= v < 256 (every lane active): its bits from the top, one ballot each.
 __device__ inline int wave_max_small(int v)
 {
     int m = 0;
 #pragma unroll
-    for (int b = 32; b >= 1; b >>= 1)
+    for (int b = 128; b >= 1; b >>= 1)
         if (__ballot(v >= (m | b))) m |= b;
     return m;
 }
 
-// Smallest v over the lanes with active set, for 0 <= v < 64 (every lane of the wave executing); 63 if none is.
+// Smallest v over the lanes with active set, for 0 <= v < 256 (every lane of the wave executing); 255 if none is.
 __device__ inline int wave_min_small(int v, bool active)
 {
-    return 63 - wave_max_small(active ? 63 - v : 0);
+    return 255 - wave_max_small(active ? 255 - v : 0);
 }
 
 // Local (A_phi, B_rho, B_z, B_rho / r, dB_rho/drho, dB_rho/dz, dB_z/dz) of one coil at local (z, r) from its series
@@ -73,7 +75,7 @@ __device__ inline void eval_multipole(const double *mp, const double *rec, doubl
     const double D2 = r * r + z * z;
     const double u = 1.0 / sqrt(D2);
     const double c = z * u, s = r * u, qr = rmax * u;
-    const double *mA = mp, *mB = mp + 64, *mZ = mp + 128, *mZZ = mp + 192;
+    const double *mA = mp, *mB = mp + kMpStride, *mZ = mp + 2 * kMpStride, *mZZ = mp + 3 * kMpStride;
     // P_(l+1), P_(l+2) and P'_l, P'_(l+1), P'_(l+2), advanced together from l = 1
     double P1 = 0.5 * (3.0 * c * c - 1.0), P2 = 0.5 * c * (5.0 * c * c - 3.0);
     double d0 = 1.0, d1 = 3.0 * c, d2 = 1.5 * (5.0 * c * c - 1.0);
@@ -90,8 +92,8 @@ __device__ inline void eval_multipole(const double *mp, const double *rec, doubl
             SRZ = fma(mZ[l] * ql, d2, SRZ);
         }
         // P_(l+3) = ((2l + 5) c P_(l+2) - (l + 2) P_(l+1)) / (l + 3), P'_(l+3) = P'_(l+1) + (2l + 5) P_(l+2)
-        const double P3 = fma(rec[l] * c, P2, -rec[64 + l] * P1);
-        const double d3 = fma(rec[128 + l], P2, d1);
+        const double P3 = fma(rec[l] * c, P2, -rec[kMpStride + l] * P1);
+        const double d3 = fma(rec[2 * kMpStride + l], P2, d1);
         P1 = P2;
         P2 = P3;
         d0 = d1;
@@ -128,7 +130,7 @@ __device__ inline void eval_multipole_pair(const double *mp, const double *rec, 
     const double D2 = r * r + z * z;
     const double u = 1.0 / sqrt(D2);
     const double c = z * u, s = r * u, qr = rmax * u;
-    const double *mA = mp, *mB = mp + 64, *mZ = mp + 128;
+    const double *mA = mp, *mB = mp + kMpStride, *mZ = mp + 2 * kMpStride;
     double P1 = 0.5 * (3.0 * c * c - 1.0), P2 = 0.5 * c * (5.0 * c * c - 3.0);
     double d0 = 1.0, d1 = 3.0 * c, d2 = 1.5 * (5.0 * c * c - 1.0);
     double q = qr;
@@ -139,8 +141,8 @@ __device__ inline void eval_multipole_pair(const double *mp, const double *rec, 
             SZ = fma(mZ[l] * ql, P1, SZ);
             SB = fma(mB[l] * ql, d1, SB);
         }
-        const double P3 = fma(rec[l] * c, P2, -rec[64 + l] * P1);
-        const double d3 = fma(rec[128 + l], P2, d1);
+        const double P3 = fma(rec[l] * c, P2, -rec[kMpStride + l] * P1);
+        const double d3 = fma(rec[2 * kMpStride + l], P2, d1);
         P1 = P2;
         P2 = P3;
         d0 = d1;

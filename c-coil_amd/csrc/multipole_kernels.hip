@@ -20,66 +20,86 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
-__global__ __launch_bounds__(256) void multipole_moments_kernel(const CoilDev *coils, Tables tab, double *mp)
+// Adds w V_l of one ring (radius^2 c2, height z0, both in units of r_max) to acc[l - L0] for the orders l of the
+// chunk [L0, L0 + 64) up to kMpTerms. The recurrence runs from l = 1 with compile-time indices, so acc stays in
+// registers (each chunk repeats the recurrence below it).
+template <int L0>
+__device__ inline void ring_moments(double w, double z0, double c2, double *acc)
 {
-    __shared__ double part[4][kMpTerms + 1];
-    const int ci = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    if (ci == 0 && t < 64) {   // recurrence coefficients of the Legendre polynomials
-        mp[t] = (2.0 * t + 5.0) / (t + 3.0);
-        mp[64 + t] = (t + 2.0) / (t + 3.0);
-        mp[128 + t] = 2.0 * t + 5.0;
+    constexpr int kEnd = L0 + 64 < kMpTerms + 1 ? L0 + 64 : kMpTerms + 1;   // orders below kEnd
+    double Um = 1.0, U = z0;     // U_(l-1), U_l
+    double Vm = 0.0, V = 1.0;    // V_(l-1), V_l
+    if constexpr (L0 <= 1) acc[1 - L0] += w;
+#pragma unroll
+    for (int l = 1; l + 1 < kEnd; ++l) {
+        const double Un = ((2.0 * l + 1.0) * z0 * U - l * c2 * Um) * (1.0 / (l + 1.0));
+        const double Vn = c2 * Vm + (2.0 * l + 1.0) * U;
+        Um = U;
+        U = Un;
+        Vm = V;
+        V = Vn;
+        if (l + 1 >= L0) acc[l + 1 - L0] += w * V;
     }
-    const CoilDev &c = coils[ci];
-    double *out = mp + kMpHeader + (size_t)ci * kMpCoil;
-    if (!(c.far_d2 > 0.0)) {   // the coil never uses its series
-        out[t] = 0.0;
-        return;
-    }
+}
+
+// The moments of orders [L0, L0 + 64) of coil c into out (a block of 256 threads over the coil's rings).
+template <int L0>
+__device__ inline void chunk_moments(const CoilDev &c, const Tables &tab, double *out, double (*part)[64])
+{
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const double inv = 1.0 / c.rmax;
     const int nt = c.n_thk, nz = c.use_fast ? kMpLenNodes : 1;
     const double *xt = tab.x + table_offset(nt), *wt = tab.w + table_offset(nt);
     const double *xz = tab.x + table_offset(kMpLenNodes), *wz = tab.w + table_offset(kMpLenNodes);
-    double acc[kMpTerms + 1];
+    double acc[64];
 #pragma unroll
-    for (int l = 0; l <= kMpTerms; ++l) acc[l] = 0.0;
+    for (int l = 0; l < 64; ++l) acc[l] = 0.0;
     for (int k = t; k < nt * nz; k += 256) {
         const int j = k / nz, i = k - j * nz;
         // ring (rho, z0) in units of r_max, weight: its share of the current (the weights sum to 1)
         const double rho = (c.R + 0.5 * c.a * (1.0 + xt[j])) * inv;
         const double z0 = nz > 1 ? 0.5 * c.b * xz[i] * inv : 0.0;
         const double w = 0.5 * wt[j] * (nz > 1 ? 0.5 * wz[i] : 1.0) * rho * rho;
-        const double c2 = rho * rho + z0 * z0;
-        double Um = 1.0, U = z0;     // U_(l-1), U_l
-        double Vm = 0.0, V = 1.0;    // V_(l-1), V_l
-        acc[1] += w;
-#pragma unroll
-        for (int l = 1; l < kMpTerms; ++l) {
-            const double Un = ((2.0 * l + 1.0) * z0 * U - l * c2 * Um) / (l + 1.0);
-            const double Vn = c2 * Vm + (2.0 * l + 1.0) * U;
-            Um = U;
-            U = Un;
-            Vm = V;
-            V = Vn;
-            acc[l + 1] += w * V;
-        }
+        ring_moments<L0>(w, z0, rho * rho + z0 * z0, acc);
     }
 #pragma unroll
-    for (int l = 1; l <= kMpTerms; ++l) {
+    for (int l = 0; l < 64; ++l) {
         const double v = wave_sum(acc[l]);
         if (lane == 0) part[wave][l] = v;
     }
     __syncthreads();
     // m_l = mu0 N I sum / (2 (l + 1)), mu0 = 4 pi 1e-7
     const double ni = c.ni * (4.0 * kPi * kMuReduced);
-    if (t < 64) {
-        const int l = t;
+    if (t < 64 && L0 + t < kMpStride) {
+        const int l = L0 + t;
         double m = 0.0;
-        if (l >= 1 && l <= kMpTerms) m = ni * (((part[0][l] + part[1][l]) + part[2][l]) + part[3][l]) / (2.0 * (l + 1));
+        if (l >= 1 && l <= kMpTerms) m = ni * (((part[0][t] + part[1][t]) + part[2][t]) + part[3][t]) / (2.0 * (l + 1));
         out[l] = l >= 1 ? m / l : 0.0;
-        out[64 + l] = m;
-        out[128 + l] = (l + 1) * m;
-        out[192 + l] = (l + 1.0) * (l + 2.0) * m;
+        out[kMpStride + l] = m;
+        out[2 * kMpStride + l] = (l + 1) * m;
+        out[3 * kMpStride + l] = (l + 1.0) * (l + 2.0) * m;
     }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void multipole_moments_kernel(const CoilDev *coils, Tables tab, double *mp)
+{
+    __shared__ double part[4][64];
+    const int ci = blockIdx.x, t = threadIdx.x;
+    if (ci == 0 && t < kMpStride) {   // recurrence coefficients of the Legendre polynomials
+        mp[t] = (2.0 * t + 5.0) / (t + 3.0);
+        mp[kMpStride + t] = (t + 2.0) / (t + 3.0);
+        mp[2 * kMpStride + t] = 2.0 * t + 5.0;
+    }
+    const CoilDev &c = coils[ci];
+    double *out = mp + kMpHeader + (size_t)ci * kMpCoil;
+    if (!(c.far_d2 > 0.0)) {   // the coil never uses its series
+        for (int i = t; i < kMpCoil; i += 256) out[i] = 0.0;
+        return;
+    }
+    chunk_moments<0>(c, tab, out, part);
+    chunk_moments<64>(c, tab, out, part);
+    chunk_moments<128>(c, tab, out, part);
 }
 
 // ---------------------------------------------------------------------------------------------

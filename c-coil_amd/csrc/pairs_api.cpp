@@ -90,8 +90,7 @@ ccoil_status build_pairs(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coi
                                       (precision == CCOIL_FP64 && (flags & CCOIL_FAR_FIELD)));
     if (out.far)
         for (auto &c : hp) {
-            const double y = far_threshold(c.n_ang, c.rmax / (c.R + c.a),
-                                           precision == CCOIL_FP64 ? kFarToleranceFP64 : kFarTolerance);
+            const double y = far_distance(c, precision == CCOIL_FP64, true);
             c.far_d2 = y > 0.0 ? (y * c.rmax) * (y * c.rmax) : 0.0;
         }
 

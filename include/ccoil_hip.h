@@ -146,8 +146,10 @@ ccoil_status ccoil_synchronize(ccoil_ctx ctx);
 ccoil_status ccoil_set_tuning_file(ccoil_ctx ctx, const char *path);
 /* Far-field series geometry (no counterpart in the reference; host only, no context or device): for each coil, the
  * radius r_max of the sphere about its centre that holds its winding and the distance beyond which a field call of
- * the given precision takes the coil's series (CCOIL_FP32C by default, CCOIL_FP64 with CCOIL_FAR_FIELD); 0: the coil
- * never does (few angular increments). Other precisions give 0. Errors: ccoil_last_error(NULL). */
+ * the given precision takes the coil's series (CCOIL_FP32C by default, CCOIL_FP64 with CCOIL_FAR_FIELD), as pair
+ * calls and field calls with one lane per point (16,384 points or more in these modes) use it; field calls with fewer
+ * points take the series from at least 2 r_max. 0: the coil never does (few angular increments). Other precisions
+ * give 0. Errors: ccoil_last_error(NULL). */
 ccoil_status ccoil_far_field_radius(const ccoil_coils *coils, int32_t precision, double *rmax, double *distance);
 /* The stream the context's kernels are ordered on (its own, or the one given to ccoil_set_stream), as hipStream_t
  * cast to void*. No counterpart in the reference: zero-copy callers hand it to a producer (DLPack __dlpack__(stream))

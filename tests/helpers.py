@@ -40,10 +40,10 @@ def rel_err_vec(x, ref, floor=0.0):
     return out
 
 
-def far_threshold(n_ang, q, tol=1e-11):
-    """Distance in units of r_max beyond which CCOIL_FP32C takes a coil's far-field series (0: never), restated from
-    the bound the library documents: the least y in [2, 100] with 1000 y^2 rho_B^(-2 n_ang) <= tol, where rho_B is
-    the Bernstein-ellipse parameter of x_s = -1 + 2i eta / pi and cosh eta = 1 + (y - 1)^2 q / (2 y),
+def far_threshold(n_ang, q, tol=1e-11, ylo=2.0):
+    """Distance in units of r_max beyond which the far-field series may replace the rule on accuracy (0: never),
+    restated from the bound the library documents: the least y in [ylo, 100] with 1000 y^2 rho_B^(-2 n_ang) <= tol,
+    where rho_B is the Bernstein-ellipse parameter of x_s = -1 + 2i eta / pi and cosh eta = 1 + (y - 1)^2 q / (2 y),
     q = r_max / (R + a)."""
     def bound(y):
         d = y - 1.0
@@ -53,11 +53,11 @@ def far_threshold(n_ang, q, tol=1e-11):
         return 1000.0 * y * y * max(abs(xs + sq), abs(xs - sq)) ** (-2.0 * n_ang)
     if n_ang < 5:
         return 0.0
-    if bound(2.0) <= tol:
-        return 2.0
+    if bound(ylo) <= tol:
+        return ylo
     if bound(100.0) > tol:
         return 0.0
-    lo, hi = 2.0, 100.0
+    lo, hi = ylo, 100.0
     for _ in range(40):
         mid = 0.5 * (lo + hi)
         if bound(mid) <= tol:
@@ -67,12 +67,37 @@ def far_threshold(n_ang, q, tol=1e-11):
     return hi
 
 
-def far_radius(row):
-    """(r_max, far-field distance) of a coil row as CCOIL_FP32C sets them (distance 0: the coil keeps its rule)."""
+FAR_MIN_Y = 1.3
+LOG_TOL = {"fp64": float(np.float32(37.5348)), "fp32c": float(np.float32(30.6267))}   # ln(2 / tol) of the terms
+
+
+def far_cost_floor(n_ang, n_thk, fast, log_tol):
+    """The library's cost floor of the far-field distance (units of r_max), restated: the series of
+    L(y) = 2 log_tol / ln(y^2) + 2 terms at 15 operations per term plus 200 against the rule's n_ang n_thk nodes at
+    55 (fast method) or 12 (slow) operations, the nearest y in [1.3, 2] where the series is not dearer (2 when the
+    rule costs 230 or less)."""
+    rule = float(n_ang) * n_thk * (55.0 if fast else 12.0)
+    if rule <= 230.0:
+        return 2.0
+    return min(2.0, max(FAR_MIN_Y, float(np.exp(15.0 * log_tol / (rule - 230.0)))))
+
+
+def far_distance(row, fp64=False, one_lane=True):
+    """(r_max, far-field distance) of a coil row as the library sets them (distance 0: the coil keeps its rule):
+    the accuracy bound at the precision's tolerance from the cost floor on for launches with one lane per point
+    (pairs, and field calls of 16,384 points or more in the series modes), from 2 otherwise."""
     R, a, b = row["R"], row["a"], row["b"]
     rmax = float(np.hypot(R + a, 0.5 * b if row["use_fast"] else 0.0))
-    y = far_threshold(row["n_ang"], rmax / (R + a))
+    ylo = (far_cost_floor(row["n_ang"], row["n_thk"], bool(row["use_fast"]), LOG_TOL["fp64" if fp64 else "fp32c"])
+           if one_lane else 2.0)
+    y = far_threshold(row["n_ang"], rmax / (R + a), 1e-14 if fp64 else 1e-11, ylo)
     return rmax, y * rmax
+
+
+def far_radius(row, n_points=1):
+    """(r_max, far-field distance) of a coil row in a CCOIL_FP32C field call of n_points points (distance 0: the coil
+    keeps its rule)."""
+    return far_distance(row, fp64=False, one_lane=n_points >= 16384)
 
 
 def fp64_pair_count(params, pts, width_floor=1e-3, rows=None):
@@ -91,7 +116,7 @@ def fp64_pair_count(params, pts, width_floor=1e-3, rows=None):
         w = max(c.a, c.b, width_floor * c.R)
         sel = (dr * dr + dz * dz < w * w) | (r * r < 0.01 * (r * r + z * z + c.R * c.R))
         if rows is not None:
-            _, dfar = far_radius(rows[k])
+            _, dfar = far_radius(rows[k], len(pts))
             if dfar > 0:
                 sel &= ~(r * r + z * z > dfar * dfar)
         n += int(np.sum(sel))

@@ -1,6 +1,6 @@
-"""ccoil_far_field_radius (host only) against the restatement of the far-field threshold's bound in helpers.py, over
-angular orders 1-100, thin and thick windings, fast and slow rules, both precisions that use the series; and the
-bound itself pinned on cases fixed by hand."""
+"""ccoil_far_field_radius (host only) against the restatement of the far-field distance in helpers.py (the accuracy
+bound searched from the cost floor on), over angular orders 1-100, thin and thick windings, fast and slow rules, both
+precisions that use the series; the bound and the cost floor pinned on cases fixed by hand."""
 import numpy as np
 import pytest
 
@@ -22,15 +22,31 @@ def _rows(rng, n):
     return rows
 
 
-@pytest.mark.parametrize("prec,tol", [(abi.FP32C, 1e-11), (abi.FP64, 1e-14)])
-def test_library_threshold_matches_the_bound(prec, tol):
+@pytest.mark.parametrize("prec", [abi.FP32C, abi.FP64])
+def test_library_threshold_matches_the_bound(prec):
+    """The distance is the accuracy bound's, searched from the cost floor on (a bisection over [floor, 100] whose
+    last step is 9e-11 of the range: equal to that)."""
     rows = _rows(np.random.default_rng(1), 400)
     rmax, dist = abi.far_field_radius(abi.CoilSet.from_rows(rows), prec)
     for row, rm, d in zip(rows, rmax, dist):
         R, a, b = row["R"], row["a"], row["b"]
         assert rm == pytest.approx(np.hypot(R + a, 0.5 * b if row["use_fast"] else 0.0), rel=1e-15)
-        y = H.far_threshold(row["n_ang"], rm / (R + a), tol)
-        assert d == pytest.approx(y * rm, rel=1e-12, abs=0.0), (row["n_ang"], y, d / rm)
+        _, dh = H.far_distance(row, fp64=prec == abi.FP64)
+        assert d == pytest.approx(dh, rel=1e-9, abs=0.0), (row["n_ang"], row["n_thk"], dh / rm, d / rm)
+
+
+def test_cost_floor_pinned_by_hand():
+    # FP64 terms: log_tol = 37.5348 (float), L(y) = 2 log_tol / ln(y^2) + 2 at 15 per term + 200.
+    # RECTANGULAR 80 x 20 nodes at 55: rule 88,000, y = exp(15 x 37.5348 / 87,770) = 1.0064: clamped to 1.3.
+    assert H.far_cost_floor(80, 20, True, H.LOG_TOL["fp64"]) == 1.3
+    # FILAMENT 80 x 1 at 12: rule 960, y = exp(563.02 / 730) = exp(0.77126) = 2.162: clamped to 2.
+    assert H.far_cost_floor(80, 1, False, H.LOG_TOL["fp64"]) == 2.0
+    # slow 40 x 3: rule 1,440, y = exp(563.02 / 1,210) = exp(0.465307) = 1.59251
+    assert H.far_cost_floor(40, 3, False, H.LOG_TOL["fp64"]) == pytest.approx(1.59251, abs=2e-5)
+    # slow 12 x 1: rule 144 <= 230: 2
+    assert H.far_cost_floor(12, 1, False, H.LOG_TOL["fp64"]) == 2.0
+    # at the floor the FP64 series needs int(2 x 37.5348 / ln 1.69) + 2 = 145 terms, within the 150 it carries
+    assert int(2 * H.LOG_TOL["fp64"] / np.log(1.3 ** 2)) + 2 == 145
 
 
 def test_other_precisions_never_use_the_series():

@@ -53,8 +53,8 @@ def _local_boundary_points(rng, c, row, n):
     # axis cone: r^2 = 0.01 (z^2 + R^2) / 0.99
     z_ax = rng.uniform(-3.0 * R, 3.0 * R, m)
     r_ax = np.sqrt(0.01 * (z_ax * z_ax + R * R) / 0.99)
-    # far-field distance
-    _, dfar = H.far_radius(row)
+    # far-field distance (of a call of n points)
+    _, dfar = H.far_radius(row, n)
     th = rng.uniform(0.0, np.pi, n - 2 * m)
     r_far, z_far = dfar * np.sin(th), dfar * np.cos(th)
     r = np.concatenate([r_sh, r_ax, r_far])

@@ -11,6 +11,8 @@ replaces the float rule) against the rule.
   - A mixed group at 65,536 points (the visit order on) against the FP64 rule away from the windings: the FP64
     series (CCOIL_FAR_FIELD) within 1e-9 per point; FP32C with the series at p99 <= 1e-6, max <= 1e-5.
   - CCOIL_FP64 with CCOIL_FAR_FIELD against the plain rule: within 1e-12 beyond the threshold, bitwise inside it.
+  - Calls of 16,384 points or more start the series at the coil's cost floor (down to 1.3 r_max, up to 150 terms):
+    FP32C within 1e-11 and FP64 within 1e-13 of the coil's field of the FP64 rule just beyond it.
 """
 import numpy as np
 import pytest
@@ -134,6 +136,37 @@ def test_fp64_series_matches_the_rule(ctx, typ, R, a, b):
             assert err <= 1e-12, (typ, k, part, err)
     inner = slice(0, 256)   # y = 1.5: inside twice r_max, the rule itself
     assert np.array_equal(got["B"][inner], ref["B"][inner])
+
+
+@pytest.mark.parametrize("typ,R,a,b", [("RECTANGULAR", 0.1, 0.05, 0.08), ("RECTANGULAR", 0.04, 0.2, 0.15),
+                                       ("THIN", 0.1, 0.0, 0.3), ("FLAT", 0.02, 0.2, 0.0)])
+def test_series_from_the_cost_floor(ctx, typ, R, a, b):
+    """Calls of 16,384 points or more (one lane per point) take the series from the coil's cost floor (down to
+    1.3 r_max) where its accuracy bound allows. Just beyond that distance and further out, for angular orders 80 to
+    16: CCOIL_FP32C within 1e-11 of the coil's field of the FP64 rule, CCOIL_FP64 with CCOIL_FAR_FIELD within 1e-13."""
+    rng = np.random.default_rng(23)
+    worst = {}
+    for n_ang, n_thk in ((80, 20), (48, 7), (32, 12), (16, 30)):
+        p, row = _coil(typ, R, a, b, n_ang, n_thk)
+        cs = abi.CoilSet.from_rows([row])
+        for prec, flags, fp64, tol in ((abi.FP32C, 0, False, 1e-11), (abi.FP64, abi.FAR_FIELD, True, 1e-13)):
+            rmax, dfar = H.far_distance(row, fp64=fp64, one_lane=True)
+            assert dfar > 0, (typ, n_ang)
+            ys = (dfar / rmax * 1.0001, dfar / rmax * 1.1, dfar / rmax * 1.4)
+            pts = np.concatenate([_shell_points(rng, row, y, 6000) for y in ys])
+            q = abi.A | abi.B | abi.G
+            ref, _ = ctx.fields(cs, pts, q, abi.FP64)
+            got, rep = ctx.fields(cs, pts, q, prec, flags)
+            assert rep.near_pairs == 0
+            for k in ("A", "B", "G"):
+                for part in range(3):
+                    sl = slice(6000 * part, 6000 * (part + 1))
+                    scale = np.max(np.linalg.norm(ref[k][sl].reshape(6000, -1), axis=1))
+                    err = np.max(np.linalg.norm((got[k][sl] - ref[k][sl]).reshape(6000, -1), axis=1)) / scale
+                    worst[(prec, n_ang)] = max(worst.get((prec, n_ang), 0.0), err)
+                    assert err <= tol, (typ, n_ang, n_thk, prec, round(dfar / rmax, 3), k, part, err)
+    print(f"\n{typ} R={R} a={a} b={b}: worst " + ", ".join(f"{'FP64' if k[0] == abi.FP64 else 'FP32C'}/{k[1]}: {v:.2g}"
+                                                      for k, v in worst.items()))
 
 
 def test_module_far_field_switch():

@@ -214,17 +214,19 @@ def test_coil_chunking_matches_single_pass(ctx, prec, flags, splits):
     """Smaller calls split a point's thickness rows over 4 or 16 lanes and the coils over grid rows (summed in
     order by field_reduce_kernel); a large call runs one lane and one pass per point. FP32C with the series runs the
     series-only kernel (one lane per point) from 16,384 points on, where the record pass takes the rule's pairs. All
-    give the same field to rounding."""
+    give the same field to rounding, except that FP32C with the series takes it from the coils' cost floor only with
+    one lane per point: a 256-point call keeps the float rule out to 2 r_max (agreement to FP32C's accuracy there)."""
     params, pts = W.group_workload(16, 600000, seed=13)
     rows, _ = H.gpu_args_rows(params)
     big, rep_big = _run(ctx, rows, pts, abi.A | abi.B | abi.G, prec, flags)
     mid, rep_mid = _run(ctx, rows, pts[:40000], abi.A | abi.B | abi.G, prec, flags)
     small, rep_small = _run(ctx, rows, pts[:256], abi.A | abi.B | abi.G, prec, flags)
     assert (rep_big.split, rep_mid.split, rep_small.split) == splits
+    small_tol = 1e-5 if (prec == abi.FP32C and not flags & abi.DIRECT) else 1e-12
     for key in ("A", "B", "G"):
-        for part in (small, mid):
+        for part, tol in ((small, small_tol), (mid, 1e-12)):
             e = H.rel_err_vec(part[key][:256], big[key][:256])
-            assert np.nanmax(e) < 1e-12, (key, np.nanmax(e))
+            assert np.nanmax(e) < tol, (key, np.nanmax(e))
 
 
 def test_fp32c_near_pass_is_deterministic(ctx):
