@@ -37,6 +37,7 @@ struct alignas(64) CoilDev {
     int32_t n_ang, n_thk, n_len;
     int32_t blk_ang, blk_thk, blk_len;
     int32_t near_key; // FP32C near pass: the coil's place in the evaluation order (most nodes first)
+    int32_t mp_terms; // highest order of the far-field series a pair of this call can use (0: no series)
 };
 
 // Gauss-Legendre tables for orders 1..100 at offset table_offset(n), generated in FP64 on the device.

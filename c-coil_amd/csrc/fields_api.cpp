@@ -97,6 +97,15 @@ double far_cost_floor(int n_ang, int n_thk, bool fast, double log_tol)
     return std::min(2.0, std::max(kFarMinY, y));
 }
 
+int far_terms(double y, bool fp64)
+{
+    if (!(y > 0.0)) return 0;
+    // multipole_terms at D = y r_max (it falls with D), in the same float arithmetic, one above for its fast log
+    const float lg = std::log((float)(y * y));
+    const int L = (int)(2.0f * (fp64 ? kMpLogTolFP64 : kMpLogTolFP32C) / lg) + 3;
+    return std::min(kMpTerms, std::max(4, L));
+}
+
 double far_distance(const CoilDev &c, bool fp64, bool cost_floor)
 {
     const double log_tol = fp64 ? kMpLogTolFP64 : kMpLogTolFP32C;
@@ -261,6 +270,7 @@ ccoil_status fields_prepare(ccoil_ctx ctx, const ccoil_coils *coils, const doubl
             // lane of the point's group, so it keeps the series from 2 r_max on
             const double y = far_distance(c, precision == CCOIL_FP64, L.split == 1);
             c.far_d2 = y > 0.0 ? (y * c.rmax) * (y * c.rmax) : 0.0;
+            c.mp_terms = far_terms(y, precision == CCOIL_FP64);
         }
 
     // The FP32C near pass sorts its (coil, point) records by near_key: coils with the most nodes first, so its

@@ -82,24 +82,29 @@ __device__ inline void chunk_moments(const CoilDev &c, const Tables &tab, double
     __syncthreads();
 }
 
+// Grid (coils, 3): block (c, k) sums the orders [64 k, 64 k + 64) of coil c, or zeroes them when the call's series
+// of that coil stops below them (mp_terms).
 __global__ __launch_bounds__(256) void multipole_moments_kernel(const CoilDev *coils, Tables tab, double *mp)
 {
     __shared__ double part[4][64];
-    const int ci = blockIdx.x, t = threadIdx.x;
-    if (ci == 0 && t < kMpStride) {   // recurrence coefficients of the Legendre polynomials
+    const int ci = blockIdx.x, chunk = blockIdx.y, t = threadIdx.x;
+    if (ci == 0 && chunk == 0 && t < kMpStride) {   // recurrence coefficients of the Legendre polynomials
         mp[t] = (2.0 * t + 5.0) / (t + 3.0);
         mp[kMpStride + t] = (t + 2.0) / (t + 3.0);
         mp[2 * kMpStride + t] = 2.0 * t + 5.0;
     }
     const CoilDev &c = coils[ci];
     double *out = mp + kMpHeader + (size_t)ci * kMpCoil;
-    if (!(c.far_d2 > 0.0)) {   // the coil never uses its series
-        for (int i = t; i < kMpCoil; i += 256) out[i] = 0.0;
+    const int l0 = 64 * chunk;
+    if (!(c.far_d2 > 0.0) || (chunk > 0 && c.mp_terms < l0)) {   // no series, or none of these orders
+        const int l = l0 + t;
+        if (t < 64 && l < kMpStride)
+            for (int k = 0; k < 4; ++k) out[k * kMpStride + l] = 0.0;
         return;
     }
-    chunk_moments<0>(c, tab, out, part);
-    chunk_moments<64>(c, tab, out, part);
-    chunk_moments<128>(c, tab, out, part);
+    if (chunk == 0) chunk_moments<0>(c, tab, out, part);
+    else if (chunk == 1) chunk_moments<64>(c, tab, out, part);
+    else chunk_moments<128>(c, tab, out, part);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -195,7 +200,7 @@ __global__ __launch_bounds__(256) void cell_place_kernel(const uint32_t *cell, i
 
 void launch_multipole_moments(hipStream_t s, const CoilDev *coils, int n_coils, const Tables &tab, double *mp)
 {
-    if (n_coils > 0) hipLaunchKernelGGL(multipole_moments_kernel, dim3(n_coils), dim3(256), 0, s, coils, tab, mp);
+    if (n_coils > 0) hipLaunchKernelGGL(multipole_moments_kernel, dim3(n_coils, 3), dim3(256), 0, s, coils, tab, mp);
 }
 
 size_t visit_order_bytes(int64_t n, int bits, size_t scan_bytes)

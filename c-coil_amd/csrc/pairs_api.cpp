@@ -92,6 +92,7 @@ ccoil_status build_pairs(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coi
         for (auto &c : hp) {
             const double y = far_distance(c, precision == CCOIL_FP64, true);
             c.far_d2 = y > 0.0 ? (y * c.rmax) * (y * c.rmax) : 0.0;
+            c.mp_terms = far_terms(y, precision == CCOIL_FP64);
         }
 
     // Pair table, longest work item first (LPT order of the persistent queue).
