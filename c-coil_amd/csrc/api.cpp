@@ -7,6 +7,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <thread>
 
 using namespace ccoil;
 
@@ -124,7 +126,8 @@ std::string pack_coils(const ccoil_coils *c, bool need_len, bool need_freq, std:
     if (need_len && (!c->n_len || !c->current || !c->turns)) return "pair coils need n_len, current and turns";
     if (need_freq && !c->frequency) return "E requested but coil frequency array is NULL";
     out.resize((size_t)c->n);
-    for (int64_t i = 0; i < c->n; ++i) {
+    // one coil's record, or the reason it has none
+    auto pack_one = [&](int64_t i) -> std::string {
         CoilDev d;
         std::memset(&d, 0, sizeof d);
         const int na = c->n_ang[i], nt = c->n_thk[i], nl = c->n_len ? c->n_len[i] : 1;
@@ -164,8 +167,42 @@ std::string pack_coils(const ccoil_coils *c, bool need_len, bool need_freq, std:
         d.far_d2 = 0.0;
         d.mp_terms = 0;
         out[(size_t)i] = d;
+        return "";
+    };
+    // ranges of coils on host threads; the error reported is the lowest coil's, as a serial loop would report
+    std::vector<std::pair<int64_t, std::string>> errs;
+    std::mutex m;
+    host_parallel_for(c->n, [&](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) {
+            std::string e = pack_one(i);
+            if (!e.empty()) {
+                std::lock_guard<std::mutex> lock(m);
+                errs.emplace_back(i, std::move(e));
+                return;
+            }
+        }
+    });
+    if (errs.empty()) return "";
+    return std::min_element(errs.begin(), errs.end())->second;
+}
+
+void host_parallel_for(int64_t n, const std::function<void(int64_t, int64_t)> &fn)
+{
+    constexpr int64_t kPerThread = 8192;
+    const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
+    const int64_t parts = std::min<int64_t>(std::min<int64_t>(16, hw), (n + kPerThread - 1) / kPerThread);
+    if (parts <= 1) {
+        if (n > 0) fn(0, n);
+        return;
     }
-    return "";
+    std::vector<std::thread> pool;
+    const int64_t per = (n + parts - 1) / parts;
+    for (int64_t k = 1; k < parts; ++k) {
+        const int64_t i0 = k * per, i1 = std::min(n, i0 + per);
+        if (i0 < i1) pool.emplace_back(fn, i0, i1);
+    }
+    fn(0, std::min(n, per));
+    for (auto &t : pool) t.join();
 }
 
 Tables ctx_tables(ccoil_ctx ctx)

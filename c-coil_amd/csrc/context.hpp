@@ -124,6 +124,8 @@ double far_cost_floor(int n_ang, int n_thk, bool fast, double log_tol);
 // precision's tolerance (fp64: kFarToleranceFP64, else kFarTolerance) from the cost floor on (cost_floor: launches
 // with one lane per point or secondary node), else from 2.
 double far_distance(const CoilDev &c, bool fp64, bool cost_floor);
+// Runs fn(i0, i1) over ranges that cover [0, n) once: on up to 16 host threads for large n, else on the caller.
+void host_parallel_for(int64_t n, const std::function<void(int64_t, int64_t)> &fn);
 // The highest order a series evaluated beyond y r_max can use (multipole_terms there, with a margin; 0 if y is 0).
 int far_terms(double y, bool fp64);
 // Enqueues the field kernel, the chunk reduction and (FP32C) the whole near pass on s, with no host

@@ -62,24 +62,40 @@ FieldShape field_shape_rule(ccoil_ctx ctx, int64_t n, int64_t n_coils, uint32_t 
 // rho_B^(-2 n_ang). Relative to the coil's field at D the error is at most 1000 y^2 rho_B^(-2 n_ang) (y = D / r_max;
 // the y^2 for the node terms that cancel down to the far field, the constant 10x above what a filament, a thin and a
 // thick coil measured in FP64 against the series at n_ang = 4..24 and y = 2..50). The threshold is the least y in
-// [ylo, 100] with that bound below tol: for n_ang >= 5 the bound falls monotonically there, so a bisection finds it.
+// [ylo, 100] with that bound below tol: for n_ang >= 5 the bound falls monotonically there, so g(y) = ln(bound / tol)
+// has one sign change, found by regula falsi with the Illinois modification (a few evaluations where a bisection
+// takes forty; a pair call sets one distance per primary) to within 1e-12 of y, returning the side where the bound
+// holds.
 double far_threshold(int n_ang, double q, double tol, double ylo)
 {
-    auto bound = [&](double y) {
+    auto g = [&](double y) {
         const double d = y - 1.0;
         const double eta = std::acosh(1.0 + d * d * q / (2.0 * y));
         const std::complex<double> xs(-1.0, 2.0 * eta / kPi);
         const std::complex<double> sq = std::sqrt(xs * xs - 1.0);
         const double rho = std::max(std::abs(xs + sq), std::abs(xs - sq));
-        return 1000.0 * y * y * std::pow(rho, -2.0 * n_ang);
+        return std::log(1000.0 / tol) + 2.0 * std::log(y) - 2.0 * n_ang * std::log(rho);
     };
     if (n_ang < 5) return 0.0;
-    if (bound(ylo) <= tol) return ylo;
-    if (bound(100.0) > tol) return 0.0;
-    double lo = ylo, hi = 100.0;
-    for (int it = 0; it < 40; ++it) {
-        const double mid = 0.5 * (lo + hi);
-        (bound(mid) <= tol ? hi : lo) = mid;
+    double lo = ylo, hi = 100.0, glo = g(lo), ghi = g(hi);
+    if (glo <= 0.0) return ylo;
+    if (ghi > 0.0) return 0.0;
+    int side = 0;
+    for (int it = 0; it < 200 && hi - lo > 1e-12 * hi; ++it) {
+        double y = (lo * ghi - hi * glo) / (ghi - glo);
+        if (!(y > lo && y < hi)) y = 0.5 * (lo + hi);
+        const double gy = g(y);
+        if (gy <= 0.0) {
+            hi = y;
+            ghi = gy;
+            if (side == -1) glo *= 0.5;
+            side = -1;
+        } else {
+            lo = y;
+            glo = gy;
+            if (side == 1) ghi *= 0.5;
+            side = 1;
+        }
     }
     return hi;
 }
@@ -461,11 +477,13 @@ extern "C" ccoil_status ccoil_far_field_radius(const ccoil_coils *coils, int32_t
     if (!hc.empty() && (!rmax || !distance))
         return fail(nullptr, CCOIL_ERR_ARGUMENT, "ccoil_far_field_radius: an output is NULL");
     const bool far = precision == CCOIL_FP32C || precision == CCOIL_FP64;
-    for (size_t i = 0; i < hc.size(); ++i) {
-        const CoilDev &c = hc[i];
-        rmax[i] = c.rmax;
-        distance[i] = far ? far_distance(c, precision == CCOIL_FP64, true) * c.rmax : 0.0;
-    }
+    host_parallel_for((int64_t)hc.size(), [&](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) {
+            const CoilDev &c = hc[(size_t)i];
+            rmax[i] = c.rmax;
+            distance[i] = far ? far_distance(c, precision == CCOIL_FP64, true) * c.rmax : 0.0;
+        }
+    });
     return CCOIL_OK;
 }
 

@@ -89,11 +89,14 @@ ccoil_status build_pairs(ccoil_ctx ctx, const ccoil_coils *prim, const ccoil_coi
     out.far = !(what & kPairGrad) && ((precision == CCOIL_FP32C && !(flags & CCOIL_DIRECT)) ||
                                       (precision == CCOIL_FP64 && (flags & CCOIL_FAR_FIELD)));
     if (out.far)
-        for (auto &c : hp) {
-            const double y = far_distance(c, precision == CCOIL_FP64, true);
-            c.far_d2 = y > 0.0 ? (y * c.rmax) * (y * c.rmax) : 0.0;
-            c.mp_terms = far_terms(y, precision == CCOIL_FP64);
-        }
+        host_parallel_for((int64_t)hp.size(), [&](int64_t i0, int64_t i1) {
+            for (int64_t i = i0; i < i1; ++i) {
+                CoilDev &c = hp[(size_t)i];
+                const double y = far_distance(c, precision == CCOIL_FP64, true);
+                c.far_d2 = y > 0.0 ? (y * c.rmax) * (y * c.rmax) : 0.0;
+                c.mp_terms = far_terms(y, precision == CCOIL_FP64);
+            }
+        });
 
     // Pair table, longest work item first (LPT order of the persistent queue).
     std::vector<PairDev> pd((size_t)n_pairs);
