@@ -498,10 +498,11 @@ def main():
         lout = {"B": torch.empty((len(lpts), 3), dtype=torch.float64, device=dev),
                 "G": torch.empty((len(lpts), 9), dtype=torch.float64, device=dev)}
         large = {}
-        for name, prec in (("fp64", abi.FP64), ("fp32c", abi.FP32C)):
+        for name, prec, lflags in (("fp64", abi.FP64, 0), ("fp64_far_field", abi.FP64, abi.FAR_FIELD),
+                                   ("fp32c", abi.FP32C, 0)):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            _, lrep = ctx.fields(lcs, lp, q, prec, abi.DEVICE_POINTERS, out=lout)
+            _, lrep = ctx.fields(lcs, lp, q, prec, abi.DEVICE_POINTERS | lflags, out=lout)
             torch.cuda.synchronize()
             ldt = time.perf_counter() - t0
             large["fields_" + name] = {"value": len(lrows) * len(lpts) / ldt, "ms": ldt * 1e3,
