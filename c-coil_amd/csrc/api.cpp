@@ -274,8 +274,10 @@ ccoil_status ccoil_create(int device, ccoil_ctx *out)
                        &ctx->gl_opc, &ctx->gl_ws, &ctx->gl_ws2, &ctx->gl_wcs};
     float **fp[7] = {&ctx->gl_xf, &ctx->gl_wf, &ctx->gl_cosf, &ctx->gl_omcf, &ctx->gl_wcosf, &ctx->gl_opcf,
                      &ctx->gl_wcsf};
-    for (auto p : dp)
-        if ((e = hipMalloc((void **)p, nd)) != hipSuccess) return bail(e, "hipMalloc(tables)");
+    for (auto p : dp)   // x and w also hold the series' length rule (kTableSizeXW)
+        if ((e = hipMalloc((void **)p, p == &ctx->gl_x || p == &ctx->gl_w ? sizeof(double) * kTableSizeXW : nd)) !=
+            hipSuccess)
+            return bail(e, "hipMalloc(tables)");
     for (auto p : fp)
         if ((e = hipMalloc((void **)p, nf)) != hipSuccess) return bail(e, "hipMalloc(tables)");
     launch_gl_tables(ctx->stream, ctx->gl_x, ctx->gl_w, ctx->gl_cos, ctx->gl_omc, ctx->gl_xf, ctx->gl_wf,

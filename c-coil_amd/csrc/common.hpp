@@ -8,6 +8,11 @@ namespace ccoil {
 
 constexpr int kMaxOrder = 100;                        // Legendre::maxLegendreOrder (LegendreMatrix.h:9)
 constexpr int kTableSize = kMaxOrder * (kMaxOrder + 1) / 2;
+// The far-field series' length nodes: a Gauss-Legendre rule of kSeriesLenNodes nodes (exact to degree 251), stored in
+// the x and w tables after the orders 1..kMaxOrder, at kSeriesLenOffset.
+constexpr int kSeriesLenNodes = 126;
+constexpr int kSeriesLenOffset = kTableSize;
+constexpr int kTableSizeXW = kTableSize + kSeriesLenNodes;
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kMuReduced = 1e-7;                    // mu0 / (4 pi), g_MiReduced (CalculateFieldsFast.cxx:10)
 constexpr double kGpuAxisRatio = 1e-5;                 // on-axis branch of the GPU gradient (FieldGGroup.cu:134)
@@ -40,7 +45,8 @@ struct alignas(64) CoilDev {
     int32_t mp_terms; // highest order of the far-field series a pair of this call can use (0: no series)
 };
 
-// Gauss-Legendre tables for orders 1..100 at offset table_offset(n), generated in FP64 on the device.
+// Gauss-Legendre tables for orders 1..100 at offset table_offset(n), generated in FP64 on the device (x and w also
+// hold the series' length rule at kSeriesLenOffset).
 struct Tables {
     const double *x;      // nodes, ascending
     const double *w;      // weights

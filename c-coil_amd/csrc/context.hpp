@@ -118,7 +118,7 @@ constexpr int64_t kRecordMaxPoints = (int64_t)1 << 20;
 bool field_record_pass(int32_t precision, uint32_t flags, int64_t n_points, bool fixed_arena, bool *rule_records);
 double far_threshold(int n_ang, double q, double tol, double ylo);
 // The nearest distance, in units of r_max, at which the series may replace a coil's rule on cost, in [kFarMinY, 2].
-constexpr double kFarMinY = 1.3;
+constexpr double kFarMinY = 1.17;
 double far_cost_floor(int n_ang, int n_thk, bool fast, double log_tol);
 // Distance, in units of r_max, beyond which the coil's series replaces its rule (0: never): far_threshold at the
 // precision's tolerance (fp64: kFarToleranceFP64, else kFarTolerance) from the cost floor on (cost_floor: launches

@@ -13,12 +13,15 @@
 namespace ccoil {
 
 // ---------------------------------------------------------------------------------------------
-// Gauss-Legendre tables in FP64 (Newton on P_n), one workgroup per order.
+// Gauss-Legendre tables in FP64 (Newton on P_n), one workgroup per order; the last workgroup writes the far-field
+// series' length rule (kSeriesLenNodes nodes) into x and w at kSeriesLenOffset.
 // ---------------------------------------------------------------------------------------------
 __global__ void gl_table_kernel(double *x, double *w, double *cosv, double *omc, float *xf, float *wf,
                                 float *cosf, float *omcf, double *rcos, double *rsin)
 {
-    const int n = blockIdx.x + 1;
+    static_assert(kSeriesLenNodes <= 128, "one node pair per thread of 64");
+    const bool series_len = blockIdx.x == kMaxOrder;
+    const int n = series_len ? kSeriesLenNodes : blockIdx.x + 1;
     const int i = threadIdx.x;
     if (i >= (n + 1) / 2) return;
     double z = cos(kPi * (i + 0.75) / (n + 0.5));
@@ -46,7 +49,7 @@ __global__ void gl_table_kernel(double *x, double *w, double *cosv, double *omc,
     }
     dp = (n == 1) ? 1.0 : n * (z * p1 - p0) / (z * z - 1.0);
     const double wi = 2.0 / ((1.0 - z * z) * dp * dp);
-    const int off = table_offset(n);
+    const int off = series_len ? kSeriesLenOffset : table_offset(n);
     int hi = n - 1 - i, lo = i;
     double xh = z, xl = -z;
     if (hi == lo) { xh = 0.0; xl = 0.0; }
@@ -57,6 +60,7 @@ __global__ void gl_table_kernel(double *x, double *w, double *cosv, double *omc,
         const double xx = xv[k];
         x[e] = xx;
         w[e] = wi;
+        if (series_len) continue;   // the other tables hold the orders 1..kMaxOrder only
         // cos(pi/2 (1 + x)) = -sin(pi x / 2): exactly antisymmetric in x.
         const double cv = -sinpi(0.5 * xx);
         const double sv = sinpi(0.25 * (1.0 + xx));
@@ -269,7 +273,7 @@ __global__ __launch_bounds__(256) void field_reduce_kernel(FieldParams p)
 void launch_gl_tables(hipStream_t s, double *x, double *w, double *cosv, double *omc, float *xf, float *wf,
                       float *cosf, float *omcf, double *rcos, double *rsin)
 {
-    hipLaunchKernelGGL(gl_table_kernel, dim3(kMaxOrder), dim3(64), 0, s, x, w, cosv, omc, xf, wf, cosf, omcf,
+    hipLaunchKernelGGL(gl_table_kernel, dim3(kMaxOrder + 1), dim3(64), 0, s, x, w, cosv, omc, xf, wf, cosf, omcf,
                        rcos, rsin);
 }
 

@@ -12,7 +12,7 @@
 //
 // The rings are the field rule's own: its thickness nodes (the rule's Gauss-Legendre nodes over [R, R + a], so the
 // series reproduces the rule's thickness sum exactly, whatever its order), and over the length either the exact
-// average (the fast rule integrates the length in closed form; 76 Gauss-Legendre nodes are exact for l <= 150) or
+// average (the fast rule integrates the length in closed form; 126 Gauss-Legendre nodes are exact for l <= 250) or
 // the mid-plane alone (the slow rule's flat sheet). What the series leaves out is then the angular rule's own error
 // and the truncation. With s = sin theta, u = 1 / D and q_l = (r_max / D)^(l+1), and the moments normalised as
 // m_l = mu0 M_l / r_max^(l+1):
@@ -29,12 +29,14 @@
 
 namespace ccoil {
 
-constexpr int kMpTerms = 150;               // highest order l
-constexpr int kMpStride = 160;              // doubles per coefficient array (index l)
+constexpr int kMpTerms = 250;               // highest order l (below 256, the range of wave_max_small)
+constexpr int kMpStride = 256;              // doubles per coefficient array (index l), one per thread of a block
 constexpr int kMpCoil = 4 * kMpStride;      // per coil: m_l / l, m_l, (l + 1) m_l, (l + 1)(l + 2) m_l
 constexpr int kMpHeader = 3 * kMpStride;    // recurrence coefficients (2l + 5) / (l + 3), (l + 2) / (l + 3), 2l + 5
-constexpr int kMpLenNodes = 76;             // Gauss-Legendre nodes over the length of a fast-rule coil: exact to
-                                            // degree 151, and V_l has degree l - 1 in the ring's height
+constexpr int kMpLenNodes = kSeriesLenNodes; // Gauss-Legendre nodes over the length of a fast-rule coil: exact to
+                                             // degree 251, and V_l has degree l - 1 in the ring's height
+static_assert(kMpTerms < 256 && kMpTerms < kMpStride && kMpStride <= 256, "series orders and their arrays");
+static_assert(2 * kMpLenNodes - 1 >= kMpTerms - 1, "the length rule integrates V_l exactly");
 
 // Number of terms for a point at distance D from a coil whose winding lies within r_max: the truncation falls like
 // (r_max / D)^L with a constant below 2, so L = ln(2 / tol) / ln(D / r_max), between 4 and kMpTerms; log_tol =

@@ -13,6 +13,9 @@
 
 namespace ccoil {
 
+constexpr int kMpChunks = kMpStride / 64;   // blocks of 64 orders per coil
+static_assert(kMpChunks == 4, "multipole_moments_kernel dispatches four chunks");
+
 __device__ inline double wave_sum(double v)
 {
 #pragma unroll
@@ -50,7 +53,7 @@ __device__ inline void chunk_moments(const CoilDev &c, const Tables &tab, double
     const double inv = 1.0 / c.rmax;
     const int nt = c.n_thk, nz = c.use_fast ? kMpLenNodes : 1;
     const double *xt = tab.x + table_offset(nt), *wt = tab.w + table_offset(nt);
-    const double *xz = tab.x + table_offset(kMpLenNodes), *wz = tab.w + table_offset(kMpLenNodes);
+    const double *xz = tab.x + kSeriesLenOffset, *wz = tab.w + kSeriesLenOffset;
     double acc[64];
 #pragma unroll
     for (int l = 0; l < 64; ++l) acc[l] = 0.0;
@@ -82,8 +85,8 @@ __device__ inline void chunk_moments(const CoilDev &c, const Tables &tab, double
     __syncthreads();
 }
 
-// Grid (coils, 3): block (c, k) sums the orders [64 k, 64 k + 64) of coil c, or zeroes them when the call's series
-// of that coil stops below them (mp_terms).
+// Grid (coils, kMpChunks): block (c, k) sums the orders [64 k, 64 k + 64) of coil c, or zeroes them when the call's
+// series of that coil stops below them (mp_terms).
 __global__ __launch_bounds__(256) void multipole_moments_kernel(const CoilDev *coils, Tables tab, double *mp)
 {
     __shared__ double part[4][64];
@@ -104,7 +107,8 @@ __global__ __launch_bounds__(256) void multipole_moments_kernel(const CoilDev *c
     }
     if (chunk == 0) chunk_moments<0>(c, tab, out, part);
     else if (chunk == 1) chunk_moments<64>(c, tab, out, part);
-    else chunk_moments<128>(c, tab, out, part);
+    else if (chunk == 2) chunk_moments<128>(c, tab, out, part);
+    else chunk_moments<192>(c, tab, out, part);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -200,7 +204,8 @@ __global__ __launch_bounds__(256) void cell_place_kernel(const uint32_t *cell, i
 
 void launch_multipole_moments(hipStream_t s, const CoilDev *coils, int n_coils, const Tables &tab, double *mp)
 {
-    if (n_coils > 0) hipLaunchKernelGGL(multipole_moments_kernel, dim3(n_coils, 3), dim3(256), 0, s, coils, tab, mp);
+    if (n_coils > 0)
+        hipLaunchKernelGGL(multipole_moments_kernel, dim3(n_coils, kMpChunks), dim3(256), 0, s, coils, tab, mp);
 }
 
 size_t visit_order_bytes(int64_t n, int bits, size_t scan_bytes)

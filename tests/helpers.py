@@ -67,14 +67,14 @@ def far_threshold(n_ang, q, tol=1e-11, ylo=2.0):
     return hi
 
 
-FAR_MIN_Y = 1.3
+FAR_MIN_Y = 1.17
 LOG_TOL = {"fp64": float(np.float32(37.5348)), "fp32c": float(np.float32(30.6267))}   # ln(2 / tol) of the terms
 
 
 def far_cost_floor(n_ang, n_thk, fast, log_tol):
     """The library's cost floor of the far-field distance (units of r_max), restated: the series of
     L(y) = 2 log_tol / ln(y^2) + 2 terms at 15 operations per term plus 200 against the rule's n_ang n_thk nodes at
-    55 (fast method) or 12 (slow) operations, the nearest y in [1.3, 2] where the series is not dearer (2 when the
+    55 (fast method) or 12 (slow) operations, the nearest y in [1.17, 2] where the series is not dearer (2 when the
     rule costs 230 or less)."""
     rule = float(n_ang) * n_thk * (55.0 if fast else 12.0)
     if rule <= 230.0:

@@ -11,7 +11,7 @@ replaces the float rule) against the rule.
   - A mixed group at 65,536 points (the visit order on) against the FP64 rule away from the windings: the FP64
     series (CCOIL_FAR_FIELD) within 1e-9 per point; FP32C with the series at p99 <= 1e-6, max <= 1e-5.
   - CCOIL_FP64 with CCOIL_FAR_FIELD against the plain rule: within 1e-12 beyond the threshold, bitwise inside it.
-  - Calls of 16,384 points or more start the series at the coil's cost floor (down to 1.3 r_max, up to 150 terms):
+  - Calls of 16,384 points or more start the series at the coil's cost floor (down to 1.17 r_max, up to 250 terms):
     FP32C within 1e-11 and FP64 within 1e-13 of the coil's field of the FP64 rule just beyond it.
 """
 import numpy as np
@@ -142,7 +142,7 @@ def test_fp64_series_matches_the_rule(ctx, typ, R, a, b):
                                        ("THIN", 0.1, 0.0, 0.3), ("FLAT", 0.02, 0.2, 0.0)])
 def test_series_from_the_cost_floor(ctx, typ, R, a, b):
     """Calls of 16,384 points or more (one lane per point) take the series from the coil's cost floor (down to
-    1.3 r_max) where its accuracy bound allows. Just beyond that distance and further out, for angular orders 80 to
+    1.17 r_max) where its accuracy bound allows. Just beyond that distance and further out, for angular orders 80 to
     16: CCOIL_FP32C within 1e-11 of the coil's field of the FP64 rule, CCOIL_FP64 with CCOIL_FAR_FIELD within 1e-13."""
     rng = np.random.default_rng(23)
     worst = {}
