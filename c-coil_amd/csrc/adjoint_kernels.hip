@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kGradBlock) void field_grad_kernel(GradParams p)
         // local Cartesian B, gradient (the same on-axis rule as the forward path) and position
         const double Bl[3] = {L.Bh * cph, L.Bh * sph, L.Bz};
         double G[9];
-        if (r / c.R <= kGpuAxisRatio) {
+        if (r <= kGpuAxisRatio * c.R) {
             G[0] = L.RR; G[1] = 0.0; G[2] = 0.0;
             G[3] = 0.0; G[4] = L.RR; G[5] = 0.0;
             G[6] = 0.0; G[7] = 0.0; G[8] = L.ZZ;

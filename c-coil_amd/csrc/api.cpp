@@ -164,6 +164,7 @@ std::string pack_coils(const ccoil_coils *c, bool need_len, bool need_freq, std:
         d.adj_scale = 1.0;
         d.ni = c->J[i] * d.a * d.b;
         d.rmax = std::hypot(d.R + d.a, d.use_fast ? 0.5 * d.b : 0.0);   // the slow rule's winding is a flat sheet
+        d.inv_rmax2 = 1.0 / (d.rmax * d.rmax);
         d.far_d2 = 0.0;
         d.mp_terms = 0;
         out[(size_t)i] = d;

@@ -38,6 +38,7 @@ struct alignas(64) CoilDev {
     double ni;               // N I: the coil's total current (J a b)
     double rmax;             // radius of the sphere about the centre that holds the winding, |(R + a, b / 2)|
     double far_d2;           // squared distance beyond which the far-field series replaces the rule (0: never)
+    double inv_rmax2;        // 1 / rmax^2
     int32_t use_fast;
     int32_t n_ang, n_thk, n_len;
     int32_t blk_ang, blk_thk, blk_len;
@@ -88,6 +89,31 @@ __device__ inline void mat_vec(const double *M, const double *v, double *o)
     o[2] = M[6] * v[0] + M[7] * v[1] + M[8] * v[2];
 }
 
+// 1 / x for a finite positive normal x: the hardware estimate (v_rcp_f64, about 2^-23 relative) refined by two
+// Newton steps on f(y) = 1 / y - x, y <- y + y (1 - x y), each of which squares the relative error; within an ulp or
+// two of the quotient and a fixed sequence of operations, so the same in every kernel.
+__device__ inline double rcp_newton(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
+// 1 / sqrt(x) for a finite positive normal x: v_rsq_f64's estimate refined by two Newton steps on f(y) = 1 / y^2 - x,
+// y <- y + (y / 2) (1 - x y^2).
+__device__ inline double rsqrt_newton(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double e = fma(-(x * y), y, 1.0);
+        y = fma(0.5 * y, e, y);
+    }
+    return y;
+}
+
 // Point -> coil-local cylindrical (z, r) and the azimuth's cos/sin (cos(atan2(y,x)) = x/r).
 // Every operation is rounded on its own (no FMA contraction), so each kernel that classifies a (coil, point) pair
 // with is_near gets bitwise the same z and r from the same inputs, whatever the inlining context.
@@ -102,7 +128,7 @@ __device__ inline void to_local(const CoilDev &c, double px, double py, double p
     z = l[2];
     r = sqrt(l[0] * l[0] + l[1] * l[1]);
     if (r > 0.0) {
-        double ir = 1.0 / r;
+        const double ir = rcp_newton(r);
         cph = l[0] * ir;
         sph = l[1] * ir;
     } else {
@@ -131,7 +157,7 @@ __device__ inline void accumulate_global(const CoilDev &c, const Local &L, doubl
     }
     if (q & (Q_G | Q_GJ)) {
         double g[9];
-        if (r / c.R <= kGpuAxisRatio) {
+        if (r <= kGpuAxisRatio * c.R) {   // r / R <= 1e-5 without the division
             g[0] = L.RR; g[1] = 0.0; g[2] = 0.0;
             g[3] = 0.0; g[4] = L.RR; g[5] = 0.0;
             g[6] = 0.0; g[7] = 0.0; g[8] = L.ZZ;

@@ -141,7 +141,7 @@ void field_kernel(FieldParams p)
             if (__ballot(far)) {
                 // the loop runs to the wave's largest term count (uniform bounds and coefficient loads); each lane
                 // stops adding at its own count, so its value does not depend on the other points of its wave
-                const int own = far ? multipole_terms(r * r + z * z, c.rmax,
+                const int own = far ? multipole_terms(r * r + z * z, c.inv_rmax2,
                                                       PREC == kFP64 ? kMpLogTolFP64 : kMpLogTolFP32C) : 0;
                 const int terms = wave_max_small(own), lo = wave_min_small(own, far);
                 if (far && lane == 0)
@@ -214,7 +214,7 @@ void field_series_kernel(FieldParams p)
                 if (m && lane == __builtin_ctzll(m)) atomicAdd(&key_counts[k * cpc + (ci - c0)], (unsigned)__popcll(m));
             }
         if (__ballot(far)) {
-            const int own = far ? multipole_terms(r * r + z * z, c.rmax,
+            const int own = far ? multipole_terms(r * r + z * z, c.inv_rmax2,
                                                   PREC == kFP64 ? kMpLogTolFP64 : kMpLogTolFP32C) : 0;
             const int terms = wave_max_small(own), lo = wave_min_small(own, far);
             if (far) {

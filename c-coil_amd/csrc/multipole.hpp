@@ -43,10 +43,10 @@ static_assert(2 * kMpLenNodes - 1 >= kMpTerms - 1, "the length rule integrates V
 // ln(2 / tol): kMpLogTolFP64 (tol = 1e-16) or kMpLogTolFP32C (1e-13, far below the float rule's own rounding).
 constexpr float kMpLogTolFP64 = 37.5348f;
 constexpr float kMpLogTolFP32C = 30.6267f;
-__device__ inline int multipole_terms(double D2, double rmax, float log_tol)
+__device__ inline int multipole_terms(double D2, double inv_rmax2, float log_tol)
 {
-    const float lg = __logf((float)(D2 / (rmax * rmax)));   // 2 ln(D / r_max) > 2 ln 2
-    const int L = (int)(2.0f * log_tol / lg) + 2;
+    const float lg = __logf((float)(D2 * inv_rmax2));   // 2 ln(D / r_max) > 2 ln 1.17
+    const int L = (int)(2.0f * log_tol * __builtin_amdgcn_rcpf(lg)) + 2;
     return L < 4 ? 4 : (L > kMpTerms ? kMpTerms : L);
 }
 
@@ -75,7 +75,7 @@ __device__ inline void eval_multipole(const double *mp, const double *rec, doubl
                                       int lo, int own, Local &out)
 {
     const double D2 = r * r + z * z;
-    const double u = 1.0 / sqrt(D2);
+    const double u = rsqrt_newton(D2);
     const double c = z * u, s = r * u, qr = rmax * u;
     const double *mA = mp, *mB = mp + kMpStride, *mZ = mp + 2 * kMpStride, *mZZ = mp + 3 * kMpStride;
     // P_(l+1), P_(l+2) and P'_l, P'_(l+1), P'_(l+2), advanced together from l = 1
@@ -130,7 +130,7 @@ __device__ inline void eval_multipole_pair(const double *mp, const double *rec, 
                                            int lo, int own, Local &out, double &a_over_r)
 {
     const double D2 = r * r + z * z;
-    const double u = 1.0 / sqrt(D2);
+    const double u = rsqrt_newton(D2);
     const double c = z * u, s = r * u, qr = rmax * u;
     const double *mA = mp, *mB = mp + kMpStride, *mZ = mp + 2 * kMpStride;
     double P1 = 0.5 * (3.0 * c * c - 1.0), P2 = 0.5 * c * (5.0 * c * c - 3.0);

@@ -241,7 +241,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void pa
         auto series = [&](const Node &n, bool far) {
             if constexpr (!GR) {
                 if (!__ballot(far)) return;
-                const int own = far ? multipole_terms(n.z * n.z + n.r * n.r, pc.rmax,
+                const int own = far ? multipole_terms(n.z * n.z + n.r * n.r, pc.inv_rmax2,
                                                       PREC == 1 ? kMpLogTolFP32C : kMpLogTolFP64) : 0;
                 const int terms = wave_max_small(own), lo = wave_min_small(own, far);
                 if (far && rg == 0) {
