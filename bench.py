@@ -52,9 +52,9 @@ if PKG not in sys.path:
     sys.path.insert(0, PKG)
 
 METRIC = "coil·point B+∇B evals/s and M-pairs/s, FP32-comp vs FP64 at ≤1e-6 rel err"
-# committed rocprofv3 --pmc summaries of field_kernel per mode (HBM bytes per launch)
+# committed rocprofv3 --pmc summaries per mode (HBM bytes per launch of each kernel they measure)
 PMC_FILES = {"fp64": os.path.join(ROOT, "profiles", "field_kernel_fp64_group128_points1M_pmc.md"),
-             "fp64_far_field": os.path.join(ROOT, "profiles", "near_eval_kernel_fp64_far_field_group128_points1M_pmc.md")}
+             "fp64_far_field": os.path.join(ROOT, "profiles", "field_pass_fp64_far_field_group128_points1M_pmc.md")}
 
 
 def log(*a):
@@ -77,14 +77,16 @@ def stats(e):
             "n": int(e.size)}
 
 
-def pmc_traffic(mode):
-    """HBM bytes per launch of the field kernel of a mode from its committed rocprofv3 --pmc summary, if present."""
+def pmc_traffic(mode, kernel):
+    """HBM bytes per launch of a mode's kernel from its committed rocprofv3 --pmc summary (lines
+    `traffic_bytes_per_launch[kernel] = bytes`), if present."""
     path = PMC_FILES.get(mode)
     if path is None or not os.path.exists(path):
         return None
+    key = f"traffic_bytes_per_launch[{kernel}]"
     with open(path) as f:
         for line in f:
-            if line.startswith("traffic_bytes_per_launch"):
+            if line.startswith(key):
                 return float(line.split("=")[1].split()[0])
     return None
 
@@ -251,7 +253,7 @@ def main():
         if "field" in parts:
             rl = part_roofline("field_kernel", parts["field"], main_ms)
         else:
-            ser = part_roofline("field_kernel (the series)", parts["series"], main_ms)
+            ser = part_roofline("field_series_kernel", parts["series"], main_ms)
             rec = part_roofline("near_eval_kernel (rule records)", parts["records"], rec_ms)
             rl = dict(rec if rec_ms >= main_ms else ser)
             tot = tuple(a + b for a, b in zip(parts["series"], parts["records"]))
@@ -272,7 +274,7 @@ def main():
     log(f"{head_name}: {dt * 1e3:.2f} ms/step, field kernel {main_ms:.2f} ms, record pass {rec_ms:.2f} ms, "
         f"all kernels {all_ms:.2f} ms")
     rl = roofline(head_name, main_ms, rec_ms, all_ms)
-    rl["traffic"] = pmc_traffic(head_name)
+    rl["traffic"] = pmc_traffic(head_name, rl["kernel"].split()[0])
     result = {"metric": METRIC, "value": value, "unit": "coil·point evals/s", "n_gpus": 1,
               "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
               "scaling": "weak", "vs_baseline": None, "dtype": "f32" if head_prec == abi.FP32C else "f64",

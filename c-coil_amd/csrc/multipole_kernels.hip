@@ -14,6 +14,8 @@
 namespace ccoil {
 
 constexpr int kMpChunks = kMpStride / 64;   // blocks of 64 orders per coil
+constexpr int kMomThreads = 512;            // threads of a moments block (8 waves; at most 2 per SIMD by registers)
+constexpr int kMomWaves = kMomThreads / 64;
 static_assert(kMpChunks == 4, "multipole_moments_kernel dispatches four chunks");
 
 __device__ inline double wave_sum(double v)
@@ -45,7 +47,7 @@ __device__ inline void ring_moments(double w, double z0, double c2, double *acc)
     }
 }
 
-// The moments of orders [L0, L0 + 64) of coil c into out (a block of 256 threads over the coil's rings).
+// The moments of orders [L0, L0 + 64) of coil c into out (a block of kMomThreads threads over the coil's rings).
 template <int L0>
 __device__ inline void chunk_moments(const CoilDev &c, const Tables &tab, double *out, double (*part)[64])
 {
@@ -57,7 +59,7 @@ __device__ inline void chunk_moments(const CoilDev &c, const Tables &tab, double
     double acc[64];
 #pragma unroll
     for (int l = 0; l < 64; ++l) acc[l] = 0.0;
-    for (int k = t; k < nt * nz; k += 256) {
+    for (int k = t; k < nt * nz; k += kMomThreads) {
         const int j = k / nz, i = k - j * nz;
         // ring (rho, z0) in units of r_max, weight: its share of the current (the weights sum to 1)
         const double rho = (c.R + 0.5 * c.a * (1.0 + xt[j])) * inv;
@@ -76,7 +78,11 @@ __device__ inline void chunk_moments(const CoilDev &c, const Tables &tab, double
     if (t < 64 && L0 + t < kMpStride) {
         const int l = L0 + t;
         double m = 0.0;
-        if (l >= 1 && l <= kMpTerms) m = ni * (((part[0][t] + part[1][t]) + part[2][t]) + part[3][t]) / (2.0 * (l + 1));
+        if (l >= 1 && l <= kMpTerms) {
+            const double s = ((part[0][t] + part[1][t]) + (part[2][t] + part[3][t])) +
+                             ((part[4][t] + part[5][t]) + (part[6][t] + part[7][t]));
+            m = ni * s / (2.0 * (l + 1));
+        }
         out[l] = l >= 1 ? m / l : 0.0;
         out[kMpStride + l] = m;
         out[2 * kMpStride + l] = (l + 1) * m;
@@ -87,9 +93,10 @@ __device__ inline void chunk_moments(const CoilDev &c, const Tables &tab, double
 
 // Grid (coils, kMpChunks): block (c, k) sums the orders [64 k, 64 k + 64) of coil c, or zeroes them when the call's
 // series of that coil stops below them (mp_terms).
-__global__ __launch_bounds__(256) void multipole_moments_kernel(const CoilDev *coils, Tables tab, double *mp)
+__global__ __launch_bounds__(kMomThreads) void multipole_moments_kernel(const CoilDev *coils, Tables tab, double *mp)
 {
-    __shared__ double part[4][64];
+    static_assert(kMomWaves == 8, "chunk_moments sums eight wave partials");
+    __shared__ double part[kMomWaves][64];
     const int ci = blockIdx.x, chunk = blockIdx.y, t = threadIdx.x;
     if (ci == 0 && chunk == 0 && t < kMpStride) {   // recurrence coefficients of the Legendre polynomials
         mp[t] = (2.0 * t + 5.0) / (t + 3.0);
@@ -205,7 +212,8 @@ __global__ __launch_bounds__(256) void cell_place_kernel(const uint32_t *cell, i
 void launch_multipole_moments(hipStream_t s, const CoilDev *coils, int n_coils, const Tables &tab, double *mp)
 {
     if (n_coils > 0)
-        hipLaunchKernelGGL(multipole_moments_kernel, dim3(n_coils, kMpChunks), dim3(256), 0, s, coils, tab, mp);
+        hipLaunchKernelGGL(multipole_moments_kernel, dim3(n_coils, kMpChunks), dim3(kMomThreads), 0, s, coils, tab,
+                           mp);
 }
 
 size_t visit_order_bytes(int64_t n, int bits, size_t scan_bytes)

@@ -11,10 +11,30 @@
 // its slot; near_add_kernel adds a point's slots to the outputs in coil order. A slot's value depends only on its
 // (coil, point), and every sum has a fixed order, so FP32C results are bitwise reproducible although the order of the
 // records within a coil is not. Nothing here is read back by the host before the work is done.
+#include <cstddef>
+
 #include "kernels.hpp"
 #include "integrands.hpp"
 
 namespace ccoil {
+
+// Whether a (coil, point) pair may be a record: false only where the point lies beyond the coil's far-field sphere by
+// more than a margin, tested in the global frame (the rotation to the coil's frame keeps lengths up to a few ulps,
+// far below the margin). The exact class is record_kind's; this lets a wave skip the coil's frame transform when none
+// of its points can be a record of that coil. The coil's centre and far_d2 are read through the constant address
+// space: the records do not change during a launch, so these are scalar loads at the wave-uniform index although the
+// loops that call this also store to global memory (through a generic pointer their loads would be vector loads, each
+// a full memory round trip before the test).
+__device__ inline bool maybe_record(const CoilDev *coils, int ci, double px, double py, double pz)
+{
+    typedef const __attribute__((address_space(4))) double cdouble;
+    cdouble *f = (cdouble *)(const void *)(coils + ci);
+    const double far_d2 = f[offsetof(CoilDev, far_d2) / sizeof(double)];
+    if (!(far_d2 > 0.0)) return true;
+    const int o = offsetof(CoilDev, pos) / sizeof(double);
+    const double dx = px - f[o], dy = py - f[o + 1], dz = pz - f[o + 2];
+    return dx * dx + dy * dy + dz * dz <= far_d2 * (1.0 + 1e-9);
+}
 
 // Calls with fewer points than kNearSplitPoints split each near record's thickness rows over kNearSplit lanes.
 constexpr int64_t kNearSplitPoints = (int64_t)1 << 17;
@@ -31,6 +51,7 @@ __global__ __launch_bounds__(256) void near_count_kernel(FieldParams p)
     const double px = p.pts[3 * pi], py = p.pts[3 * pi + 1], pz = p.pts[3 * pi + 2];
     const int lane = threadIdx.x & 63;
     for (int ci = 0; ci < p.n_coils; ++ci) {
+        if (!__ballot(maybe_record(p.coils, ci, px, py, pz))) continue;   // no point of the wave can be a record
         const CoilDev &c = p.coils[ci];
         double z, r, cph, sph;
         to_local(c, px, py, pz, z, r, cph, sph);
@@ -88,6 +109,7 @@ __global__ __launch_bounds__(256) void near_fill_kernel(FieldParams p)
     const int lane = threadIdx.x & 63;
     bool bad = false;
     for (int ci = 0; ci < p.n_coils; ++ci) {
+        if (!__ballot(maybe_record(p.coils, ci, px, py, pz))) continue;   // no point of the wave can be a record
         const CoilDev &c = p.coils[ci];
         double z, r, cph, sph;
         to_local(c, px, py, pz, z, r, cph, sph);
