@@ -103,7 +103,7 @@ __device__ inline void eval_multipole(const double *mp, const double *rec, doubl
         d2 = d3;
     };
     int l = 1;
-#pragma unroll 2
+#pragma unroll 4
     for (; l <= lo; ++l) {
         q *= qr;
         term(l, q);
