@@ -25,7 +25,8 @@ The far-field series (a field call's (coil, point) pairs beyond the coil's far-f
 counted per pair from its FP64 loop: per term 13 flops for B (the next power of r_max / D, two sums with their
 coefficient products, the Legendre recurrence of P and P') and 18 with the gradient (two more sums), and per pair
 20 (B) or 30 (B + gradient) flops and 2 special ops (1 / sqrt(D^2), the log of the term count) for the set-up and
-the outputs. The term count per pair is the kernel's: int(2 ln(2 / tol) / ln(D^2 / r_max^2)) + 2 in [4, 60].
+the outputs. The term count per pair is the kernel's: int(2 ln(2 / tol) / ln(D^2 / r_max^2)) + 2 in [4, SERIES_MAX_TERMS]
+(multipole.hpp's kMpTerms).
 """
 
 FAST_B = (3, 19)
@@ -58,6 +59,7 @@ def field_work(rows, n_points, with_gradient=True):
 
 SERIES_B = (2, 20, 13)    # special ops and flops per pair, flops per term
 SERIES_BG = (2, 30, 18)
+SERIES_MAX_TERMS = 250    # kMpTerms
 LOG_TOL_FP64 = 37.5348    # ln(2 / 1e-16)
 LOG_TOL_FP32C = 30.6267   # ln(2 / 1e-13)
 
@@ -94,7 +96,7 @@ def series_field_work(rows, pts, dist, rmax, log_tol, with_gradient=True, near_p
         out["rule_flops"] += float(nr) * nodes * f
         out["rule_special"] += float(nr) * nodes * t
         if far.any():
-            L = np.clip((2.0 * log_tol / np.log(D2[far] / (rm * rm))).astype(np.int64) + 2, 4, 60)
+            L = np.clip((2.0 * log_tol / np.log(D2[far] / (rm * rm))).astype(np.int64) + 2, 4, SERIES_MAX_TERMS)
             out["series_flops"] += float(far.sum()) * sf + float(L.sum()) * st
             out["series_special"] += float(far.sum()) * sp
         out["rule_pairs"] += nr

@@ -73,3 +73,16 @@ def test_bound_pinned_by_hand():
     # the threshold grows as the tolerance tightens and falls as the angular order grows
     assert H.far_threshold(16, 1.0, 1e-14) > H.far_threshold(16, 1.0, 1e-11) >= 2.0
     assert H.far_threshold(14, 1.0, 1e-11) > H.far_threshold(18, 1.0, 1e-11)
+
+
+def test_restatements_follow_the_device_constants():
+    """The host-side restatements (helpers.FAR_MIN_Y, the roofline model's term cap) carry the device's constants
+    (kFarMinY in context.hpp, kMpTerms in multipole.hpp), read from the sources."""
+    import os
+    import re
+    import roofline_model as RM
+    src = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "c-coil_amd", "csrc")
+    ctx = open(os.path.join(src, "context.hpp")).read()
+    mp = open(os.path.join(src, "multipole.hpp")).read()
+    assert float(re.search(r"kFarMinY = ([0-9.]+);", ctx).group(1)) == H.FAR_MIN_Y
+    assert int(re.search(r"kMpTerms = (\d+);", mp).group(1)) == RM.SERIES_MAX_TERMS
