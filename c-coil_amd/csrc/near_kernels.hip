@@ -18,22 +18,36 @@
 
 namespace ccoil {
 
-// Whether a (coil, point) pair may be a record: false only where the point lies beyond the coil's far-field sphere by
-// more than a margin, tested in the global frame (the rotation to the coil's frame keeps lengths up to a few ulps,
-// far below the margin). The exact class is record_kind's; this lets a wave skip the coil's frame transform when none
-// of its points can be a record of that coil. The coil's centre and far_d2 are read through the constant address
-// space: the records do not change during a launch, so these are scalar loads at the wave-uniform index although the
-// loops that call this also store to global memory (through a generic pointer their loads would be vector loads, each
-// a full memory round trip before the test).
-__device__ inline bool maybe_record(const CoilDev *coils, int ci, double px, double py, double pz)
+// Whether the points of a wave may be records of the coils c0 .. c0 + 7 (bit j for coil c0 + j < n_coils): false
+// only where every point lies beyond the coil's far-field sphere by more than a margin, tested in the global frame
+// (the rotation to the coil's frame keeps lengths up to a few ulps, far below the margin). The exact class is
+// record_kind's; this lets a wave skip the frame transform of the coils none of its points can be a record of. The
+// coils' centres and far_d2 are read through the constant address space: the records do not change during a
+// launch, so these are scalar loads at wave-uniform indices although the loops that call this also store to global
+// memory (through a generic pointer their loads would be vector loads, each a full memory round trip). The eight
+// coils' loads come first, so a wave waits for them once.
+__device__ inline uint32_t record_candidates(const CoilDev *coils, int n_coils, int c0, double px, double py,
+                                             double pz)
 {
     typedef const __attribute__((address_space(4))) double cdouble;
-    cdouble *f = (cdouble *)(const void *)(coils + ci);
-    const double far_d2 = f[offsetof(CoilDev, far_d2) / sizeof(double)];
-    if (!(far_d2 > 0.0)) return true;
-    const int o = offsetof(CoilDev, pos) / sizeof(double);
-    const double dx = px - f[o], dy = py - f[o + 1], dz = pz - f[o + 2];
-    return dx * dx + dy * dy + dz * dz <= far_d2 * (1.0 + 1e-9);
+    constexpr int kPos = offsetof(CoilDev, pos) / sizeof(double), kFar = offsetof(CoilDev, far_d2) / sizeof(double);
+    double fd[8], cx[8], cy[8], cz[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {   // past the last coil, the last coil again (its bit is cleared below)
+        cdouble *f = (cdouble *)(const void *)(coils + min(c0 + j, n_coils - 1));
+        fd[j] = f[kFar];
+        cx[j] = f[kPos];
+        cy[j] = f[kPos + 1];
+        cz[j] = f[kPos + 2];
+    }
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const double dx = px - cx[j], dy = py - cy[j], dz = pz - cz[j];
+        const bool may = !(fd[j] > 0.0) | (dx * dx + dy * dy + dz * dz <= fd[j] * (1.0 + 1e-9));
+        m |= (__ballot(may) != 0 && c0 + j < n_coils) ? 1u << j : 0u;
+    }
+    return m;
 }
 
 // Calls with fewer points than kNearSplitPoints split each near record's thickness rows over kNearSplit lanes.
@@ -50,18 +64,19 @@ __global__ __launch_bounds__(256) void near_count_kernel(FieldParams p)
     if (p.near_counts[pi] == 0) return;
     const double px = p.pts[3 * pi], py = p.pts[3 * pi + 1], pz = p.pts[3 * pi + 2];
     const int lane = threadIdx.x & 63;
-    for (int ci = 0; ci < p.n_coils; ++ci) {
-        if (!__ballot(maybe_record(p.coils, ci, px, py, pz))) continue;   // no point of the wave can be a record
-        const CoilDev &c = p.coils[ci];
-        double z, r, cph, sph;
-        to_local(c, px, py, pz, z, r, cph, sph);
-        const int kind = record_kind(p, c, z, r, p.q & (Q_G | Q_GJ));   // the field kernel's test, bitwise
-        for (int k = 0; k <= p.float_records; ++k) {
-            const uint64_t m = __ballot(kind == k);
-            if (m && lane == __builtin_ctzll(m))
-                atomicAdd(&p.near_hist[k * p.n_coils + c.near_key], (unsigned)__popcll(m));
+    // coils in ascending order, skipping those no point of the wave can be a record of
+    for (int c0 = 0; c0 < p.n_coils; c0 += 8)
+        for (uint32_t cand = record_candidates(p.coils, p.n_coils, c0, px, py, pz); cand; cand &= cand - 1) {
+            const CoilDev &c = p.coils[c0 + __builtin_ctz(cand)];
+            double z, r, cph, sph;
+            to_local(c, px, py, pz, z, r, cph, sph);
+            const int kind = record_kind(p, c, z, r, p.q & (Q_G | Q_GJ));   // the field kernel's test, bitwise
+            for (int k = 0; k <= p.float_records; ++k) {
+                const uint64_t m = __ballot(kind == k);
+                if (m && lane == __builtin_ctzll(m))
+                    atomicAdd(&p.near_hist[k * p.n_coils + c.near_key], (unsigned)__popcll(m));
+            }
         }
-    }
 }
 
 // One workgroup: cursor = exclusive prefix sum of near_hist over the n_coils + 1 keys, and the call's total.
@@ -108,31 +123,35 @@ __global__ __launch_bounds__(256) void near_fill_kernel(FieldParams p)
     const uint64_t hi = (uint64_t)pi << 32;
     const int lane = threadIdx.x & 63;
     bool bad = false;
-    for (int ci = 0; ci < p.n_coils; ++ci) {
-        if (!__ballot(maybe_record(p.coils, ci, px, py, pz))) continue;   // no point of the wave can be a record
-        const CoilDev &c = p.coils[ci];
-        double z, r, cph, sph;
-        to_local(c, px, py, pz, z, r, cph, sph);
-        const int kind = record_kind(p, c, z, r, p.q & (Q_G | Q_GJ));   // the field kernel's test, bitwise
-        // the wave's records of this coil and kind take consecutive places from one atomic on the key's cursor
-        for (int k = 0; k <= p.float_records; ++k) {
-            const bool mine = kind == k;
-            const uint64_t m = __ballot(mine);
-            if (!m) continue;
-            const int leader = __builtin_ctzll(m);
-            const uint32_t key = (uint32_t)(k * p.n_coils + c.near_key);
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&p.near_cursor[key], (unsigned)__popcll(m));
-            base = __shfl(base, leader);
-            if (!mine) continue;
-            const uint64_t pos = base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
-            if (slot >= end || pos >= total) {   // classified differently from field_kernel: report, write nothing
-                bad = true;
-                continue;
+    // coils in ascending order (a point's slots follow its coils), skipping those no point of the wave can be a
+    // record of
+    for (int c0 = 0; c0 < p.n_coils; c0 += 8) {
+        for (uint32_t cand = record_candidates(p.coils, p.n_coils, c0, px, py, pz); cand; cand &= cand - 1) {
+            const CoilDev &c = p.coils[c0 + __builtin_ctz(cand)];
+            double z, r, cph, sph;
+            to_local(c, px, py, pz, z, r, cph, sph);
+            const int kind = record_kind(p, c, z, r, p.q & (Q_G | Q_GJ));   // the field kernel's test, bitwise
+            // the wave's records of this coil and kind take consecutive places from one atomic on the key's cursor
+            for (int k = 0; k <= p.float_records; ++k) {
+                const bool mine = kind == k;
+                const uint64_t m = __ballot(mine);
+                if (!m) continue;
+                const int leader = __builtin_ctzll(m);
+                const uint32_t key = (uint32_t)(k * p.n_coils + c.near_key);
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(&p.near_cursor[key], (unsigned)__popcll(m));
+                base = __shfl(base, leader);
+                if (!mine) continue;
+                const uint64_t pos = base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
+                // a pair classified differently from the series or field kernel: report it, write nothing
+                if (slot >= end || pos >= total) {
+                    bad = true;
+                    continue;
+                }
+                p.near_keys[pos] = key;
+                p.near_recs[pos] = hi | (uint64_t)slot;
+                ++slot;
             }
-            p.near_keys[pos] = key;
-            p.near_recs[pos] = hi | (uint64_t)slot;
-            ++slot;
         }
     }
     if (bad || slot != end) atomicOr(&p.near_state->mismatch, 1u);
