@@ -53,7 +53,7 @@ FieldShape field_shape_rule(ccoil_ctx ctx, int64_t n, int64_t n_coils, uint32_t 
 // ylo (far_distance below).
 //
 // The series sums the rule's own rings exactly, so it departs from the rule by the angular rule's error and its
-// truncation (below 1e-16 from D = 2 r_max on). The angular integrand over phi in [0, pi] (Gauss-Legendre on
+// truncation (below 1e-15 from D = 1.17 r_max on). The angular integrand over phi in [0, pi] (Gauss-Legendre on
 // phi = pi (1 + x) / 2, n_ang nodes) is singular where the distance from the point to the ring vanishes, at
 // phi = +-i eta with cosh(eta) = 1 + d^2 / (2 r rho), d the point's distance from the ring in the meridian plane;
 // for every point at distance D from the centre and every ring of the winding, d >= D - r_max and r rho <= D (R + a),

@@ -40,8 +40,9 @@ static_assert(2 * kMpLenNodes - 1 >= kMpTerms - 1, "the length rule integrates V
 
 // Number of terms for a point at distance D from a coil whose winding lies within r_max: the truncation falls like
 // (r_max / D)^L with a constant below 2, so L = ln(2 / tol) / ln(D / r_max), between 4 and kMpTerms; log_tol =
-// ln(2 / tol): kMpLogTolFP64 (tol = 1e-16) or kMpLogTolFP32C (1e-13, far below the float rule's own rounding).
-constexpr float kMpLogTolFP64 = 37.5348f;
+// ln(2 / tol): kMpLogTolFP64 (tol = 1e-15, a hundredth of the FP64 rule's own error against the exact integral) or
+// kMpLogTolFP32C (1e-13, far below the float rule's own rounding).
+constexpr float kMpLogTolFP64 = 35.2319f;
 constexpr float kMpLogTolFP32C = 30.6267f;
 __device__ inline int multipole_terms(double D2, double inv_rmax2, float log_tol)
 {

@@ -60,7 +60,7 @@ def field_work(rows, n_points, with_gradient=True):
 SERIES_B = (2, 20, 13)    # special ops and flops per pair, flops per term
 SERIES_BG = (2, 30, 18)
 SERIES_MAX_TERMS = 250    # kMpTerms
-LOG_TOL_FP64 = 37.5348    # ln(2 / 1e-16)
+LOG_TOL_FP64 = 35.2319    # ln(2 / 1e-15)
 LOG_TOL_FP32C = 30.6267   # ln(2 / 1e-13)
 
 

@@ -68,7 +68,7 @@ def far_threshold(n_ang, q, tol=1e-11, ylo=2.0):
 
 
 FAR_MIN_Y = 1.17
-LOG_TOL = {"fp64": float(np.float32(37.5348)), "fp32c": float(np.float32(30.6267))}   # ln(2 / tol) of the terms
+LOG_TOL = {"fp64": float(np.float32(35.2319)), "fp32c": float(np.float32(30.6267))}   # ln(2 / tol) of the terms
 
 
 def far_cost_floor(n_ang, n_thk, fast, log_tol):

@@ -36,17 +36,17 @@ def test_library_threshold_matches_the_bound(prec):
 
 
 def test_cost_floor_pinned_by_hand():
-    # FP64 terms: log_tol = 37.5348 (float), L(y) = 2 log_tol / ln(y^2) + 2 at 15 per term + 200.
-    # RECTANGULAR 80 x 20 nodes at 55: rule 88,000, y = exp(15 x 37.5348 / 87,770) = 1.0064: clamped to 1.17.
+    # FP64 terms: log_tol = 35.2319 (float), L(y) = 2 log_tol / ln(y^2) + 2 at 15 per term + 200.
+    # RECTANGULAR 80 x 20 nodes at 55: rule 88,000, y = exp(15 x 35.2319 / 87,770) = 1.0060: clamped to 1.17.
     assert H.far_cost_floor(80, 20, True, H.LOG_TOL["fp64"]) == 1.17
-    # FILAMENT 80 x 1 at 12: rule 960, y = exp(563.02 / 730) = exp(0.77126) = 2.162: clamped to 2.
+    # FILAMENT 80 x 1 at 12: rule 960, y = exp(528.48 / 730) = exp(0.72394) = 2.0626: clamped to 2.
     assert H.far_cost_floor(80, 1, False, H.LOG_TOL["fp64"]) == 2.0
-    # slow 40 x 3: rule 1,440, y = exp(563.02 / 1,210) = exp(0.465307) = 1.59251
-    assert H.far_cost_floor(40, 3, False, H.LOG_TOL["fp64"]) == pytest.approx(1.59251, abs=2e-5)
+    # slow 40 x 3: rule 1,440, y = exp(528.48 / 1,210) = exp(0.436759) = 1.54768
+    assert H.far_cost_floor(40, 3, False, H.LOG_TOL["fp64"]) == pytest.approx(1.54768, abs=2e-5)
     # slow 12 x 1: rule 144 <= 230: 2
     assert H.far_cost_floor(12, 1, False, H.LOG_TOL["fp64"]) == 2.0
-    # at the floor the FP64 series needs int(2 x 37.5348 / ln 1.3689) + 2 = 241 terms, within the 250 it carries
-    assert int(2 * H.LOG_TOL["fp64"] / np.log(1.17 ** 2)) + 2 == 241
+    # at the floor the FP64 series needs int(2 x 35.2319 / ln 1.3689) + 2 = 226 terms, within the 250 it carries
+    assert int(2 * H.LOG_TOL["fp64"] / np.log(1.17 ** 2)) + 2 == 226
 
 
 def test_other_precisions_never_use_the_series():
