@@ -89,20 +89,9 @@ __device__ inline void mat_vec(const double *M, const double *v, double *o)
     o[2] = M[6] * v[0] + M[7] * v[1] + M[8] * v[2];
 }
 
-// 1 / x for a finite positive normal x: the hardware estimate (v_rcp_f64, about 2^-23 relative) refined by two
-// Newton steps on f(y) = 1 / y - x, y <- y + y (1 - x y), each of which squares the relative error; within an ulp or
-// two of the quotient and a fixed sequence of operations, so the same in every kernel.
-__device__ inline double rcp_newton(double x)
-{
-    double y = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, y, 1.0);
-    y = fma(y, e, y);
-    e = fma(-x, y, 1.0);
-    return fma(y, e, y);
-}
-
-// 1 / sqrt(x) for a finite positive normal x: v_rsq_f64's estimate refined by two Newton steps on f(y) = 1 / y^2 - x,
-// y <- y + (y / 2) (1 - x y^2).
+// 1 / sqrt(x) for a finite positive normal x: v_rsq_f64's estimate (about 2^-23 relative) refined by two Newton steps
+// on f(y) = 1 / y^2 - x, y <- y + (y / 2) (1 - x y^2), each of which squares the relative error; within an ulp or two
+// of the exact value and a fixed sequence of operations, so the same in every kernel.
 __device__ inline double rsqrt_newton(double x)
 {
     double y = __builtin_amdgcn_rsq(x);
@@ -126,12 +115,19 @@ __device__ inline void to_local(const CoilDev &c, double px, double py, double p
     const double l[3] = {M[0] * d[0] + M[1] * d[1] + M[2] * d[2], M[3] * d[0] + M[4] * d[1] + M[5] * d[2],
                          M[6] * d[0] + M[7] * d[1] + M[8] * d[2]};
     z = l[2];
-    r = sqrt(l[0] * l[0] + l[1] * l[1]);
-    if (r > 0.0) {
-        const double ir = rcp_newton(r);
+    const double r2 = l[0] * l[0] + l[1] * l[1];
+    if (r2 >= 1e-200) {
+        // r and 1 / r from one reciprocal square root (within about an ulp of the correctly rounded values)
+        const double ir = rsqrt_newton(r2);
+        r = r2 * ir;
         cph = l[0] * ir;
         sph = l[1] * ir;
+    } else if (r2 > 0.0) {   // within 1e-100 of the axis: the IEEE operations, clear of the estimates' range limits
+        r = sqrt(r2);
+        cph = l[0] / r;
+        sph = l[1] / r;
     } else {
+        r = 0.0;
         cph = 1.0;
         sph = 0.0;
     }
