@@ -530,6 +530,17 @@ def main():
                                   "dM_vs_F_over_II": {"all": stats(e), "separated_1": stats(e[sep > 1.0]),
                                                       "separated_2": stats(e[sep > 2.0])}}
         log(f"large M matrix: {mdt * 1e3:.0f} ms, dM vs F/(II) {json.dumps(large['m_matrix_fp64']['dM_vs_F_over_II'])}")
+        # the same matrix with the primaries' far-field series, against the rule at every node
+        t0 = time.perf_counter()
+        so, srep = ctx.pairs(prim, sec, len(pi), abi.PAIR_M | abi.PAIR_FT | abi.PAIR_DM, abi.FP64, abi.FAR_FIELD,
+                             prim_index=pi, sec_index=si)
+        sdt = time.perf_counter() - t0
+        rel = lambda k: (np.linalg.norm((so[k] - mo[k]).reshape(len(pi), -1), axis=1)
+                         / np.linalg.norm(mo[k].reshape(len(pi), -1), axis=1))
+        large["m_matrix_fp64_far_field"] = {"pairs": int(len(pi)), "ms": sdt * 1e3, "kernel_ms": srep.main_kernel_ms,
+                                            "pairs_per_s": len(pi) / sdt,
+                                            "vs_rule": {k: stats(rel(k)) for k in ("M", "F", "dM")}}
+        log(f"large M matrix with the series: {sdt * 1e3:.0f} ms")
         result["large"] = large
 
     if not args.no_cpu:
