@@ -48,14 +48,20 @@ __device__ inline void ring_moments(double w, double z0, double c2, double *acc)
 }
 
 // The moments of orders [L0, L0 + 64) of coil c into out (a block of kMomThreads threads over the coil's rings).
+// The length rule is symmetric about the mid-plane and V_l(rho, -z0) = (-1)^(l-1) V_l(rho, z0) (every operation of
+// the recurrence flips sign exactly with z0), so a node pair adds 2 V_l for odd l and exactly nothing for even l:
+// the rings take the nodes z0 > 0 at twice their weight, and the even orders of a coil with a length rule are zero.
+// (A slow-rule coil's mid-plane ring has V_l = 0 for even l by itself.)
 template <int L0>
 __device__ inline void chunk_moments(const CoilDev &c, const Tables &tab, double *out, double (*part)[64])
 {
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const double inv = 1.0 / c.rmax;
-    const int nt = c.n_thk, nz = c.use_fast ? kMpLenNodes : 1;
+    static_assert(kMpLenNodes % 2 == 0, "the length rule's nodes come in pairs +-z0");
+    const int nt = c.n_thk, nz = c.use_fast ? kMpLenNodes / 2 : 1;
     const double *xt = tab.x + table_offset(nt), *wt = tab.w + table_offset(nt);
-    const double *xz = tab.x + kSeriesLenOffset, *wz = tab.w + kSeriesLenOffset;
+    // the upper half of the ascending length nodes: z0 > 0
+    const double *xz = tab.x + kSeriesLenOffset + kMpLenNodes / 2, *wz = tab.w + kSeriesLenOffset + kMpLenNodes / 2;
     double acc[64];
 #pragma unroll
     for (int l = 0; l < 64; ++l) acc[l] = 0.0;
@@ -64,7 +70,7 @@ __device__ inline void chunk_moments(const CoilDev &c, const Tables &tab, double
         // ring (rho, z0) in units of r_max, weight: its share of the current (the weights sum to 1)
         const double rho = (c.R + 0.5 * c.a * (1.0 + xt[j])) * inv;
         const double z0 = nz > 1 ? 0.5 * c.b * xz[i] * inv : 0.0;
-        const double w = 0.5 * wt[j] * (nz > 1 ? 0.5 * wz[i] : 1.0) * rho * rho;
+        const double w = 0.5 * wt[j] * (nz > 1 ? wz[i] : 1.0) * rho * rho;   // 2 x (1/2) wz for the node pair
         ring_moments<L0>(w, z0, rho * rho + z0 * z0, acc);
     }
 #pragma unroll
@@ -78,7 +84,7 @@ __device__ inline void chunk_moments(const CoilDev &c, const Tables &tab, double
     if (t < 64 && L0 + t < kMpStride) {
         const int l = L0 + t;
         double m = 0.0;
-        if (l >= 1 && l <= kMpTerms) {
+        if (l >= 1 && l <= kMpTerms && (l % 2 == 1 || !c.use_fast)) {
             const double s = ((part[0][t] + part[1][t]) + (part[2][t] + part[3][t])) +
                              ((part[4][t] + part[5][t]) + (part[6][t] + part[7][t]));
             m = ni * s / (2.0 * (l + 1));
