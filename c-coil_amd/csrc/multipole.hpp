@@ -37,6 +37,10 @@ constexpr int kMpLenNodes = kSeriesLenNodes; // Gauss-Legendre nodes over the le
                                              // degree 251, and V_l has degree l - 1 in the ring's height
 static_assert(kMpTerms < 256 && kMpTerms < kMpStride && kMpStride <= 256, "series orders and their arrays");
 static_assert(2 * kMpLenNodes - 1 >= kMpTerms - 1, "the length rule integrates V_l exactly");
+// Every coil's winding is symmetric about its mid-plane (the length rule's nodes come in pairs +-z0, a slow-rule
+// coil is a mid-plane sheet), so its moments of even order are zero: multipole_moments_kernel writes them as exact
+// zeros, and the series loops add only the odd orders.
+constexpr bool kMpEvenZero = true;
 
 // Number of terms for a point at distance D from a coil whose winding lies within r_max: the truncation falls like
 // (r_max / D)^L with a constant below 2, so L = ln(2 / tol) / ln(D / r_max), between 4 and kMpTerms; log_tol =
@@ -70,7 +74,7 @@ __device__ inline int wave_min_small(int v, bool active)
 // Local (A_phi, B_rho, B_z, B_rho / r, dB_rho/drho, dB_rho/dz, dB_z/dz) of one coil at local (z, r) from its series
 // (mp: the coil's kMpCoil doubles, rec: the shared recurrence coefficients, rmax: the coil's r_max): the loop runs L
 // terms (wave-uniform), and terms beyond own add exact zeros, so the value is that of own terms. The first lo terms
-// (the least own count among the lanes that call; wave-uniform) need no such test, and run unrolled by two.
+// (the least own count among the lanes that call; wave-uniform) need no such test.
 template <bool WA, bool WB, bool WG>
 __device__ inline void eval_multipole(const double *mp, const double *rec, double rmax, double z, double r, int L,
                                       int lo, int own, Local &out)
@@ -84,7 +88,7 @@ __device__ inline void eval_multipole(const double *mp, const double *rec, doubl
     double d0 = 1.0, d1 = 3.0 * c, d2 = 1.5 * (5.0 * c * c - 1.0);
     double q = qr;   // q_l = (r_max / D)^(l + 1)
     double SA = 0.0, SZ = 0.0, SB = 0.0, SZZ = 0.0, SRZ = 0.0;
-    auto term = [&](int l, double ql) {
+    auto accumulate = [&](int l, double ql) {
         if (WA) SA = fma(mA[l] * ql, d0, SA);
         if (WB || WG) {
             SZ = fma(mZ[l] * ql, P1, SZ);
@@ -94,6 +98,8 @@ __device__ inline void eval_multipole(const double *mp, const double *rec, doubl
             SZZ = fma(mZZ[l] * ql, P2, SZZ);
             SRZ = fma(mZ[l] * ql, d2, SRZ);
         }
+    };
+    auto step = [&](int l) {
         // P_(l+3) = ((2l + 5) c P_(l+2) - (l + 2) P_(l+1)) / (l + 3), P'_(l+3) = P'_(l+1) + (2l + 5) P_(l+2)
         const double P3 = fma(rec[l] * c, P2, -rec[kMpStride + l] * P1);
         const double d3 = fma(rec[2 * kMpStride + l], P2, d1);
@@ -103,16 +109,24 @@ __device__ inline void eval_multipole(const double *mp, const double *rec, doubl
         d1 = d2;
         d2 = d3;
     };
+    // The moments of even order are exactly zero (kMpEvenZero), so only odd l accumulate while the recurrences and
+    // the powers q_l step through every l: the same operations on the same values as summing every term.
     int l = 1;
-#pragma unroll 4
-    for (; l <= lo; ++l) {
+#pragma unroll 2
+    for (; l <= lo; l += 2) {
         q *= qr;
-        term(l, q);
+        accumulate(l, q);
+        step(l);
+        q *= qr;
+        step(l + 1);
     }
 #pragma unroll 2
-    for (; l <= L; ++l) {
+    for (; l <= L; l += 2) {
         q *= qr;
-        term(l, l <= own ? q : 0.0);
+        accumulate(l, l <= own ? q : 0.0);
+        step(l);
+        q *= qr;
+        step(l + 1);
     }
     out.A = s * SA;
     out.Bz = u * SZ;
@@ -138,12 +152,14 @@ __device__ inline void eval_multipole_pair(const double *mp, const double *rec, 
     double d0 = 1.0, d1 = 3.0 * c, d2 = 1.5 * (5.0 * c * c - 1.0);
     double q = qr;
     double SA = 0.0, SZ = 0.0, SB = 0.0;
-    auto term = [&](int l, double ql) {
+    auto accumulate = [&](int l, double ql) {
         if (WA) SA = fma(mA[l] * ql, d0, SA);
         if (WB) {
             SZ = fma(mZ[l] * ql, P1, SZ);
             SB = fma(mB[l] * ql, d1, SB);
         }
+    };
+    auto step = [&](int l) {
         const double P3 = fma(rec[l] * c, P2, -rec[kMpStride + l] * P1);
         const double d3 = fma(rec[2 * kMpStride + l], P2, d1);
         P1 = P2;
@@ -152,16 +168,22 @@ __device__ inline void eval_multipole_pair(const double *mp, const double *rec, 
         d1 = d2;
         d2 = d3;
     };
-    int l = 1;
+    int l = 1;   // odd l only, as in eval_multipole
 #pragma unroll 2
-    for (; l <= lo; ++l) {
+    for (; l <= lo; l += 2) {
         q *= qr;
-        term(l, q);
+        accumulate(l, q);
+        step(l);
+        q *= qr;
+        step(l + 1);
     }
 #pragma unroll 2
-    for (; l <= L; ++l) {
+    for (; l <= L; l += 2) {
         q *= qr;
-        term(l, l <= own ? q : 0.0);
+        accumulate(l, l <= own ? q : 0.0);
+        step(l);
+        q *= qr;
+        step(l + 1);
     }
     out.A = s * SA;
     out.Bz = u * SZ;

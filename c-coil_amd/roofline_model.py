@@ -22,8 +22,10 @@ this; the extra instructions (and any compensation) are not algorithmic work and
 Pairs: one node-pair of M costs one A evaluation; one node-pair of F/T costs one B evaluation.
 
 The far-field series (a field call's (coil, point) pairs beyond the coil's far-field distance, multipole.hpp) is
-counted per pair from its FP64 loop: per term 13 flops for B (the next power of r_max / D, two sums with their
-coefficient products, the Legendre recurrence of P and P') and 18 with the gradient (two more sums), and per pair
+counted per pair from its FP64 loop: per odd term 13 flops for B (the next power of r_max / D, two sums with their
+coefficient products, the Legendre recurrence of P and P') and 18 with the gradient (two more sums); per even term
+7 (the power and the recurrences: a coil symmetric about its mid-plane, as every coil here is, has no even moments,
+so those terms have no sums), and per pair
 20 (B) or 30 (B + gradient) flops and 2 special ops (1 / sqrt(D^2), the log of the term count) for the set-up and
 the outputs. The term count per pair is the kernel's: int(2 ln(2 / tol) / ln(D^2 / r_max^2)) + 2 in [4, SERIES_MAX_TERMS]
 (multipole.hpp's kMpTerms).
@@ -57,8 +59,9 @@ def field_work(rows, n_points, with_gradient=True):
     return float(flops) * n_points, float(trans) * n_points
 
 
-SERIES_B = (2, 20, 13)    # special ops and flops per pair, flops per term
+SERIES_B = (2, 20, 13)    # special ops and flops per pair, flops per odd term
 SERIES_BG = (2, 30, 18)
+SERIES_EVEN_TERM = 7      # flops per even term (no sums: the even moments are zero)
 SERIES_MAX_TERMS = 250    # kMpTerms
 LOG_TOL_FP64 = 35.2319    # ln(2 / 1e-15)
 LOG_TOL_FP32C = 30.6267   # ln(2 / 1e-13)
@@ -97,7 +100,8 @@ def series_field_work(rows, pts, dist, rmax, log_tol, with_gradient=True, near_p
         out["rule_special"] += float(nr) * nodes * t
         if far.any():
             L = np.clip((2.0 * log_tol / np.log(D2[far] / (rm * rm))).astype(np.int64) + 2, 4, SERIES_MAX_TERMS)
-            out["series_flops"] += float(far.sum()) * sf + float(L.sum()) * st
+            odd, even = (L + 1) // 2, L // 2
+            out["series_flops"] += float(far.sum()) * sf + float(odd.sum()) * st + float(even.sum()) * SERIES_EVEN_TERM
             out["series_special"] += float(far.sum()) * sp
         out["rule_pairs"] += nr
         out["series_pairs"] += int(far.sum())
